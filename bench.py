@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark of the CVF bulk-water MC hot path on one MI355X.
+
+Headline workload (the metric's "batched mixed-prec" form): 128 independent
+(T, P) state points in one launch sequence, T = 0.55 + 0.45 k/7 (k = 0..7),
+P = -0.10 + 0.06 j (j = 0..15), L = 16, 24, 32, 48 for point i % 4 (32 points
+each, 5,160,960 cells), seed_i = 44648604 + 1000003 i, random start, 1
+Metropolis sweep (12 half-lattice sigma sweeps) + 1 eta cube sweep + 1
+volume move per MC step, FP32-compensated vdW with FP64 acceptance, the
+step captured in a hipGraph.  A step = one MC step of every state point.
+
+Beside it, the plain drop-in form is timed: one state point, L = 32,
+T = 0.815, P = 0.45, FP64 vdW, host volume move (the reference's
+production shape).  value = site-updates/s of the batched form
+(site-update = one molecule advanced one full MC step).
+
+The roofline is for the dominant kernel (k_sigma, the half-lattice sigma
+sweep): algorithmic bytes per launch = (cells/2) x 12 B (4 B own word read,
+4 B neighbour word read, 4 B write), divided by the mean launch duration
+measured with HIP events on the engine's stream.
+
+Inputs are synthetic (random initial lattices); nothing is read from disk.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+SIGMA_BYTES_PER_ACTIVE_CELL = 12
+
+
+def batch_params(fp32: bool = True):
+    from cvfbulkwater_amd import FP32_COMPENSATED, FP64, VOLUME_DEVICE, Params
+
+    pts = []
+    for i in range(128):
+        k, j = i % 8, i // 8
+        pts.append(Params(L=(16, 24, 32, 48)[i % 4], T=0.55 + 0.45 * k / 7, P=-0.10 + 0.06 * j,
+                          seed=44648604 + 1000003 * i, n_metropolis=1, n_cluster=0,
+                          precision=FP32_COMPENSATED if fp32 else FP64, volume_mode=VOLUME_DEVICE))
+    return pts
+
+
+class HipEvents:
+    """hipEventRecord on an explicit stream (the engine's own)."""
+
+    def __init__(self):
+        self.lib = C.CDLL("libamdhip64.so")
+        self.lib.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.lib.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.lib.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.lib.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.lib.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.lib.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.lib.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.lib.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.lib.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def time_steps(sim, steps: int, warmup: int, ev: HipEvents) -> float:
+    """ms per MC step over exactly `steps` steps, HIP events on the engine stream."""
+    sim.step(warmup)
+    sim.sync()
+    a, b = ev.event(), ev.event()
+    ev.record(a, sim.stream)
+    sim.step(steps)
+    ev.record(b, sim.stream)
+    ms = ev.elapsed_ms(a, b)
+    sim.sync()
+    return ms / steps
+
+
+def cpu_baseline(budget_s: float) -> dict:
+    """The CPU oracle (same rules, serial C) on a bounded sample of the batch:
+    one state point of each lattice size, stepped until the budget is used."""
+    from oracle import py as O
+
+    O.build()
+    cells_steps = 0
+    t0 = time.perf_counter()
+    per = budget_s / 4
+    used = []
+    for i, L in enumerate((16, 24, 32, 48)):
+        k, j = i % 8, i // 8
+        sim = O.OracleSim(L=L, T=0.55 + 0.45 * k / 7, P=-0.10 + 0.06 * j, seed=44648604 + 1000003 * i)
+        n, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < per:
+            sim.step(1)
+            n += 1
+        cells_steps += n * L ** 3
+        used.append(f"L={L}x{n}")
+        sim.close()
+    dt = time.perf_counter() - t0
+    return {"value": cells_steps / dt, "unit": "site-updates/s", "cores": 1, "kind": "port",
+            "sample": "CPU oracle, serial C, one state point of each L of the batch (" + ", ".join(used)
+                      + f" MC steps), {dt:.1f} s on {platform.processor() or platform.machine()}"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=2000)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["batch", "plain"], default=None,
+                    help="run one leg only (for profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
+                                   "(state points shard across GPUs with no exchange, "
+                                   "but no multi-GPU run is provided)"}))
+        return 2
+
+    from cvfbulkwater_amd import FP64, VOLUME_HOST, Params, Simulation
+
+    ev = HipEvents()
+    out = {}
+    if args.only in (None, "batch"):
+        pts = batch_params(fp32=True)
+        cells = sum(p.L ** 3 for p in pts)
+        sim = Simulation(pts)
+        ms = time_steps(sim, args.steps, args.warmup, ev)
+        prof = sim.profile_steps(args.profile_steps)
+        sim.close()
+        sig_ms = prof["sigma"][0]
+        sig_bytes = (cells // 2) * SIGMA_BYTES_PER_ACTIVE_CELL
+        achieved = sig_bytes / (sig_ms * 1e-3) / 1e9
+        out.update(batch_ms=ms, batch_cells=cells, batch_rate=cells / (ms * 1e-3), prof=prof,
+                   sig_ms=sig_ms, sig_bytes=sig_bytes, achieved=achieved)
+    if args.only in (None, "plain"):
+        sim = Simulation(Params(L=32, T=0.815, P=0.45, seed=44648604, precision=FP64,
+                                volume_mode=VOLUME_HOST))
+        pms = time_steps(sim, args.plain_steps, args.warmup, ev)
+        sim.close()
+        out.update(plain_ms=pms, plain_rate=32 ** 3 / (pms * 1e-3))
+
+    line = {"metric": "site-updates/s per GPU, batched mixed-prec vs plain; achieved HBM GB/s",
+            "unit": "site-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+            "data": "synthetic"}
+    if "batch_rate" in out:
+        line.update(value=out["batch_rate"], ms_per_step=out["batch_ms"])
+        line["config"] = {"workload": "batched scan: 128 state points, L in {16,24,32,48} x32, "
+                                      "T in [0.55,1.00] x P in [-0.10,0.80], 1 Metropolis + 1 eta "
+                                      "sweep + 1 volume move per step, FP32-compensated vdW, "
+                                      "hipGraph step", "cells": out["batch_cells"],
+                          "parallelism": "one GPU, state points batched"}
+        line["roofline"] = {"bound": "hbm", "kernel": "k_sigma", "achieved": out["achieved"],
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": out["achieved"] / HBM_PEAK_GBS, "traffic": None,
+                            "bytes_per_launch": out["sig_bytes"], "launch_ms": out["sig_ms"]}
+        line["kernel_ms"] = {k: v[0] for k, v in out["prof"].items() if v[1]}
+    if "plain_rate" in out:
+        line["plain"] = {"workload": "plain drop-in form: L=32, T=0.815, P=0.45, FP64 vdW, host "
+                                     "volume move", "value": out["plain_rate"],
+                         "ms_per_step": out["plain_ms"], "steps": args.plain_steps}
+        if "batch_rate" in out:
+            line["batched_over_plain"] = out["batch_rate"] / out["plain_rate"]
+        else:
+            line.update(value=out["plain_rate"], ms_per_step=out["plain_ms"])
+    if not args.no_cpu_baseline and args.only is None:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        line["cpu_baseline"]["cores"] = 1
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,842 @@
+// C ABI of the engine (include/cvf_api.h): handle management, state
+// packing, block tables, hipGraph capture of one MC step, observables and
+// cluster statistics.
+#include "cvf_api.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cvf_core.h"
+#include "cvf_host.h"
+#include "cvf_kernels.h"
+
+using namespace cvf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPC(expr)                                                                  \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(CVF_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                              \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    reset();
+    n = count;
+    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+};
+
+struct PointHost {
+  cvf_params prm;
+  uint32_t L, N, woff, loff, shell_off;
+  Thresholds thr;
+  std::vector<float> dist;
+  std::vector<int32_t> freq;
+  VolState vol;  // host copy (authoritative in host-volume mode)
+  VolParams vp;
+};
+
+}  // namespace
+
+struct cvf_sim {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int npts = 0;
+  int nm = 1, nc = 0, precision = CVF_FP64, vol_device = 0;
+  int nsched = 12;
+  std::vector<PointHost> ph;
+  size_t total_cells = 0, total_nodes = 0;
+  uint32_t shell_cap = 0;
+
+  DevBuf<uint32_t> words, parent, cover, nslot_nperc;
+  DevBuf<uint8_t> cov, sched;
+  DevBuf<int32_t> csize, slot;
+  DevBuf<PointDev> pts;
+  DevBuf<PointState> pst;
+  DevBuf<float> dist;
+  DevBuf<int32_t> freq;
+  DevBuf<unsigned long long> counters;
+  DevBuf<BlockEnt> tab_half, tab_cube, tab_cell;
+  uint32_t nb_half = 0, nb_cube = 0, nb_cell = 0;
+  uint32_t max_slots = 0;
+
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int64_t steps = 0;
+  int64_t obs_at = -1;
+  std::vector<unsigned long long> obs_counters;
+  bool sw_done = false;
+
+  ~cvf_sim() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Block table over `count(p)` work items per point, padded to a multiple of
+// the XCD count.
+template <class F>
+std::vector<BlockEnt> make_table(const std::vector<PointHost>& ph, F count) {
+  std::vector<BlockEnt> t;
+  for (size_t p = 0; p < ph.size(); ++p) {
+    const uint32_t n = count(ph[p]);
+    for (uint32_t s = 0; s < n; s += kBlock) t.push_back(BlockEnt{(int32_t)p, s});
+  }
+  while (t.size() % kNumXcd) t.push_back(BlockEnt{-1, 0});
+  return t;
+}
+
+int validate(const cvf_params& p) {
+  if (p.L < 4 || p.L % 4 != 0 || p.L > kMaxL)
+    return fail(CVF_EINVAL, "L=%d: the chessboard needs a cubic lattice with L %% 4 == 0 (4..%d)",
+                p.L, kMaxL);
+  if (!(p.T > 0.f)) return fail(CVF_EINVAL, "T must be > 0");
+  if (p.n_metropolis < 0 || p.n_cluster < 0) return fail(CVF_EINVAL, "negative step counts");
+  if (p.n_metropolis * 12 > 255 * 12) return fail(CVF_EINVAL, "n_metropolis too large");
+  if (p.flag_chessboard != 1)
+    return fail(CVF_EUNSUPPORTED, "Flag_chessboard=%d is not supported (only 1)", p.flag_chessboard);
+  if (p.flag_config < 0 || p.flag_config > 2)
+    return fail(CVF_EINVAL, "Error: Wrong value for flag_config = %d", p.flag_config);
+  if (p.precision != CVF_FP64 && p.precision != CVF_FP32_COMPENSATED)
+    return fail(CVF_EINVAL, "unknown precision %d", p.precision);
+  if (p.precision == CVF_FP32_COMPENSATED && p.volume_mode != CVF_VOLUME_DEVICE)
+    return fail(CVF_EINVAL, "the FP32-compensated vdW form runs the volume move on the device");
+  if (!(p.r_cutoff > 0.f)) return fail(CVF_EINVAL, "R_cutoff must be > 0");
+  return CVF_OK;
+}
+
+// ---------------------------------------------------------- packing
+inline uint32_t pack_cell(const uint8_t* s6, const uint8_t* e6) {
+  uint32_t w = 0;
+  for (int a = 0; a < 6; ++a) w |= (uint32_t)s6[a] << (3 * a);
+  for (int a = 0; a < 6; ++a) w |= (uint32_t)(e6[a] & 1) << (kEtaShift + a);
+  return w;
+}
+
+// corner cell of the cube with origin (ox,oy,oz) at local corner b
+inline uint32_t corner_cell(uint32_t L, uint32_t ox, uint32_t oy, uint32_t oz, int b) {
+  const uint32_t x = (ox + (b & 1)) % L, y = (oy + ((b >> 1) & 1)) % L, z = (oz + ((b >> 2) & 1)) % L;
+  return x + L * (y + L * z);
+}
+
+// rewrite the eta bits of one cube from a solution mask
+void set_cube_eta(std::vector<uint32_t>& w, uint32_t L, uint32_t ox, uint32_t oy, uint32_t oz,
+                  uint32_t M) {
+  for (int b = 0; b < 8; ++b) {
+    const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
+    const uint32_t c = corner_cell(L, ox, oy, oz, b);
+    const uint32_t own = (1u << bx) | (1u << (2 + by)) | (1u << (4 + bz));
+    const uint32_t bits = (((M >> corner_edge(b, 0)) & 1u) << bx) |
+                          (((M >> corner_edge(b, 1)) & 1u) << (2 + by)) |
+                          (((M >> corner_edge(b, 2)) & 1u) << (4 + bz));
+    w[c] = (w[c] & ~(own << kEtaShift)) | (bits << kEtaShift);
+  }
+}
+
+// gpu_Chessboard_set_state decision tree (gpu_water3D.cu:3350-3368) on the
+// packed words: v1=b0, v2=b1, v3=b2, v6=b6, v7=b5, v8=b3.
+uint32_t cube_state_from_words(const std::vector<uint32_t>& w, uint32_t L, uint32_t ox, uint32_t oy,
+                               uint32_t oz) {
+  auto eta = [&](int b, int arm) { return eta_of(w[corner_cell(L, ox, oy, oz, b)], arm); };
+  if (eta(0, 0) == 0) {
+    if (eta(6, 0) == 0) return 2;
+    if (eta(2, 4) == 0) return 3;
+    return 8;
+  } else if (eta(0, 2) == 0) {
+    if (eta(1, 4) == 0) return 4;
+    if (eta(5, 2) == 0) return 5;
+    return 7;
+  }
+  if (eta(2, 0) == 0) return 0;
+  if (eta(3, 4) == 0) return 1;
+  return 6;
+}
+
+// 12-bit active-edge mask of a cube as stored in the words
+uint32_t cube_mask_from_words(const std::vector<uint32_t>& w, uint32_t L, uint32_t ox, uint32_t oy,
+                              uint32_t oz, bool* consistent) {
+  uint32_t m = 0;
+  bool ok = true;
+  for (int j = 0; j < kEdges; ++j) {
+    const int A = edge_start(j), ax = edge_axis(j), B = A | (1 << ax);
+    const uint32_t ea = eta_of(w[corner_cell(L, ox, oy, oz, A)], 2 * ax);
+    const uint32_t eb = eta_of(w[corner_cell(L, ox, oy, oz, B)], 2 * ax + 1);
+    if (ea != eb) ok = false;
+    m |= ea << j;
+  }
+  *consistent = ok;
+  return m;
+}
+
+template <class F>
+void for_each_cube(uint32_t L, F f) {
+  const uint32_t hL = L / 2;
+  for (uint32_t fam = 0; fam < 2; ++fam)
+    for (uint32_t k = 0; k < hL; ++k)
+      for (uint32_t j = 0; j < hL; ++j)
+        for (uint32_t i = 0; i < hL; ++i) {
+          const uint32_t oz = 2 * k + fam;
+          f(2 * i + fam, 2 * j + fam, oz, i + hL * j + hL * hL * oz);
+        }
+}
+
+// Initial state of one point (gpu_water3D.cu:639-826 with Philox streams).
+void init_words(const PointHost& P, std::vector<uint32_t>& w) {
+  const uint32_t L = P.L, N = P.N;
+  const uint64_t seed = P.prm.seed;
+  w.assign(N, 0);
+  if (P.prm.flag_config == 2) {
+    const U4 r = draw(seed, kStInitSigma, 0, 0, 2);
+    const uint32_t v = uniform_int(r.x, 6);
+    for (uint32_t c = 0; c < N; ++c) w[c] = v * kFieldLsb;
+  } else {
+    for (uint32_t c = 0; c < N; ++c) {
+      const U4 r0 = draw(seed, kStInitSigma, c, 0, 0), r1 = draw(seed, kStInitSigma, c, 0, 1);
+      const uint32_t rw[6] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y};
+      uint32_t s = 0;
+      for (int a = 0; a < 6; ++a) s |= uniform_int(rw[a], 6) << (3 * a);
+      w[c] = s;
+    }
+  }
+  for_each_cube(L, [&](uint32_t ox, uint32_t oy, uint32_t oz, uint32_t cube) {
+    const U4 r = draw(seed, kStInitCube, cube, 0, 0);
+    set_cube_eta(w, L, ox, oy, oz, sol_mask(uniform_int(r.x, 9)));
+  });
+}
+
+int upload_point_words(cvf_sim* s, int pt, const std::vector<uint32_t>& w) {
+  HIPC(hipMemcpyAsync(s->words.p + s->ph[pt].woff, w.data(), w.size() * 4, hipMemcpyHostToDevice,
+                      s->stream));
+  HIPC(hipStreamSynchronize(s->stream));
+  s->obs_at = -1;
+  return CVF_OK;
+}
+
+int download_point_words(cvf_sim* s, int pt, std::vector<uint32_t>& w) {
+  w.resize(s->ph[pt].N);
+  HIPC(hipStreamSynchronize(s->stream));
+  HIPC(hipMemcpy(w.data(), s->words.p + s->ph[pt].woff, w.size() * 4, hipMemcpyDeviceToHost));
+  return CVF_OK;
+}
+
+PointDev make_point_dev(const PointHost& P, uint32_t sched_off) {
+  PointDev d;
+  memset(&d, 0, sizeof d);
+  d.seed = P.prm.seed;
+  d.L = P.L;
+  d.N = P.N;
+  d.hL = P.L / 2;
+  d.woff = P.woff;
+  d.loff = P.loff;
+  d.shell_off = P.shell_off;
+  d.divL = make_fastdiv(P.L);
+  d.divH = make_fastdiv(P.L / 2);
+  d.jeff_sign = P.thr.Jeff > 0 ? 1 : (P.thr.Jeff < 0 ? -1 : 0);
+  d.sched_off = (int32_t)sched_off;
+  d.vp = P.vp;
+  memcpy(d.thr_sigma, P.thr.sigma, sizeof d.thr_sigma);
+  memcpy(d.thr_eta, P.thr.eta, sizeof d.thr_eta);
+  d.thr_sw[0] = P.thr.sw_hb;
+  d.thr_sw[1] = P.thr.sw_s;
+  return d;
+}
+
+int upload_tables(cvf_sim* s) {
+  std::vector<PointDev> pd;
+  std::vector<float> dist;
+  std::vector<int32_t> freq;
+  for (int p = 0; p < s->npts; ++p) {
+    auto& P = s->ph[p];
+    P.shell_off = (uint32_t)dist.size();
+    P.vp.nshell = (int32_t)P.dist.size();
+    dist.insert(dist.end(), P.dist.begin(), P.dist.end());
+    freq.insert(freq.end(), P.freq.begin(), P.freq.end());
+    pd.push_back(make_point_dev(P, (uint32_t)(p * s->nsched)));
+  }
+  if (s->dist.p == nullptr || s->dist.n < dist.size()) {
+    // a captured step graph holds the old table pointers
+    if (s->gexec) {
+      (void)hipGraphExecDestroy(s->gexec);
+      s->gexec = nullptr;
+    }
+    if (s->graph) {
+      (void)hipGraphDestroy(s->graph);
+      s->graph = nullptr;
+    }
+    HIPC(s->dist.alloc(std::max<size_t>(dist.size(), 64)));
+    HIPC(s->freq.alloc(std::max<size_t>(freq.size(), 64)));
+  }
+  HIPC(hipMemcpy(s->dist.p, dist.data(), dist.size() * 4, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(s->freq.p, freq.data(), freq.size() * 4, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(s->pts.p, pd.data(), pd.size() * sizeof(PointDev), hipMemcpyHostToDevice));
+  return CVF_OK;
+}
+
+int capture_step_graph(cvf_sim* s) {
+  if (s->gexec) return CVF_OK;
+  HIPC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  const Launch lh{s->tab_half.p, s->nb_half}, lc{s->tab_cube.p, s->nb_cube},
+      ll{s->tab_cell.p, s->nb_cell};
+  launch_step_begin(s->stream, s->pts.p, s->npts, s->pst.p, s->sched.p, s->nsched, s->dist.p,
+                    s->freq.p, s->vol_device, s->precision);
+  launch_eta(s->stream, lc, s->pts.p, s->pst.p, s->words.p, 0);
+  launch_eta(s->stream, lc, s->pts.p, s->pst.p, s->words.p, 1);
+  for (int sub = 0; sub < 12 * s->nm; ++sub)
+    launch_sigma(s->stream, lh, s->pts.p, s->pst.p, s->words.p, s->sched.p, (uint32_t)sub);
+  for (int jc = 0; jc < s->nc; ++jc) {
+    launch_sw_local(s->stream, ll, s->pts.p, s->pst.p, s->words.p, s->parent.p, s->cov.p,
+                    (uint32_t)jc);
+    launch_sw_union(s->stream, ll, s->pts.p, s->parent.p, s->cov.p);
+    launch_sw_flip(s->stream, ll, s->pts.p, s->pst.p, s->words.p, s->parent.p, (uint32_t)jc);
+  }
+  hipError_t e = hipStreamEndCapture(s->stream, &s->graph);
+  if (e != hipSuccess) return fail(CVF_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+  HIPC(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+  return CVF_OK;
+}
+
+int check_pt(cvf_sim* s, int32_t pt) {
+  if (!s) return fail(CVF_EINVAL, "null handle");
+  if (pt < 0 || pt >= s->npts) return fail(CVF_EINVAL, "state point %d out of range [0,%d)", pt, s->npts);
+  return CVF_OK;
+}
+
+int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
+  if (!pts || !out || n <= 0) return fail(CVF_EINVAL, "bad arguments");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    int rc = validate(pts[i]);
+    if (rc) return rc;
+    if (pts[i].n_metropolis != pts[0].n_metropolis || pts[i].n_cluster != pts[0].n_cluster ||
+        pts[i].precision != pts[0].precision || pts[i].device_id != pts[0].device_id ||
+        pts[i].volume_mode != pts[0].volume_mode)
+      return fail(CVF_EINVAL, "state points of a batch must share the MC schedule, precision, "
+                              "volume mode and device");
+  }
+  int ndev = 0;
+  HIPC(hipGetDeviceCount(&ndev));
+  if (pts[0].device_id < 0 || pts[0].device_id >= ndev)
+    return fail(CVF_EINVAL, "Error: Tried to execute in GPU device ID = %d (total %d)",
+                pts[0].device_id, ndev);
+  HIPC(hipSetDevice(pts[0].device_id));
+  std::unique_ptr<cvf_sim> s(new cvf_sim);
+  s->device = pts[0].device_id;
+  s->npts = n;
+  s->nm = pts[0].n_metropolis;
+  s->nc = pts[0].n_cluster;
+  s->precision = pts[0].precision;
+  s->vol_device = pts[0].volume_mode == CVF_VOLUME_DEVICE;
+  s->nsched = std::max(12 * s->nm, 1);
+  HIPC(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  size_t woff = 0;
+  uint32_t maxL = 0;
+  for (int i = 0; i < n; ++i) {
+    PointHost P;
+    P.prm = pts[i];
+    P.L = (uint32_t)pts[i].L;
+    P.N = P.L * P.L * P.L;
+    woff = (woff + 63) & ~(size_t)63;  // 256-byte aligned point start
+    P.woff = (uint32_t)woff;
+    woff += P.N;
+    P.thr = make_thresholds(P.prm.P, P.prm.T, P.prm.J_hb, P.prm.J_sig, P.prm.v_hb);
+    shell_table((int)P.L, P.prm.r_cutoff, P.dist, P.freq);
+    P.vp.seed = P.prm.seed;
+    P.vp.L = (int32_t)P.L;
+    P.vp.N = (int32_t)P.N;
+    P.vp.T = P.prm.T;
+    P.vp.P = P.prm.P;
+    P.vp.r_cutoff = P.prm.r_cutoff;
+    P.vp.nshell = (int32_t)P.dist.size();
+    P.vp.beta = P.thr.beta;
+    // volume chain start (gpu_water3D.cu:150, 481, 1128)
+    P.vol.r_cell = 1.0001;
+    P.vol.V = (double)(int)P.N * P.vol.r_cell * P.vol.r_cell * P.vol.r_cell;
+    P.vol.E_vdw = vdw_energy_fp64(P.vol.r_cell, P.prm.r_cutoff, P.dist.data(), P.freq.data(),
+                                  (int)P.dist.size());
+    P.vol.accepted = 0;
+    P.vol.pad = 0;
+    maxL = std::max(maxL, P.L);
+    s->ph.push_back(P);
+  }
+  s->total_cells = woff;
+  size_t loff = 0;
+  for (auto& P : s->ph) {
+    P.loff = (uint32_t)loff;
+    loff += 6 * (size_t)P.N;
+  }
+  s->total_nodes = loff;
+  if (loff >= 0xFFFFFFFFull) return fail(CVF_ENOMEM, "batch too large for 32-bit node offsets");
+  // device buffers
+  HIPC(s->words.alloc(s->total_cells));
+  HIPC(s->parent.alloc(s->total_nodes));
+  HIPC(hipMemset(s->parent.p, 0, s->total_nodes * 4));  // labels read as 0 before the first SW step
+  HIPC(s->cov.alloc(s->total_cells));
+  HIPC(s->sched.alloc((size_t)n * s->nsched));
+  HIPC(hipMemset(s->sched.p, 0, (size_t)n * s->nsched));
+  HIPC(s->pts.alloc(n));
+  HIPC(s->pst.alloc(n));
+  HIPC(s->counters.alloc(2 * (size_t)n));
+  uint32_t maxN = maxL * maxL * maxL;
+  HIPC(s->csize.alloc(6 * (size_t)maxN));
+  HIPC(s->slot.alloc(6 * (size_t)maxN));
+  s->max_slots = 3 * maxL * maxL + 1;  // a cluster of >= 2L nodes: at most 6N/(2L)
+  HIPC(s->cover.alloc((size_t)s->max_slots * 3 * ((maxL + 31) / 32)));
+  HIPC(s->nslot_nperc.alloc(2));
+  int rc = upload_tables(s.get());
+  if (rc) return rc;
+  std::vector<PointState> st(n);
+  for (int i = 0; i < n; ++i) {
+    st[i].step = 0;
+    st[i].cur_step = 0;
+    st[i].vol = s->ph[i].vol;
+  }
+  HIPC(hipMemcpy(s->pst.p, st.data(), n * sizeof(PointState), hipMemcpyHostToDevice));
+  // block tables
+  auto th = make_table(s->ph, [](const PointHost& P) { return P.N / 2; });
+  auto tc = make_table(s->ph, [](const PointHost& P) { return (P.L / 2) * (P.L / 2) * (P.L / 2); });
+  auto tl = make_table(s->ph, [](const PointHost& P) { return P.N; });
+  HIPC(s->tab_half.alloc(th.size()));
+  HIPC(s->tab_cube.alloc(tc.size()));
+  HIPC(s->tab_cell.alloc(tl.size()));
+  HIPC(hipMemcpy(s->tab_half.p, th.data(), th.size() * sizeof(BlockEnt), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(s->tab_cube.p, tc.data(), tc.size() * sizeof(BlockEnt), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(s->tab_cell.p, tl.data(), tl.size() * sizeof(BlockEnt), hipMemcpyHostToDevice));
+  s->nb_half = (uint32_t)th.size();
+  s->nb_cube = (uint32_t)tc.size();
+  s->nb_cell = (uint32_t)tl.size();
+  // initial configurations
+  std::vector<uint32_t> w;
+  for (int i = 0; i < n; ++i) {
+    init_words(s->ph[i], w);
+    rc = upload_point_words(s.get(), i, w);
+    if (rc) return rc;
+  }
+  HIPC(hipDeviceSynchronize());
+  *out = s.release();
+  return CVF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cvf_params_default(cvf_params* p) {
+  memset(p, 0, sizeof *p);
+  p->L = 32;
+  p->P = 0.45f;
+  p->T = 0.815f;
+  p->seed = 44648604ull;
+  p->n_metropolis = 1;
+  p->n_cluster = 0;
+  p->r_cutoff = 6.0f;
+  p->flag_config = 0;
+  p->flag_chessboard = 1;
+  p->device_id = 0;
+  p->J_hb = 0.5f;
+  p->J_sig = 0.08f;
+  p->v_hb = 0.6f;
+  p->precision = CVF_FP64;
+  p->volume_mode = CVF_VOLUME_HOST;
+}
+
+int cvf_create(const cvf_params* p, cvf_sim** out) { return create_impl(p, 1, out); }
+
+int cvf_create_batch(const cvf_params* pts, int32_t n, cvf_sim** out) {
+  return create_impl(pts, n, out);
+}
+
+void cvf_destroy(cvf_sim* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  (void)hipStreamSynchronize(s->stream);
+  delete s;
+}
+
+const char* cvf_last_error(void) { return g_err.c_str(); }
+
+int cvf_set_config(cvf_sim* s, int32_t pt, const uint8_t* sigma6N, const uint8_t* eta6N) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  if (!sigma6N || !eta6N) return fail(CVF_EINVAL, "null config buffer");
+  HIPC(hipSetDevice(s->device));
+  const PointHost& P = s->ph[pt];
+  const uint32_t N = P.N, L = P.L;
+  for (uint32_t i = 0; i < 6 * N; ++i) {
+    if (sigma6N[i] > 5)
+      return fail(CVF_ERANGE, "Unexpected value for spin (%d) appeared in line %u", sigma6N[i], i % N);
+    if (eta6N[i] > 1)
+      return fail(CVF_ERANGE, "Unexpected value for active_bond (%d) appeared in line %u", eta6N[i],
+                  i % N);
+  }
+  std::vector<uint32_t> w(N);
+  for (uint32_t c = 0; c < N; ++c) {
+    uint8_t s6[6], e6[6];
+    for (int a = 0; a < 6; ++a) {
+      s6[a] = sigma6N[a * N + c];
+      e6[a] = eta6N[a * N + c];
+    }
+    w[c] = pack_cell(s6, e6);
+  }
+  int bad = 0;
+  std::vector<uint32_t> masks;
+  std::vector<uint32_t> org;
+  for_each_cube(L, [&](uint32_t ox, uint32_t oy, uint32_t oz, uint32_t) {
+    bool ok = true;
+    const uint32_t st = cube_state_from_words(w, L, ox, oy, oz);
+    const uint32_t cur = cube_mask_from_words(w, L, ox, oy, oz, &ok);
+    const uint32_t M = sol_mask(st);
+    if (!ok || cur != M) ++bad;
+    masks.push_back(M);
+    org.push_back(ox | (oy << 8) | (oz << 16));
+  });
+  // rewrite after all decisions are taken (each decision reads the input eta)
+  for (size_t k = 0; k < masks.size(); ++k)
+    set_cube_eta(w, L, org[k] & 0xFF, (org[k] >> 8) & 0xFF, org[k] >> 16, masks[k]);
+  rc = upload_point_words(s, pt, w);
+  if (rc) return rc;
+  return bad;
+}
+
+int cvf_get_config(cvf_sim* s, int32_t pt, uint8_t* sigma6N, uint8_t* eta6N) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  HIPC(hipSetDevice(s->device));
+  std::vector<uint32_t> w;
+  rc = download_point_words(s, pt, w);
+  if (rc) return rc;
+  const uint32_t N = s->ph[pt].N;
+  for (uint32_t c = 0; c < N; ++c)
+    for (int a = 0; a < 6; ++a) {
+      if (sigma6N) sigma6N[a * N + c] = (uint8_t)sigma_of(w[c], a);
+      if (eta6N) eta6N[a * N + c] = (uint8_t)eta_of(w[c], a);
+    }
+  return CVF_OK;
+}
+
+int cvf_step(cvf_sim* s, int64_t n_steps) {
+  if (!s) return fail(CVF_EINVAL, "null handle");
+  if (n_steps < 0) return fail(CVF_EINVAL, "negative step count");
+  HIPC(hipSetDevice(s->device));
+  int rc = capture_step_graph(s);
+  if (rc) return rc;
+  for (int64_t i = 0; i < n_steps; ++i) {
+    if (!s->vol_device) {
+      for (auto& P : s->ph) {
+        double u2 = 0;
+        const double trial = volume_trial(P.vol, P.vp, (uint64_t)s->steps, &u2);
+        const double te =
+            vdw_energy_fp64(trial, P.prm.r_cutoff, P.dist.data(), P.freq.data(), (int)P.dist.size());
+        volume_accept(P.vol, P.vp, trial, te, u2);
+      }
+    }
+    HIPC(hipGraphLaunch(s->gexec, s->stream));
+    ++s->steps;
+  }
+  if (s->nc > 0 && n_steps > 0) s->sw_done = true;
+  return CVF_OK;
+}
+
+int cvf_sync(cvf_sim* s) {
+  if (!s) return fail(CVF_EINVAL, "null handle");
+  HIPC(hipSetDevice(s->device));
+  HIPC(hipStreamSynchronize(s->stream));
+  return CVF_OK;
+}
+
+int cvf_get_volume(cvf_sim* s, int32_t pt, double* r_cell, double* V, double* E_vdw) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  HIPC(hipSetDevice(s->device));
+  VolState v = s->ph[pt].vol;
+  if (s->vol_device) {
+    PointState st;
+    HIPC(hipStreamSynchronize(s->stream));
+    HIPC(hipMemcpy(&st, s->pst.p + pt, sizeof st, hipMemcpyDeviceToHost));
+    v = st.vol;
+  }
+  if (r_cell) *r_cell = v.r_cell;
+  if (V) *V = v.V;
+  if (E_vdw) *E_vdw = v.E_vdw;
+  return CVF_OK;
+}
+
+int cvf_observe(cvf_sim* s, int32_t pt, cvf_obs* o) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  if (!o) return fail(CVF_EINVAL, "null output");
+  HIPC(hipSetDevice(s->device));
+  if (s->obs_at != s->steps) {
+    launch_zero_u32(s->stream, (uint32_t*)s->counters.p, 4 * (size_t)s->npts);
+    launch_observe(s->stream, Launch{s->tab_cell.p, s->nb_cell}, s->pts.p, s->words.p, s->counters.p);
+    s->obs_counters.resize(2 * (size_t)s->npts);
+    HIPC(hipMemcpyAsync(s->obs_counters.data(), s->counters.p, 16 * (size_t)s->npts,
+                        hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+    s->obs_at = s->steps;
+  }
+  double r_cell, V, E_vdw;
+  rc = cvf_get_volume(s, pt, &r_cell, &V, &E_vdw);
+  if (rc) return rc;
+  const PointHost& P = s->ph[pt];
+  const int64_t nhb2 = (int64_t)s->obs_counters[2 * pt], nsig = (int64_t)s->obs_counters[2 * pt + 1];
+  // measure(), gpu_water3D.cu:2138-2152
+  const double n_hb = 0.5 * (double)nhb2;
+  double e = E_vdw + n_hb * (double)(-P.prm.J_hb);
+  const double v = (double)(int)P.N * r_cell * r_cell * r_cell + n_hb * (double)P.prm.v_hb;
+  e += (double)(-P.prm.J_sig) * (double)nsig;
+  o->step = s->steps;
+  o->E = e;
+  o->H = e + (double)P.prm.P * v;
+  o->V = v;
+  o->n_hb = n_hb;
+  o->m = (double)nsig / (1. * (double)P.N * 15.);
+  o->r_cell = r_cell;
+  o->E_vdw = E_vdw;
+  o->nhb2 = nhb2;
+  o->n_sigma = nsig;
+  return CVF_OK;
+}
+
+int cvf_get_labels(cvf_sim* s, int32_t pt, int32_t* labels6N) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  HIPC(hipSetDevice(s->device));
+  HIPC(hipStreamSynchronize(s->stream));
+  const PointHost& P = s->ph[pt];
+  HIPC(hipMemcpy(labels6N, s->parent.p + P.loff, 6 * (size_t)P.N * 4, hipMemcpyDeviceToHost));
+  return CVF_OK;
+}
+
+int cvf_cluster_stats(cvf_sim* s, int32_t pt, cvf_cstat* c) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  if (!c) return fail(CVF_EINVAL, "null output");
+  HIPC(hipSetDevice(s->device));
+  const PointHost& P = s->ph[pt];
+  const uint32_t nn = 6 * P.N, L = P.L;
+  const uint32_t* label = s->parent.p + P.loff;
+  const int32_t minsize = 2 * (int32_t)L;
+  const uint32_t nw = (L + 31) / 32;
+  launch_zero_u32(s->stream, (uint32_t*)s->csize.p, nn);
+  launch_zero_u32(s->stream, s->nslot_nperc.p, 2);
+  launch_zero_u32(s->stream, s->cover.p, (size_t)s->max_slots * 3 * nw);
+  launch_cluster_sizes(s->stream, label, nn, s->csize.p);
+  launch_perc_slots(s->stream, label, nn, s->csize.p, minsize, s->slot.p, s->nslot_nperc.p,
+                    s->max_slots);
+  launch_perc_cover(s->stream, label, nn, L, s->csize.p, minsize, s->slot.p, s->cover.p, s->max_slots);
+  launch_perc_count(s->stream, s->cover.p, s->nslot_nperc.p, L, s->max_slots, s->nslot_nperc.p + 1);
+  std::vector<int32_t> size(nn);
+  uint32_t np[2];
+  HIPC(hipMemcpyAsync(size.data(), s->csize.p, (size_t)nn * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipMemcpyAsync(np, s->nslot_nperc.p, 8, hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipStreamSynchronize(s->stream));
+  if (np[0] > s->max_slots) return fail(CVF_EDEVICE, "percolation slot overflow");
+  // print_cluster_statistics (gpu_water3D.cu:2677-2721): float sums in
+  // ascending label order, int32 size*size with wrap-around.
+  int32_t largest = 0, second = 0, ncl = 0;
+  float average = 0, average2 = 0;
+  for (uint32_t i = 0; i < nn; ++i) {
+    const int32_t sz = size[i];
+    if (sz == 0) continue;
+    ++ncl;
+    average += (float)sz;
+    average2 += (float)(int32_t)((uint32_t)sz * (uint32_t)sz);
+    if (sz > largest) {
+      second = largest;
+      largest = sz;
+    } else if (sz > second) {
+      second = sz;
+    }
+  }
+  average /= (float)ncl;
+  average2 /= (float)ncl;
+  average2 -= average * average;
+  c->largest = largest;
+  c->second = second;
+  c->n_perc = (int32_t)np[1];
+  c->n_clusters = ncl;
+  c->mean = average;
+  c->var = average2;
+  return CVF_OK;
+}
+
+int cvf_get_shells(cvf_sim* s, int32_t pt, float* dist, int32_t* freq, int32_t cap, int32_t* nshell) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  const PointHost& P = s->ph[pt];
+  if (nshell) *nshell = (int32_t)P.dist.size();
+  const int32_t n = std::min<int32_t>(cap, (int32_t)P.dist.size());
+  for (int32_t k = 0; k < n; ++k) {
+    if (dist) dist[k] = P.dist[k];
+    if (freq) freq[k] = P.freq[k];
+  }
+  return CVF_OK;
+}
+
+int cvf_set_shells(cvf_sim* s, int32_t pt, const float* dist, const int32_t* freq, int32_t nshell) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  if (nshell < 0 || (nshell > 0 && (!dist || !freq))) return fail(CVF_EINVAL, "bad shell table");
+  HIPC(hipSetDevice(s->device));
+  HIPC(hipStreamSynchronize(s->stream));
+  PointHost& P = s->ph[pt];
+  P.dist.assign(dist, dist + nshell);
+  P.freq.assign(freq, freq + nshell);
+  P.vol.E_vdw = vdw_energy_fp64(P.vol.r_cell, P.prm.r_cutoff, P.dist.data(), P.freq.data(), nshell);
+  rc = upload_tables(s);
+  if (rc) return rc;
+  // device-side volume state follows the host one
+  PointState st;
+  HIPC(hipMemcpy(&st, s->pst.p + pt, sizeof st, hipMemcpyDeviceToHost));
+  st.vol.E_vdw = vdw_energy_fp64(st.vol.r_cell, P.prm.r_cutoff, P.dist.data(), P.freq.data(), nshell);
+  HIPC(hipMemcpy(s->pst.p + pt, &st, sizeof st, hipMemcpyHostToDevice));
+  return CVF_OK;
+}
+
+int cvf_info(cvf_sim* s, int32_t pt, int32_t* n_points, int32_t* L, int64_t* steps) {
+  if (!s) return fail(CVF_EINVAL, "null handle");
+  if (n_points) *n_points = s->npts;
+  if (steps) *steps = s->steps;
+  if (L) {
+    int rc = check_pt(s, pt);
+    if (rc) return rc;
+    *L = (int32_t)s->ph[pt].L;
+  }
+  return CVF_OK;
+}
+
+int cvf_get_stream(cvf_sim* s, void** stream) {
+  if (!s || !stream) return fail(CVF_EINVAL, "null argument");
+  *stream = (void*)s->stream;
+  return CVF_OK;
+}
+
+int cvf_profile_steps(cvf_sim* s, int64_t n_steps, double* kernel_ms, int64_t* launches,
+                      int32_t n_classes) {
+  if (!s || n_steps < 0 || !kernel_ms || n_classes < CVF_KCLASS_COUNT)
+    return fail(CVF_EINVAL, "bad arguments");
+  HIPC(hipSetDevice(s->device));
+  if (!s->vol_device && n_steps > 0)
+    return fail(CVF_EUNSUPPORTED, "profiling needs the device volume mode");
+  std::vector<hipEvent_t> ev;
+  std::vector<int> cls;
+  const Launch lh{s->tab_half.p, s->nb_half}, lc{s->tab_cube.p, s->nb_cube},
+      ll{s->tab_cell.p, s->nb_cell};
+  auto mark = [&](int c) -> int {
+    hipEvent_t e;
+    HIPC(hipEventCreate(&e));
+    HIPC(hipEventRecord(e, s->stream));
+    ev.push_back(e);
+    cls.push_back(c);
+    return CVF_OK;
+  };
+  int rc = mark(-1);
+  if (rc) return rc;
+  for (int64_t i = 0; i < n_steps; ++i) {
+    launch_step_begin(s->stream, s->pts.p, s->npts, s->pst.p, s->sched.p, s->nsched, s->dist.p,
+                      s->freq.p, s->vol_device, s->precision);
+    if ((rc = mark(CVF_KCLASS_BEGIN))) return rc;
+    for (int f = 0; f < 2; ++f) {
+      launch_eta(s->stream, lc, s->pts.p, s->pst.p, s->words.p, f);
+      if ((rc = mark(CVF_KCLASS_ETA))) return rc;
+    }
+    for (int sub = 0; sub < 12 * s->nm; ++sub) {
+      launch_sigma(s->stream, lh, s->pts.p, s->pst.p, s->words.p, s->sched.p, (uint32_t)sub);
+      if ((rc = mark(CVF_KCLASS_SIGMA))) return rc;
+    }
+    for (int jc = 0; jc < s->nc; ++jc) {
+      launch_sw_local(s->stream, ll, s->pts.p, s->pst.p, s->words.p, s->parent.p, s->cov.p,
+                      (uint32_t)jc);
+      if ((rc = mark(CVF_KCLASS_SW_LOCAL))) return rc;
+      launch_sw_union(s->stream, ll, s->pts.p, s->parent.p, s->cov.p);
+      if ((rc = mark(CVF_KCLASS_SW_UNION))) return rc;
+      launch_sw_flip(s->stream, ll, s->pts.p, s->pst.p, s->words.p, s->parent.p, (uint32_t)jc);
+      if ((rc = mark(CVF_KCLASS_SW_FLIP))) return rc;
+    }
+    ++s->steps;
+  }
+  if (s->nc > 0 && n_steps > 0) s->sw_done = true;
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(s->stream));
+  std::vector<double> tot(CVF_KCLASS_COUNT, 0.0);
+  std::vector<int64_t> cnt(CVF_KCLASS_COUNT, 0);
+  for (size_t k = 1; k < ev.size(); ++k) {
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, ev[k - 1], ev[k]));
+    tot[cls[k]] += ms;
+    cnt[cls[k]] += 1;
+  }
+  for (auto e : ev) (void)hipEventDestroy(e);
+  for (int c = 0; c < CVF_KCLASS_COUNT; ++c) {
+    kernel_ms[c] = cnt[c] ? tot[c] / (double)cnt[c] : 0.0;
+    if (launches) launches[c] = cnt[c];
+  }
+  s->obs_at = -1;
+  return CVF_OK;
+}
+
+int cvf_vdw_energy(int32_t device_id, int32_t L, const float* dist, const int32_t* freq,
+                   int32_t nshell, float r_cutoff, const double* r, int32_t n_r, int32_t precision,
+                   double* energy) {
+  if (L < 2 || nshell <= 0 || n_r <= 0 || !dist || !freq || !r || !energy)
+    return fail(CVF_EINVAL, "bad arguments");
+  if (precision != CVF_FP64 && precision != CVF_FP32_COMPENSATED)
+    return fail(CVF_EINVAL, "unknown precision");
+  HIPC(hipSetDevice(device_id));
+  const int N = L * L * L;  // f_k = (count_k/2) * N
+  DevBuf<float> d;
+  DevBuf<int32_t> f;
+  DevBuf<double> dr, de;
+  HIPC(d.alloc(nshell));
+  HIPC(f.alloc(nshell));
+  HIPC(dr.alloc(n_r));
+  HIPC(de.alloc(n_r));
+  HIPC(hipMemcpy(d.p, dist, nshell * 4, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(f.p, freq, nshell * 4, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(dr.p, r, n_r * 8, hipMemcpyHostToDevice));
+  launch_vdw_eval(nullptr, d.p, f.p, nshell, r_cutoff, N, dr.p, n_r, precision, de.p);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpy(energy, de.p, n_r * 8, hipMemcpyDeviceToHost));
+  return CVF_OK;
+}
+
+}  // extern "C"

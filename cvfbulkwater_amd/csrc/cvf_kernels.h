@@ -1,0 +1,77 @@
+// Device-side data structures and the host launchers of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cvf_core.h"
+
+namespace cvf {
+
+constexpr int kBlock = 256;
+constexpr int kNumXcd = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// Immutable per-state-point description (device copy).
+struct PointDev {
+  uint64_t seed;
+  uint32_t L, N, hL;
+  uint32_t woff;       // offset of the point's words in the state array
+  uint32_t loff;       // offset of the point's 6N labels
+  uint32_t shell_off;  // offset of its shell table
+  FastDiv divL, divH;  // division by L and by L/2
+  int32_t jeff_sign;   // sign of Jeff (SW covalent rule, gpu_water3D.cu:2399-2424)
+  int32_t sched_off;   // offset of its (arm, parity) schedule for the current step
+  VolParams vp;
+  uint64_t thr_sigma[33];  // index (eta*dhb + 1)*11 + dsig + 5
+  uint64_t thr_eta[9];     // index DeltaN_HB + 4
+  uint64_t thr_sw[2];      // covalent, intra-molecule link activation
+};
+
+// Mutable per-state-point device state.
+struct PointState {
+  uint64_t step;      // next MC step index
+  uint64_t cur_step;  // step being executed
+  VolState vol;
+};
+
+// One workgroup's work: state point and first work item.
+struct BlockEnt {
+  int32_t point;
+  uint32_t start;
+};
+
+struct Launch {
+  const BlockEnt* tab;
+  uint32_t nblocks;  // padded to a multiple of kNumXcd
+};
+
+// Launchers (all asynchronous on `st`).
+void launch_step_begin(hipStream_t st, const PointDev* pts, int npts, PointState* ps, uint8_t* sched,
+                       int nsched, const float* dist, const int32_t* freq, int vol_device,
+                       int precision);
+void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps, uint32_t* words,
+                int family);
+void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
+                  uint32_t* words, const uint8_t* sched, uint32_t sub);
+void launch_zero_u32(hipStream_t st, uint32_t* p, size_t n);
+void launch_observe(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* words,
+                    unsigned long long* counters);
+void launch_sw_local(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
+                     const uint32_t* words, uint32_t* parent, uint8_t* cov, uint32_t jc);
+void launch_sw_union(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* parent,
+                     const uint8_t* cov);
+void launch_sw_flip(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
+                    uint32_t* words, uint32_t* parent, uint32_t jc);
+// cluster statistics of one point (labels at `label`, 6N nodes)
+void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, int32_t* size);
+void launch_perc_slots(hipStream_t st, const uint32_t* label, uint32_t nn, const int32_t* size,
+                       int32_t minsize, int32_t* slot, uint32_t* nslot, uint32_t max_slots);
+void launch_perc_cover(hipStream_t st, const uint32_t* label, uint32_t nn, uint32_t L,
+                       const int32_t* size, int32_t minsize, const int32_t* slot, uint32_t* cover,
+                       uint32_t max_slots);
+void launch_perc_count(hipStream_t st, const uint32_t* cover, const uint32_t* nslot, uint32_t L,
+                       uint32_t max_slots, uint32_t* nperc);
+void launch_vdw_eval(hipStream_t st, const float* dist, const int32_t* freq, int nshell,
+                     float r_cutoff, int N, const double* r, int n_r, int precision, double* out);
+
+}  // namespace cvf

@@ -1,0 +1,610 @@
+// HIP kernels of the CVF bulk-water Monte Carlo engine (gfx950).
+//
+// The lattice state is one packed 32-bit word per cell (cvf_core.h).  One MC
+// step of the reference schedule (MonteCarlo_Step, gpu_water3D.cu:1346-1733)
+// is the launch sequence
+//     step_begin -> eta(family 0) -> eta(family 1)
+//                -> sigma(sub 0) ... sigma(sub 12*n_metropolis - 1)
+//                -> [sw_local -> sw_union -> sw_flip] x n_cluster
+// which the host captures once into a hipGraph and replays.  Every kernel
+// takes a block table: entry b names the state point and first work item of
+// workgroup b, so one launch advances any number of state points of uneven
+// L.  Workgroup ids are remapped so that each XCD gets a contiguous range of
+// the table (each XCD has its own L2; a point's words then stay in one L2
+// from launch to launch).
+#include "cvf_kernels.h"
+
+namespace cvf {
+
+__device__ __forceinline__ uint32_t xcd_logical_block(uint32_t b, uint32_t nb) {
+  // hardware places workgroup b on XCD (b % 8); give XCD k the table range
+  // [k*nb/8, (k+1)*nb/8).  nb is a multiple of 8.
+  const uint32_t per = nb / kNumXcd;
+  return (b % kNumXcd) * per + b / kNumXcd;
+}
+
+__device__ __forceinline__ uint64_t load_step(const PointState* ps, int p) {
+  return ps[p].cur_step;
+}
+
+// ------------------------------------------------------------ step begin
+// One wavefront per state point: reads and advances the point's step
+// counter, draws the (arm, parity) schedule of the step's half-sweeps
+// (gpu_water3D.cu:1393-1394) and, in device-volume mode, performs the
+// volume move (volume_step, 1740-1781).
+__device__ __forceinline__ void two_sum_acc(float& s, float& c, float v) {
+  // Neumaier's compensated step
+  const float t = s + v;
+  if (fabsf(s) >= fabsf(v))
+    c += (s - t) + v;
+  else
+    c += (v - t) + s;
+  s = t;
+}
+
+__device__ __forceinline__ void two_sum_pair(float& s, float& c, float s2, float c2) {
+  const float t = s + s2;
+  const float bp = t - s;
+  const float err = (s - (t - bp)) + (s2 - bp);
+  s = t;
+  c = c + c2 + err;
+}
+
+// Per-molecule vdW sum with FP32 terms, lane-local Neumaier sums and a
+// compensated butterfly across the wavefront; all lanes get the result.
+__device__ double vdw_fp32_wave(double r, const float* dist, const int32_t* freq, int nshell,
+                                float r_cutoff, int N) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  float tc = 1.0f / r_cutoff;
+  tc = tc * tc * tc;
+  const float shift = tc * tc * tc * tc - tc * tc;
+  float s = 0.f, c = 0.f;
+  for (int k = lane; k < nshell; k += 64) {
+    const double rd = r * (double)dist[k];
+    if (rd < (double)r_cutoff) {
+      const float x = (float)rd;
+      float t = 1.0f / x;
+      t = t * t * t;
+      const float att = t * t;
+      const float rep = att * att;
+      const float g = rep - att - shift;
+      two_sum_acc(s, c, (float)(freq[k] / N) * g);
+    }
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float s2 = __shfl_xor(s, m, 64);
+    const float c2 = __shfl_xor(c, m, 64);
+    two_sum_pair(s, c, s2, c2);
+  }
+  return ((double)s + (double)c) * (double)N;
+}
+
+__global__ __launch_bounds__(64) void k_step_begin(const PointDev* __restrict__ pts,
+                                                   PointState* __restrict__ ps,
+                                                   uint8_t* __restrict__ sched, int nsched,
+                                                   const float* __restrict__ dist,
+                                                   const int32_t* __restrict__ freq, int vol_device,
+                                                   int precision) {
+  const int p = blockIdx.x;
+  const int lane = threadIdx.x;
+  const PointDev& P = pts[p];
+  const uint64_t step = ps[p].step;
+  for (int s = lane; s < nsched; s += 64) {
+    const U4 r = draw(P.seed, kStSched, 0, step, (uint32_t)s);
+    sched[P.sched_off + s] = (uint8_t)(uniform_int(r.x, 6) | ((r.y >> 31) << 3));
+  }
+  if (vol_device) {
+    VolState vs = ps[p].vol;
+    double u2 = 0;
+    const double trial = volume_trial(vs, P.vp, step, &u2);
+    const float* d = dist + P.shell_off;
+    const int32_t* f = freq + P.shell_off;
+    double te;
+    if (precision == 1)
+      te = vdw_fp32_wave(trial, d, f, P.vp.nshell, P.vp.r_cutoff, P.vp.N);
+    else
+      te = vdw_energy_fp64(trial, P.vp.r_cutoff, d, f, P.vp.nshell);
+    if (lane == 0) {
+      volume_accept(vs, P.vp, trial, te, u2);
+      ps[p].vol = vs;
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    ps[p].cur_step = step;
+    ps[p].step = step + 1;
+  }
+}
+
+// ------------------------------------------------------------ eta sweep
+// Cube Metropolis of gpu_Chessboard_set_active_bonds (gpu_water3D.cu:
+// 3374-3466).  Family 0 holds the cubes with all-even origin, family 1 the
+// all-odd ones; within a family the cubes' corners are disjoint, so each
+// thread updates its 8 corner words without atomics.  Cube states of both
+// families are independent given sigma, so running the families one after
+// the other equals the reference's simultaneous update.
+__global__ __launch_bounds__(kBlock) void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb,
+                                                const PointDev* __restrict__ pts,
+                                                const PointState* __restrict__ ps,
+                                                uint32_t* __restrict__ words, int family) {
+  const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
+  if (e.point < 0) return;
+  const PointDev& P = pts[e.point];
+  __shared__ uint64_t thr[9];
+  if (threadIdx.x < 9) thr[threadIdx.x] = P.thr_eta[threadIdx.x];
+  __syncthreads();
+  const uint32_t hL = P.hL, L = P.L;
+  const uint32_t t = e.start + threadIdx.x;
+  if (t >= hL * hL * hL) return;
+  const uint64_t step = load_step(ps, e.point);
+  const uint32_t r1 = fdiv(t, P.divH);
+  const uint32_t i = t - r1 * hL;
+  const uint32_t k = fdiv(r1, P.divH);
+  const uint32_t j = r1 - k * hL;
+  const uint32_t ox = 2 * i + family, oy = 2 * j + family, oz = 2 * k + family;
+  const uint32_t x1 = ox + 1 == L ? 0 : ox + 1;
+  const uint32_t y1 = oy + 1 == L ? 0 : oy + 1;
+  const uint32_t z1 = oz + 1 == L ? 0 : oz + 1;
+  uint32_t* W = words + P.woff;
+  uint32_t idx[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint32_t x = (b & 1) ? x1 : ox, y = (b & 2) ? y1 : oy, z = (b & 4) ? z1 : oz;
+    idx[b] = x + L * (y + L * z);
+  }
+  uint32_t w[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) w[b] = W[idx[b]];
+  uint32_t D = 0, E = 0;
+#pragma unroll
+  for (int jj = 0; jj < kEdges; ++jj) {
+    const int A = edge_start(jj), ax = edge_axis(jj), B = A | (1 << ax);
+    D |= (uint32_t)(sigma_of(w[A], 2 * ax) == sigma_of(w[B], 2 * ax + 1)) << jj;
+    E |= eta_of(w[A], 2 * ax) << jj;
+  }
+  const uint32_t cube_ref = i + hL * j + hL * hL * oz;  // gpu_water3D.cu:3405
+  const U4 r = draw(P.seed, kStEta, cube_ref, step, 0);
+  const uint32_t M = sol_mask(uniform_int(r.x, 9));
+  const int dn = popc32(M & D) - popc32(E & D);
+  if ((uint64_t)r.y < thr[dn + 4] && M != E) {
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
+      const uint32_t ax_arm = bx, ay_arm = 2 + by, az_arm = 4 + bz;
+      const uint32_t own = (1u << ax_arm) | (1u << ay_arm) | (1u << az_arm);
+      const uint32_t bits = (((M >> corner_edge(b, 0)) & 1u) << ax_arm) |
+                            (((M >> corner_edge(b, 1)) & 1u) << ay_arm) |
+                            (((M >> corner_edge(b, 2)) & 1u) << az_arm);
+      W[idx[b]] = (w[b] & ~(own << kEtaShift)) | (bits << kEtaShift);
+    }
+  }
+}
+
+// --------------------------------------------------------- sigma half-sweep
+// gpu_update (gpu_water3D.cu:1890-2008), chessboard branch: arm `a` of the
+// cells whose coordinate along a's axis is even (indices = 1) or odd
+// (indices = 0).  A cell's new sigma_a depends on its own word and on the
+// facing arm of the neighbour across arm a; that neighbour has the other
+// parity along the axis, so no word read in a launch is written in it.
+__global__ __launch_bounds__(kBlock) void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb,
+                                                  const PointDev* __restrict__ pts,
+                                                  const PointState* __restrict__ ps,
+                                                  uint32_t* __restrict__ words,
+                                                  const uint8_t* __restrict__ sched, uint32_t sub) {
+  const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
+  if (e.point < 0) return;
+  const PointDev& P = pts[e.point];
+  __shared__ uint64_t thr[33];
+  if (threadIdx.x < 33) thr[threadIdx.x] = P.thr_sigma[threadIdx.x];
+  __syncthreads();
+  const uint32_t L = P.L, hL = P.hL;
+  const uint32_t t = e.start + threadIdx.x;
+  if (t >= P.N / 2) return;
+  const uint64_t step = load_step(ps, e.point);
+  const uint32_t sc = sched[P.sched_off + sub];
+  const int arm = (int)(sc & 7u);
+  const uint32_t q = (sc >> 3) ? 0u : 1u;  // kept parity along the axis
+  const int ax = arm >> 1;
+  uint32_t x, y, z;
+  if (ax == 0) {
+    const uint32_t r1 = fdiv(t, P.divH);
+    x = 2 * (t - r1 * hL) + q;
+    z = fdiv(r1, P.divL);
+    y = r1 - z * L;
+  } else if (ax == 1) {
+    const uint32_t r1 = fdiv(t, P.divL);
+    x = t - r1 * L;
+    z = fdiv(r1, P.divH);
+    y = 2 * (r1 - z * hL) + q;
+  } else {
+    const uint32_t r1 = fdiv(t, P.divL);
+    x = t - r1 * L;
+    const uint32_t hz = fdiv(r1, P.divL);
+    y = r1 - hz * L;
+    z = 2 * hz + q;
+  }
+  const uint32_t c = x + L * (y + L * z);
+  const uint32_t coord = ax == 0 ? x : (ax == 1 ? y : z);
+  const uint32_t stride = ax == 0 ? 1u : (ax == 1 ? L : L * L);
+  uint32_t nbc;
+  if (arm & 1)
+    nbc = coord == 0 ? c + (L - 1) * stride : c - stride;
+  else
+    nbc = coord == L - 1 ? c - (L - 1) * stride : c + stride;
+  uint32_t* W = words + P.woff;
+  const uint32_t w = W[c];
+  const uint32_t wn = W[nbc];
+  const U4 r = draw(P.seed, kStSigma, c, step, sub);
+  const uint32_t sa = sigma_of(w, arm);
+  const uint32_t sp = uniform_int(r.x, 6);
+  if (sp == sa) return;
+  const uint32_t snb = sigma_of(wn, arm ^ 1);
+  const int h = eta_of(w, arm) ? ((int)(sp == snb) - (int)(sa == snb)) : 0;
+  const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
+  if ((uint64_t)r.y < thr[(h + 1) * 11 + d + 5]) W[c] = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
+}
+
+// ----------------------------------------------------------- observables
+// Integer part of measure() (gpu_water3D.cu:2014-2103), chessboard branch:
+// 2*N_HB = sum over forward bonds with equal facing sigma of the two eta
+// bits; N_sigma = sum over cells of equal-sigma arm pairs.
+__global__ void k_zero_u32(uint32_t* p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_observe(const BlockEnt* __restrict__ tab, uint32_t nb,
+                                                    const PointDev* __restrict__ pts,
+                                                    const uint32_t* __restrict__ words,
+                                                    unsigned long long* __restrict__ counters) {
+  const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
+  if (e.point < 0) return;
+  const PointDev& P = pts[e.point];
+  const uint32_t L = P.L;
+  const uint32_t t = e.start + threadIdx.x;
+  uint32_t nhb2 = 0, nsig = 0;
+  if (t < P.N) {
+    const uint32_t r1 = fdiv(t, P.divL);
+    const uint32_t x = t - r1 * L;
+    const uint32_t z = fdiv(r1, P.divL);
+    const uint32_t y = r1 - z * L;
+    const uint32_t* W = words + P.woff;
+    const uint32_t w = W[t];
+    const uint32_t nx = x == L - 1 ? t - (L - 1) : t + 1;
+    const uint32_t ny = y == L - 1 ? t - (L - 1) * L : t + L;
+    const uint32_t nz = z == L - 1 ? t - (L - 1) * L * L : t + L * L;
+    const uint32_t nbs[3] = {nx, ny, nz};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      const uint32_t wn = W[nbs[ax]];
+      if (sigma_of(w, 2 * ax) == sigma_of(wn, 2 * ax + 1))
+        nhb2 += eta_of(w, 2 * ax) + eta_of(wn, 2 * ax + 1);
+    }
+    uint32_t pairs = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) pairs += (uint32_t)count_eq(w, sigma_of(w, a)) - 1;
+    nsig = pairs / 2;
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    nhb2 += __shfl_xor(nhb2, m, 64);
+    nsig += __shfl_xor(nsig, m, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && (nhb2 | nsig)) {
+    atomicAdd(&counters[2 * e.point], (unsigned long long)nhb2);
+    atomicAdd(&counters[2 * e.point + 1], (unsigned long long)nsig);
+  }
+}
+
+// --------------------------------------------------------- Swendsen-Wang
+// Bond activation of gpu_create_cluster (gpu_water3D.cu:2383-2438), then
+// union-find with minimum-root linking.  Every link only ever points a
+// larger node id to a smaller one, so each component's root is its minimum
+// node id, the same canonical label the reference's iterative min-label
+// propagation converges to (2440-2529), whatever the order of the atomics.
+__device__ __forceinline__ uint32_t uf_load(const uint32_t* p, uint32_t i) {
+  return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t uf_find(const uint32_t* par, uint32_t x) {
+  uint32_t p = uf_load(par, x);
+  while (p != x) {
+    x = p;
+    p = uf_load(par, x);
+  }
+  return x;
+}
+
+__device__ void uf_unite(uint32_t* par, uint32_t a, uint32_t b) {
+  while (true) {
+    a = uf_find(par, a);
+    b = uf_find(par, b);
+    if (a == b) return;
+    if (a < b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    uint32_t expected = a;
+    if (__hip_atomic_compare_exchange_strong(par + a, &expected, b, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return;
+  }
+}
+
+__device__ __forceinline__ void cell_coords(const PointDev& P, uint32_t t, uint32_t& x, uint32_t& y,
+                                            uint32_t& z) {
+  const uint32_t r1 = fdiv(t, P.divL);
+  x = t - r1 * P.L;
+  z = fdiv(r1, P.divL);
+  y = r1 - z * P.L;
+}
+
+// Links of one cell: 3 covalent (arm 0|2|4 of c with arm 1|3|5 of the +x|+y|+z
+// neighbour) and 15 intra-molecule (arm pairs k<l in order).  Writes the
+// intra-cell forest into `parent` and the active covalent links into cov[c].
+__global__ __launch_bounds__(kBlock) void k_sw_local(const BlockEnt* __restrict__ tab, uint32_t nb,
+                                                     const PointDev* __restrict__ pts,
+                                                     const PointState* __restrict__ ps,
+                                                     const uint32_t* __restrict__ words,
+                                                     uint32_t* __restrict__ parent,
+                                                     uint8_t* __restrict__ cov, uint32_t jc) {
+  const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
+  if (e.point < 0) return;
+  const PointDev& P = pts[e.point];
+  const uint32_t t = e.start + threadIdx.x;
+  if (t >= P.N) return;
+  const uint32_t L = P.L, N = P.N;
+  const uint64_t step = load_step(ps, e.point);
+  uint32_t x, y, z;
+  cell_coords(P, t, x, y, z);
+  const uint32_t* W = words + P.woff;
+  const uint32_t w = W[t];
+  const uint32_t nbs[3] = {x == L - 1 ? t - (L - 1) : t + 1, y == L - 1 ? t - (L - 1) * L : t + L,
+                           z == L - 1 ? t - (L - 1) * L * L : t + L * L};
+  uint32_t rw[20];
+#pragma unroll
+  for (int g = 0; g < 5; ++g) {
+    const U4 r = draw(P.seed, kStSwBond, t, step, jc * 5 + g);
+    rw[4 * g] = r.x;
+    rw[4 * g + 1] = r.y;
+    rw[4 * g + 2] = r.z;
+    rw[4 * g + 3] = r.w;
+  }
+  const uint64_t thb = P.thr_sw[0], ts = P.thr_sw[1];
+  uint32_t cmask = 0;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const uint32_t wn = W[nbs[ax]];
+    const uint32_t s0 = sigma_of(w, 2 * ax), s1 = sigma_of(wn, 2 * ax + 1);
+    bool act = false;
+    if (P.jeff_sign > 0)
+      act = eta_of(w, 2 * ax) && s0 == s1 && (uint64_t)rw[ax] < thb;
+    else if (P.jeff_sign < 0)
+      act = s0 != s1 && (uint64_t)rw[ax] < thb;
+    cmask |= (uint32_t)act << ax;
+  }
+  // intra-cell union-find over 6 arms, minimum-arm roots
+  uint32_t lab[6] = {0, 1, 2, 3, 4, 5};
+  int link = 3;
+#pragma unroll
+  for (int ii = 0; ii < 5; ++ii)
+#pragma unroll
+    for (int jj = ii + 1; jj < 6; ++jj) {
+      if (sigma_of(w, ii) == sigma_of(w, jj) && (uint64_t)rw[link] < ts) {
+        uint32_t ra = ii, rb = jj;
+        while (lab[ra] != ra) ra = lab[ra];
+        while (lab[rb] != rb) rb = lab[rb];
+        if (ra < rb) lab[rb] = ra;
+        else if (rb < ra) lab[ra] = rb;
+      }
+      ++link;
+    }
+  uint32_t* par = parent + P.loff;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    uint32_t r = a;
+    while (lab[r] != r) r = lab[r];
+    par[a * N + t] = r * N + t;
+  }
+  cov[P.woff + t] = (uint8_t)cmask;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sw_union(const BlockEnt* __restrict__ tab, uint32_t nb,
+                                                     const PointDev* __restrict__ pts,
+                                                     uint32_t* __restrict__ parent,
+                                                     const uint8_t* __restrict__ cov) {
+  const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
+  if (e.point < 0) return;
+  const PointDev& P = pts[e.point];
+  const uint32_t t = e.start + threadIdx.x;
+  if (t >= P.N) return;
+  const uint32_t cm = cov[P.woff + t];
+  if (!cm) return;
+  const uint32_t L = P.L, N = P.N;
+  uint32_t x, y, z;
+  cell_coords(P, t, x, y, z);
+  const uint32_t nbs[3] = {x == L - 1 ? t - (L - 1) : t + 1, y == L - 1 ? t - (L - 1) * L : t + L,
+                           z == L - 1 ? t - (L - 1) * L * L : t + L * L};
+  uint32_t* par = parent + P.loff;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+    if (cm & (1u << ax)) uf_unite(par, (2 * ax) * N + t, (2 * ax + 1) * N + nbs[ax]);
+}
+
+// Final labels (full path compression) and the cluster rotation of
+// gpu_update_cluster (gpu_water3D.cu:2531-2542): sigma += floor(6u(label)).
+__global__ __launch_bounds__(kBlock) void k_sw_flip(const BlockEnt* __restrict__ tab, uint32_t nb,
+                                                    const PointDev* __restrict__ pts,
+                                                    const PointState* __restrict__ ps,
+                                                    uint32_t* __restrict__ words,
+                                                    uint32_t* __restrict__ parent, uint32_t jc) {
+  const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
+  if (e.point < 0) return;
+  const PointDev& P = pts[e.point];
+  const uint32_t t = e.start + threadIdx.x;
+  if (t >= P.N) return;
+  const uint32_t N = P.N;
+  const uint64_t step = load_step(ps, e.point);
+  uint32_t* par = parent + P.loff;
+  uint32_t* W = words + P.woff;
+  uint32_t w = W[t];
+  uint32_t nw = w & ~kSigmaBits;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    const uint32_t node = a * N + t;
+    const uint32_t root = uf_find(par, node);
+    __hip_atomic_store(par + node, root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const U4 r = draw(P.seed, kStSwFlip, root, step, jc);
+    uint32_t s = sigma_of(w, a) + uniform_int(r.x, 6);
+    s = s >= 6 ? s - 6 : s;
+    nw |= s << (3 * a);
+  }
+  W[t] = nw;
+}
+
+// ------------------------------------------------- cluster statistics
+__global__ void k_cluster_sizes(const uint32_t* __restrict__ label, uint32_t nn,
+                                int32_t* __restrict__ size) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nn) atomicAdd(&size[label[i]], 1);
+}
+
+__global__ void k_perc_slots(const uint32_t* __restrict__ label, uint32_t nn,
+                             const int32_t* __restrict__ size, int32_t minsize,
+                             int32_t* __restrict__ slot, uint32_t* __restrict__ nslot,
+                             uint32_t max_slots) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nn && label[i] == i && size[i] >= minsize) {
+    const uint32_t s = atomicAdd(nslot, 1u);
+    slot[i] = s < max_slots ? (int32_t)s : -1;
+  }
+}
+
+__global__ void k_perc_cover(const uint32_t* __restrict__ label, uint32_t nn, uint32_t L,
+                             const int32_t* __restrict__ size, int32_t minsize,
+                             const int32_t* __restrict__ slot, uint32_t* __restrict__ cover,
+                             uint32_t max_slots) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const uint32_t r = label[i];
+  if (size[r] < minsize) return;
+  const int32_t s = slot[r];
+  if (s < 0 || (uint32_t)s >= max_slots) return;
+  const uint32_t N = L * L * L;
+  const uint32_t cell = i % N;
+  const uint32_t x = cell % L, y = (cell / L) % L, z = cell / (L * L);
+  const uint32_t nw = (L + 31) / 32;
+  uint32_t* cv = cover + (size_t)s * 3 * nw;
+  atomicOr(&cv[x / 32], 1u << (x % 32));
+  atomicOr(&cv[nw + y / 32], 1u << (y % 32));
+  atomicOr(&cv[2 * nw + z / 32], 1u << (z % 32));
+}
+
+__global__ void k_perc_count(const uint32_t* __restrict__ cover, const uint32_t* __restrict__ nslot,
+                             uint32_t L, uint32_t max_slots, uint32_t* __restrict__ nperc) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t ns = *nslot < max_slots ? *nslot : max_slots;
+  if (s >= ns) return;
+  const uint32_t nw = (L + 31) / 32;
+  const uint32_t* cv = cover + (size_t)s * 3 * nw;
+  bool any = false;
+  for (int ax = 0; ax < 3; ++ax) {
+    bool full = true;
+    for (uint32_t k = 0; k < nw; ++k) {
+      const uint32_t bits = (k == nw - 1 && (L % 32)) ? ((1u << (L % 32)) - 1u) : 0xFFFFFFFFu;
+      if ((cv[ax * nw + k] & bits) != bits) full = false;
+    }
+    any = any || full;
+  }
+  if (any) atomicAdd(nperc, 1u);
+}
+
+// ------------------------------------------------------ vdW evaluation
+__global__ __launch_bounds__(64) void k_vdw_eval(const float* __restrict__ dist,
+                                                 const int32_t* __restrict__ freq, int nshell,
+                                                 float r_cutoff, int N, const double* __restrict__ r,
+                                                 int precision, double* __restrict__ out) {
+  const double rr = r[blockIdx.x];
+  double e;
+  if (precision == 1)
+    e = vdw_fp32_wave(rr, dist, freq, nshell, r_cutoff, N);
+  else
+    e = vdw_energy_fp64(rr, r_cutoff, dist, freq, nshell);
+  if (threadIdx.x == 0) out[blockIdx.x] = e;
+}
+
+// ------------------------------------------------------------ launchers
+static inline dim3 grid1(size_t n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
+
+void launch_step_begin(hipStream_t st, const PointDev* pts, int npts, PointState* ps, uint8_t* sched,
+                       int nsched, const float* dist, const int32_t* freq, int vol_device,
+                       int precision) {
+  hipLaunchKernelGGL(k_step_begin, dim3(npts), dim3(64), 0, st, pts, ps, sched, nsched, dist, freq,
+                     vol_device, precision);
+}
+void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps, uint32_t* words,
+                int family) {
+  hipLaunchKernelGGL(k_eta, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts, ps, words,
+                     family);
+}
+void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
+                  uint32_t* words, const uint8_t* sched, uint32_t sub) {
+  hipLaunchKernelGGL(k_sigma, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts, ps,
+                     words, sched, sub);
+}
+void launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
+  size_t blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)blocks), dim3(kBlock), 0, st, p, n);
+}
+void launch_observe(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* words,
+                    unsigned long long* counters) {
+  hipLaunchKernelGGL(k_observe, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts, words,
+                     counters);
+}
+void launch_sw_local(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
+                     const uint32_t* words, uint32_t* parent, uint8_t* cov, uint32_t jc) {
+  hipLaunchKernelGGL(k_sw_local, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts, ps,
+                     words, parent, cov, jc);
+}
+void launch_sw_union(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* parent,
+                     const uint8_t* cov) {
+  hipLaunchKernelGGL(k_sw_union, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts,
+                     parent, cov);
+}
+void launch_sw_flip(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
+                    uint32_t* words, uint32_t* parent, uint32_t jc) {
+  hipLaunchKernelGGL(k_sw_flip, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts, ps,
+                     words, parent, jc);
+}
+void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, int32_t* size) {
+  hipLaunchKernelGGL(k_cluster_sizes, grid1(nn, kBlock), dim3(kBlock), 0, st, label, nn, size);
+}
+void launch_perc_slots(hipStream_t st, const uint32_t* label, uint32_t nn, const int32_t* size,
+                       int32_t minsize, int32_t* slot, uint32_t* nslot, uint32_t max_slots) {
+  hipLaunchKernelGGL(k_perc_slots, grid1(nn, kBlock), dim3(kBlock), 0, st, label, nn, size, minsize,
+                     slot, nslot, max_slots);
+}
+void launch_perc_cover(hipStream_t st, const uint32_t* label, uint32_t nn, uint32_t L,
+                       const int32_t* size, int32_t minsize, const int32_t* slot, uint32_t* cover,
+                       uint32_t max_slots) {
+  hipLaunchKernelGGL(k_perc_cover, grid1(nn, kBlock), dim3(kBlock), 0, st, label, nn, L, size,
+                     minsize, slot, cover, max_slots);
+}
+void launch_perc_count(hipStream_t st, const uint32_t* cover, const uint32_t* nslot, uint32_t L,
+                       uint32_t max_slots, uint32_t* nperc) {
+  hipLaunchKernelGGL(k_perc_count, grid1(max_slots, kBlock), dim3(kBlock), 0, st, cover, nslot, L,
+                     max_slots, nperc);
+}
+void launch_vdw_eval(hipStream_t st, const float* dist, const int32_t* freq, int nshell,
+                     float r_cutoff, int N, const double* r, int n_r, int precision, double* out) {
+  hipLaunchKernelGGL(k_vdw_eval, dim3(n_r), dim3(64), 0, st, dist, freq, nshell, r_cutoff, N, r,
+                     precision, out);
+}
+
+}  // namespace cvf

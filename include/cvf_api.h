@@ -1,0 +1,161 @@
+/*
+ * cvf_api.h - C ABI of the MI355X CVF bulk-water Monte Carlo engine.
+ *
+ * The reference (lcoronas/CVFBulkWater, SOURCE/gpu_water3D.cu) exposes no
+ * library interface: its hot path is the body of one executable, driven by
+ * the files input_data / input_config / distances and writing data.out /
+ * cluster.out / config.out / water3D.log.  Each entry point below names the
+ * reference routine or file-format step it replaces.  The drop-in executable
+ * cvf_water3d keeps the file contract and calls the engine only through this
+ * header.
+ *
+ * Conventions: every int-returning function returns 0 (CVF_OK) on success
+ * and a negative cvf_status on failure, with a message available from
+ * cvf_last_error(); nothing here calls exit().  The library owns all device
+ * memory.  Host buffers belong to the caller and use the reference's
+ * arm-major uint8 layout (index arm*N + cell, cell = x + L*y + L*L*z).
+ * A handle is not thread-safe and owns one HIP stream; cvf_step is
+ * asynchronous on that stream, all other calls synchronise it.
+ */
+#ifndef CVF_API_H
+#define CVF_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cvf_status {
+  CVF_OK = 0,
+  CVF_EINVAL = -1,   /* bad argument or parameter */
+  CVF_ERANGE = -2,   /* configuration value out of range (input_config) */
+  CVF_EDEVICE = -3,  /* HIP runtime error */
+  CVF_ENOMEM = -4,
+  CVF_EUNSUPPORTED = -5
+} cvf_status;
+
+enum { CVF_FP64 = 0, CVF_FP32_COMPENSATED = 1 };
+enum { CVF_VOLUME_HOST = 0, CVF_VOLUME_DEVICE = 1 };
+
+/* Mirrors input_data (gpu_water3D.cu:292-386) plus the reference's
+ * compile-time constants (lattice size 35-37, model constants 57-64). */
+typedef struct cvf_params {
+  int32_t L;                 /* cubic lattice, L % 4 == 0, 4 <= L <= 252 */
+  float P, T;                /* float, as gpu_water3D.cu:76 */
+  uint64_t seed;
+  int32_t n_metropolis;      /* Metropolis_Steps */
+  int32_t n_cluster;         /* Cluster_Steps */
+  float r_cutoff;            /* R_cutoff */
+  int32_t flag_config;       /* 0 random, 1 caller-supplied (cvf_set_config), 2 ordered */
+  int32_t flag_chessboard;   /* must be 1 */
+  int32_t device_id;
+  float J_hb, J_sig, v_hb;   /* 0.5, 0.08, 0.6 */
+  int32_t precision;         /* CVF_FP64 | CVF_FP32_COMPENSATED */
+  int32_t volume_mode;       /* CVF_VOLUME_HOST | CVF_VOLUME_DEVICE */
+} cvf_params;
+
+/* One sample of measure() (gpu_water3D.cu:2014-2160). */
+typedef struct cvf_obs {
+  int64_t step;              /* MC steps done so far */
+  double E, H, V, n_hb, m, r_cell, E_vdw;
+  int64_t nhb2;              /* 2*N_HB, exact integer */
+  int64_t n_sigma;           /* N_sigma, exact integer */
+} cvf_obs;
+
+/* One row of cluster.out (print_cluster_statistics, gpu_water3D.cu:2677-2721). */
+typedef struct cvf_cstat {
+  int32_t largest, second, n_perc, n_clusters;
+  float mean, var;
+} cvf_cstat;
+
+typedef struct cvf_sim cvf_sim;
+
+/* Defaults of SOURCE/input_data with L = 32. */
+void cvf_params_default(cvf_params *p);
+
+/* Replaces main()'s set-up (gpu_water3D.cu:449-1170): validates the
+ * parameters, builds the distance-shell table (865-915), initialises sigma
+ * (812-826) and the chessboard cubes (639-720) and the volume chain
+ * (150, 481, 1128). */
+int cvf_create(const cvf_params *p, cvf_sim **out);
+
+/* n independent state points of uneven L advanced by one launch sequence;
+ * they must agree on n_metropolis, n_cluster, precision and device_id. */
+int cvf_create_batch(const cvf_params *pts, int32_t n, cvf_sim **out);
+
+/* input_config read (gpu_water3D.cu:730-800): sigma in [0,5], eta in {0,1};
+ * cube states are re-derived from eta by the reference's decision tree
+ * (3314-3371) and the eta bits rewritten from the solution table.  Returns
+ * the number of cubes whose eta was not a solution (>= 0), or < 0. */
+int cvf_set_config(cvf_sim *s, int32_t pt, const uint8_t *sigma6N, const uint8_t *eta6N);
+
+/* config.out write (gpu_water3D.cu:1296-1305). */
+int cvf_get_config(cvf_sim *s, int32_t pt, uint8_t *sigma6N, uint8_t *eta6N);
+
+/* n_steps calls of MonteCarlo_Step (gpu_water3D.cu:1346-1733): eta cube
+ * sweep, volume move, n_metropolis x 12 (arm, parity) half-lattice sigma
+ * sweeps, n_cluster Swendsen-Wang steps.  Asynchronous. */
+int cvf_step(cvf_sim *s, int64_t n_steps);
+
+/* measure() (gpu_water3D.cu:2014-2160) for state point pt. */
+int cvf_observe(cvf_sim *s, int32_t pt, cvf_obs *o);
+
+/* print_cluster_statistics (gpu_water3D.cu:2677-2780) on the labels of the
+ * last Swendsen-Wang step (all zero before the first, as in the reference). */
+int cvf_cluster_stats(cvf_sim *s, int32_t pt, cvf_cstat *c);
+
+/* dev_label D2H (gpu_water3D.cu:1235): canonical minimum-node labels,
+ * node = arm*N + cell. */
+int cvf_get_labels(cvf_sim *s, int32_t pt, int32_t *labels6N);
+
+/* Volume-chain state (r_cell, V, E_vdW: gpu_water3D.cu:77, 150-151). */
+int cvf_get_volume(cvf_sim *s, int32_t pt, double *r_cell, double *V, double *E_vdw);
+
+/* Distance-shell table (gpu_water3D.cu:865-932; the `distances` file). */
+int cvf_get_shells(cvf_sim *s, int32_t pt, float *dist, int32_t *freq, int32_t cap,
+                   int32_t *nshell);
+
+/* Replaces the Flag_distances=1 read path (gpu_water3D.cu:934-985). */
+int cvf_set_shells(cvf_sim *s, int32_t pt, const float *dist, const int32_t *freq,
+                   int32_t nshell);
+
+/* Number of state points, lattice size of one, MC steps done. */
+int cvf_info(cvf_sim *s, int32_t pt, int32_t *n_points, int32_t *L, int64_t *steps);
+
+/* The handle's HIP stream (hipStream_t), for event timing by callers. */
+int cvf_get_stream(cvf_sim *s, void **stream);
+
+int cvf_sync(cvf_sim *s);
+const char *cvf_last_error(void);
+void cvf_destroy(cvf_sim *s);
+
+/* Kernel classes of one MC step (profiling). */
+enum {
+  CVF_KCLASS_BEGIN = 0,   /* step counter, schedule, device volume move */
+  CVF_KCLASS_ETA = 1,     /* cube (eta) Metropolis, per family */
+  CVF_KCLASS_SIGMA = 2,   /* sigma half-lattice sweep */
+  CVF_KCLASS_SW_LOCAL = 3,
+  CVF_KCLASS_SW_UNION = 4,
+  CVF_KCLASS_SW_FLIP = 5,
+  CVF_KCLASS_COUNT = 6
+};
+
+/* Instrumented cvf_step for measurement (no reference counterpart; the
+ * reference's timers are commented out, gpu_water3D.cu:1180-1192): runs
+ * n_steps MC steps with individual launches, a HIP event after each, and
+ * returns the mean duration in ms of one launch of each kernel class and
+ * the launch counts.  Needs the device volume mode. */
+int cvf_profile_steps(cvf_sim *s, int64_t n_steps, double *kernel_ms, int64_t *launches,
+                      int32_t n_classes);
+
+/* E_vdW of a shell table at r (calculate_energy_vdW, 1784-1805), evaluated
+ * on the device: FP64, or FP32 terms with a compensated wavefront sum. */
+int cvf_vdw_energy(int32_t device_id, int32_t L, const float *dist, const int32_t *freq,
+                   int32_t nshell, float r_cutoff, const double *r, int32_t n_r,
+                   int32_t precision, double *energy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

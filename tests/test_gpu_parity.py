@@ -1,0 +1,132 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact sigma/eta words and Swendsen-Wang labels, exact integer
+N_HB / N_sigma, identical volume chains (r_cell, V, E_vdW compared with ==),
+E/H/V within 1e-12 relative (the oracle sums per cell like measure(), the
+engine from integer totals).
+"""
+import numpy as np
+import pytest
+
+from cvfbulkwater_amd import FP32_COMPENSATED, VOLUME_DEVICE, VOLUME_HOST, Params, Simulation
+from oracle import py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(L, T=0.815, P=0.45, seed=44648604, nm=1, nc=0, flag_config=0, volume_mode=VOLUME_HOST):
+    eng = Simulation(Params(L=L, T=T, P=P, seed=seed, n_metropolis=nm, n_cluster=nc,
+                            flag_config=flag_config, volume_mode=volume_mode))
+    ora = O.OracleSim(L=L, T=T, P=P, seed=seed, n_metropolis=nm, n_cluster=nc,
+                      flag_config=flag_config)
+    return eng, ora
+
+
+def assert_same_state(eng, ora, what=""):
+    s1, e1 = eng.get_config()
+    s2, e2 = ora.get_config()
+    if not np.array_equal(s1, s2):
+        bad = np.flatnonzero(s1 != s2)
+        raise AssertionError(f"{what}: sigma differs at {bad.size} sites, first {bad[:8]}")
+    if not np.array_equal(e1, e2):
+        bad = np.flatnonzero(e1 != e2)
+        raise AssertionError(f"{what}: eta differs at {bad.size} sites, first {bad[:8]}")
+
+
+def assert_same_obs(eng, ora, what=""):
+    a, b = eng.observe(), ora.measure()
+    assert a.nhb2 == b.nhb2, what
+    assert a.n_sigma == b.n_sigma, what
+    assert a.r_cell == b.r_cell, what
+    for k in ("E", "H", "V", "n_hb", "m"):
+        x, y = getattr(a, k), getattr(b, k)
+        assert abs(x - y) <= 1e-12 * max(1.0, abs(y)), (what, k, x, y)
+
+
+@pytest.mark.parametrize("flag_config", [0, 2])
+def test_initial_state_matches(flag_config):
+    eng, ora = pair(8, flag_config=flag_config)
+    assert_same_state(eng, ora, "init")
+    assert_same_obs(eng, ora, "init")
+
+
+def test_config1_L8_every_step():
+    """Config 1 shape: L=8, T=0.815, P=0.45, 1 Metropolis + 1 volume move,
+    1000 + 2000 steps; state compared after every step."""
+    eng, ora = pair(8)
+    for t in range(3000):
+        eng.step(1)
+        ora.step(1)
+        if t < 300 or t % 10 == 0:
+            assert_same_state(eng, ora, f"step {t}")
+    assert_same_obs(eng, ora, "end")
+    assert eng.volume() == ora.volume()
+
+
+def test_config2_L32_first_steps():
+    """Config 2 shape (L=32), first 200 steps compared every step."""
+    eng, ora = pair(32)
+    for t in range(200):
+        eng.step(1)
+        ora.step(1)
+        assert_same_state(eng, ora, f"step {t}")
+        if t % 20 == 0:
+            assert_same_obs(eng, ora, f"step {t}")
+    assert eng.volume() == ora.volume()
+
+
+@pytest.mark.parametrize("L,T", [(8, 0.70), (16, 0.70), (12, 0.55)])
+def test_cluster_form_labels_every_step(L, T):
+    """Config 3 shape (1 Metropolis + 1 SW per step) at small L: labels,
+    sigma and eta identical after every step."""
+    eng, ora = pair(L, T=T, nc=1)
+    for t in range(60):
+        eng.step(1)
+        ora.step(1)
+        assert np.array_equal(eng.labels(), ora.labels()), f"labels step {t}"
+        assert_same_state(eng, ora, f"step {t}")
+        if t % 10 == 0:
+            a = eng.cluster_stats()
+            b = ora.cluster_stats()
+            assert (a.largest, a.second, a.n_perc, a.n_clusters) == tuple(b[:4]), t
+            assert a.mean == b[4] and a.var == b[5], t
+    assert_same_obs(eng, ora)
+
+
+def test_cluster_form_L32_first_steps():
+    eng, ora = pair(32, T=0.70, nc=1)
+    for t in range(20):
+        eng.step(1)
+        ora.step(1)
+        assert np.array_equal(eng.labels(), ora.labels()), f"labels step {t}"
+        assert_same_state(eng, ora, f"step {t}")
+    a, b = eng.cluster_stats(), ora.cluster_stats()
+    assert (a.largest, a.second, a.n_perc, a.n_clusters, a.mean, a.var) == tuple(b)
+
+
+def test_device_volume_chain_equals_host_chain():
+    """The device volume move (batched form) and the host one (plain form)
+    share the project exp/log: identical r_cell chains."""
+    eng_h, ora = pair(16)
+    eng_d = Simulation(Params(L=16, volume_mode=VOLUME_DEVICE))
+    for _ in range(5):
+        eng_h.step(200)
+        eng_d.step(200)
+        ora.step(200)
+        assert eng_h.volume() == eng_d.volume() == ora.volume()
+
+
+def test_config_roundtrip_and_normalisation():
+    rng = np.random.default_rng(5)
+    L, N = 8, 512
+    eng, ora = pair(L)
+    s = rng.integers(0, 6, 6 * N, dtype=np.uint8)
+    e = rng.integers(0, 2, 6 * N, dtype=np.uint8)
+    nb1 = eng.set_config(s, e)
+    nb2 = ora.set_config(s, e)
+    assert nb1 == nb2 > 0
+    assert_same_state(eng, ora, "after set_config")
+    for t in range(50):
+        eng.step(1)
+        ora.step(1)
+    assert_same_state(eng, ora, "after 50 steps")
