@@ -431,8 +431,10 @@ int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
   }
   HIPC(hipMemcpy(s->pst.p, st.data(), n * sizeof(PointState), hipMemcpyHostToDevice));
   // block tables
-  auto th = make_table(s->ph, [](const PointHost& P) { return P.N / 2; });
-  auto tc = make_table(s->ph, [](const PointHost& P) { return (P.L / 2) * (P.L / 2) * (P.L / 2); });
+  // sigma: one thread per pair of active cells (N/4); eta: one thread per
+  // pair of cubes of one family (N/16)
+  auto th = make_table(s->ph, [](const PointHost& P) { return P.N / 4; });
+  auto tc = make_table(s->ph, [](const PointHost& P) { return (P.L / 2) * (P.L / 2) * (P.L / 4); });
   auto tl = make_table(s->ph, [](const PointHost& P) { return P.N; });
   HIPC(s->tab_half.alloc(th.size()));
   HIPC(s->tab_cube.alloc(tc.size()));

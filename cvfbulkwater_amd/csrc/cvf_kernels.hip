@@ -50,26 +50,55 @@ __device__ __forceinline__ void two_sum_pair(float& s, float& c, float s2, float
   c = c + c2 + err;
 }
 
-// Per-molecule vdW sum with FP32 terms, lane-local Neumaier sums and a
-// compensated butterfly across the wavefront; all lanes get the result.
+// Per-molecule vdW sum with FP32 arithmetic.  Rounding r*d_k to a float
+// alone would cost 6 x 6e-8 relative in (r d)^-6, so each term is carried as
+// an unevaluated float pair (hi + lo) built with fma-based error-free
+// transformations: x = r d_k, x^6, u = x^-6 and g = u (u - 1) - shift are
+// formed from their FP32 heads plus first-order FP32 corrections.  The pairs
+// are summed with Neumaier's compensated sum per lane and a compensated
+// butterfly across the wavefront; the result is widened to FP64 once and
+// scaled by N (the shell weight f_k / N = count_k / 2 is a small integer).
 __device__ double vdw_fp32_wave(double r, const float* dist, const int32_t* freq, int nshell,
                                 float r_cutoff, int N) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
-  float tc = 1.0f / r_cutoff;
-  tc = tc * tc * tc;
-  const float shift = tc * tc * tc * tc - tc * tc;
+  const double rc6 = 1.0 / ((double)r_cutoff * r_cutoff * r_cutoff * r_cutoff * r_cutoff * r_cutoff);
+  const double shd = rc6 * rc6 - rc6;
+  const float sh_hi = (float)shd, sh_lo = (float)(shd - (double)sh_hi);
+  const float r_hi = (float)r, r_lo = (float)(r - (double)r_hi);
   float s = 0.f, c = 0.f;
   for (int k = lane; k < nshell; k += 64) {
-    const double rd = r * (double)dist[k];
-    if (rd < (double)r_cutoff) {
-      const float x = (float)rd;
-      float t = 1.0f / x;
-      t = t * t * t;
-      const float att = t * t;
-      const float rep = att * att;
-      const float g = rep - att - shift;
-      two_sum_acc(s, c, (float)(freq[k] / N) * g);
+    const float d = dist[k];
+    if (r * (double)d < (double)r_cutoff) {
+      // x = r d as x_hi + x_lo
+      const float x_hi = r_hi * d;
+      const float x_lo = fmaf(r_hi, d, -x_hi) + r_lo * d;
+      // x^2, x^3, x^6 with first-order tails
+      const float x2 = x_hi * x_hi;
+      const float x2e = fmaf(x_hi, x_hi, -x2) + 2.f * x_hi * x_lo;
+      const float x3 = x2 * x_hi;
+      const float x3e = fmaf(x2, x_hi, -x3) + x2e * x_hi + x2 * x_lo;
+      const float x6 = x3 * x3;
+      const float x6e = fmaf(x3, x3, -x6) + 2.f * x3 * x3e;
+      // u = 1/x^6 = u0 + u1
+      const float u0 = 1.0f / x6;
+      const float u1 = (fmaf(-x6, u0, 1.0f) - x6e * u0) * u0;
+      // g = u (u - 1) - shift
+      const float v0 = u0 - 1.0f;
+      const float bp = v0 - u0;
+      const float ev = (u0 - (v0 - bp)) + (-1.0f - bp);
+      const float p0 = u0 * v0;
+      const float p0e = fmaf(u0, v0, -p0);
+      const float g_hi = p0 - sh_hi;
+      const float gb = g_hi - p0;
+      const float ge = (p0 - (g_hi - gb)) + (-sh_hi - gb);
+      const float g_lo = ge + p0e + u0 * ev + u1 * (u0 + v0) - sh_lo;
+      // weight f_k / N = count_k / 2
+      const float w = (float)(freq[k] / N);
+      const float t_hi = w * g_hi;
+      const float t_lo = fmaf(w, g_hi, -t_hi) + w * g_lo;
+      two_sum_acc(s, c, t_hi);
+      c += t_lo;
     }
   }
 #pragma unroll
@@ -125,36 +154,18 @@ __global__ __launch_bounds__(64) void k_step_begin(const PointDev* __restrict__ 
 // thread updates its 8 corner words without atomics.  Cube states of both
 // families are independent given sigma, so running the families one after
 // the other equals the reference's simultaneous update.
-__global__ __launch_bounds__(kBlock) void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb,
-                                                const PointDev* __restrict__ pts,
-                                                const PointState* __restrict__ ps,
-                                                uint32_t* __restrict__ words, int family) {
-  const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
-  if (e.point < 0) return;
-  const PointDev& P = pts[e.point];
-  __shared__ uint64_t thr[9];
-  if (threadIdx.x < 9) thr[threadIdx.x] = P.thr_eta[threadIdx.x];
-  __syncthreads();
-  const uint32_t hL = P.hL, L = P.L;
-  const uint32_t t = e.start + threadIdx.x;
-  if (t >= hL * hL * hL) return;
-  const uint64_t step = load_step(ps, e.point);
-  const uint32_t r1 = fdiv(t, P.divH);
-  const uint32_t i = t - r1 * hL;
-  const uint32_t k = fdiv(r1, P.divH);
-  const uint32_t j = r1 - k * hL;
-  const uint32_t ox = 2 * i + family, oy = 2 * j + family, oz = 2 * k + family;
+__device__ __forceinline__ void eta_cube(uint32_t* __restrict__ W, const uint64_t* thr, uint32_t L,
+                                         uint32_t ox, uint32_t oy, uint32_t oz, uint32_t ru,
+                                         uint32_t ra) {
   const uint32_t x1 = ox + 1 == L ? 0 : ox + 1;
   const uint32_t y1 = oy + 1 == L ? 0 : oy + 1;
   const uint32_t z1 = oz + 1 == L ? 0 : oz + 1;
-  uint32_t* W = words + P.woff;
-  uint32_t idx[8];
+  uint32_t idx[8], w[8];
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
     const uint32_t x = (b & 1) ? x1 : ox, y = (b & 2) ? y1 : oy, z = (b & 4) ? z1 : oz;
     idx[b] = x + L * (y + L * z);
   }
-  uint32_t w[8];
 #pragma unroll
   for (int b = 0; b < 8; ++b) w[b] = W[idx[b]];
   uint32_t D = 0, E = 0;
@@ -164,11 +175,9 @@ __global__ __launch_bounds__(kBlock) void k_eta(const BlockEnt* __restrict__ tab
     D |= (uint32_t)(sigma_of(w[A], 2 * ax) == sigma_of(w[B], 2 * ax + 1)) << jj;
     E |= eta_of(w[A], 2 * ax) << jj;
   }
-  const uint32_t cube_ref = i + hL * j + hL * hL * oz;  // gpu_water3D.cu:3405
-  const U4 r = draw(P.seed, kStEta, cube_ref, step, 0);
-  const uint32_t M = sol_mask(uniform_int(r.x, 9));
+  const uint32_t M = sol_mask(uniform_int(ru, 9));
   const int dn = popc32(M & D) - popc32(E & D);
-  if ((uint64_t)r.y < thr[dn + 4] && M != E) {
+  if ((uint64_t)ra < thr[dn + 4] && M != E) {
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
@@ -182,12 +191,56 @@ __global__ __launch_bounds__(kBlock) void k_eta(const BlockEnt* __restrict__ tab
   }
 }
 
+// One thread per pair of cubes of the family with origins (ox,oy,oz) and
+// (ox,oy,oz+L/2): both use one Philox block (words x,y and z,w).
+__global__ __launch_bounds__(kBlock) void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb,
+                                                const PointDev* __restrict__ pts,
+                                                const PointState* __restrict__ ps,
+                                                uint32_t* __restrict__ words, int family) {
+  const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
+  if (e.point < 0) return;
+  const PointDev& P = pts[e.point];
+  __shared__ uint64_t thr[9];
+  if (threadIdx.x < 9) thr[threadIdx.x] = P.thr_eta[threadIdx.x];
+  __syncthreads();
+  const uint32_t hL = P.hL, L = P.L;
+  const uint32_t t = e.start + threadIdx.x;
+  if (t >= hL * hL * (L / 4)) return;
+  const uint64_t step = load_step(ps, e.point);
+  const uint32_t r1 = fdiv(t, P.divH);
+  const uint32_t i = t - r1 * hL;
+  const uint32_t k = fdiv(r1, P.divH);
+  const uint32_t j = r1 - k * hL;
+  const uint32_t ox = 2 * i + family, oy = 2 * j + family, oz = 2 * k + family;
+  const uint32_t cube_ref = i + hL * j + hL * hL * oz;  // gpu_water3D.cu:3405
+  const U4 r = draw(P.seed, kStEta, cube_ref, step, 0);
+  uint32_t* W = words + P.woff;
+  eta_cube(W, thr, L, ox, oy, oz, r.x, r.y);
+  eta_cube(W, thr, L, ox, oy, oz + L / 2, r.z, r.w);
+}
+
 // --------------------------------------------------------- sigma half-sweep
 // gpu_update (gpu_water3D.cu:1890-2008), chessboard branch: arm `a` of the
 // cells whose coordinate along a's axis is even (indices = 1) or odd
 // (indices = 0).  A cell's new sigma_a depends on its own word and on the
 // facing arm of the neighbour across arm a; that neighbour has the other
 // parity along the axis, so no word read in a launch is written in it.
+__device__ __forceinline__ void sigma_update(uint32_t* __restrict__ W, const uint64_t* thr, uint32_t c,
+                                             uint32_t nbc, int arm, uint32_t ru, uint32_t ra) {
+  const uint32_t w = W[c];
+  const uint32_t wn = W[nbc];
+  const uint32_t sa = sigma_of(w, arm);
+  const uint32_t sp = uniform_int(ru, 6);
+  if (sp == sa) return;
+  const uint32_t snb = sigma_of(wn, arm ^ 1);
+  const int h = eta_of(w, arm) ? ((int)(sp == snb) - (int)(sa == snb)) : 0;
+  const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
+  if ((uint64_t)ra < thr[(h + 1) * 11 + d + 5]) W[c] = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
+}
+
+// One thread per pair of active cells (x,y,z) and (x,y,z+L/2), z < L/2: both
+// have the same parity along every axis (L/2 is even) and share one Philox
+// block (words x,y for the lower cell, z,w for the upper).
 __global__ __launch_bounds__(kBlock) void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb,
                                                   const PointDev* __restrict__ pts,
                                                   const PointState* __restrict__ ps,
@@ -199,15 +252,15 @@ __global__ __launch_bounds__(kBlock) void k_sigma(const BlockEnt* __restrict__ t
   __shared__ uint64_t thr[33];
   if (threadIdx.x < 33) thr[threadIdx.x] = P.thr_sigma[threadIdx.x];
   __syncthreads();
-  const uint32_t L = P.L, hL = P.hL;
+  const uint32_t L = P.L, hL = P.hL, N = P.N;
   const uint32_t t = e.start + threadIdx.x;
-  if (t >= P.N / 2) return;
+  if (t >= N / 4) return;
   const uint64_t step = load_step(ps, e.point);
   const uint32_t sc = sched[P.sched_off + sub];
   const int arm = (int)(sc & 7u);
   const uint32_t q = (sc >> 3) ? 0u : 1u;  // kept parity along the axis
   const int ax = arm >> 1;
-  uint32_t x, y, z;
+  uint32_t x, y, z;  // lower cell, z < L/2
   if (ax == 0) {
     const uint32_t r1 = fdiv(t, P.divH);
     x = 2 * (t - r1 * hL) + q;
@@ -226,24 +279,30 @@ __global__ __launch_bounds__(kBlock) void k_sigma(const BlockEnt* __restrict__ t
     z = 2 * hz + q;
   }
   const uint32_t c = x + L * (y + L * z);
-  const uint32_t coord = ax == 0 ? x : (ax == 1 ? y : z);
-  const uint32_t stride = ax == 0 ? 1u : (ax == 1 ? L : L * L);
-  uint32_t nbc;
-  if (arm & 1)
-    nbc = coord == 0 ? c + (L - 1) * stride : c - stride;
-  else
-    nbc = coord == L - 1 ? c - (L - 1) * stride : c + stride;
+  const uint32_t c2 = c + N / 2;
+  uint32_t nb1, nb2;
+  if (ax == 2) {
+    const uint32_t LL = L * L, z2 = z + hL;
+    if (arm & 1) {
+      nb1 = z == 0 ? c + (L - 1) * LL : c - LL;
+      nb2 = c2 - LL;  // z2 >= L/2 > 0
+    } else {
+      nb1 = c + LL;  // z < L/2 <= L-1
+      nb2 = z2 == L - 1 ? c2 - (L - 1) * LL : c2 + LL;
+    }
+  } else {
+    const uint32_t coord = ax == 0 ? x : y;
+    const uint32_t stride = ax == 0 ? 1u : L;
+    if (arm & 1)
+      nb1 = coord == 0 ? c + (L - 1) * stride : c - stride;
+    else
+      nb1 = coord == L - 1 ? c - (L - 1) * stride : c + stride;
+    nb2 = nb1 + N / 2;
+  }
   uint32_t* W = words + P.woff;
-  const uint32_t w = W[c];
-  const uint32_t wn = W[nbc];
   const U4 r = draw(P.seed, kStSigma, c, step, sub);
-  const uint32_t sa = sigma_of(w, arm);
-  const uint32_t sp = uniform_int(r.x, 6);
-  if (sp == sa) return;
-  const uint32_t snb = sigma_of(wn, arm ^ 1);
-  const int h = eta_of(w, arm) ? ((int)(sp == snb) - (int)(sa == snb)) : 0;
-  const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
-  if ((uint64_t)r.y < thr[(h + 1) * 11 + d + 5]) W[c] = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
+  sigma_update(W, thr, c, nb1, arm, r.x, r.y);
+  sigma_update(W, thr, c2, nb2, arm, r.z, r.w);
 }
 
 // ----------------------------------------------------------- observables

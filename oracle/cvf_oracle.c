@@ -388,16 +388,19 @@ void ora_eta_sweep(ora_sim *s, uint64_t step) {
   int N = s->N, NC = s->NC;
   for (int cube = 0; cube < NC; ++cube) {
     int old = s->state[cube];
+    /* cubes with origin z and z + L/2 share one Philox block: the lower
+     * cube uses words 0,1, the upper one words 2,3 (site = lower cube) */
+    int upper = cube >= NC / 2;
     uint32_t r[4];
-    draw(s->p.seed, ORA_ST_ETA, (uint32_t)cube, step, 0, r);
-    int nw = (int)uniform_int(r[0], 9);
+    draw(s->p.seed, ORA_ST_ETA, (uint32_t)(upper ? cube - NC / 2 : cube), step, 0, r);
+    int nw = (int)uniform_int(r[upper ? 2 : 0], 9);
     int dn = 0;
     for (int j = 0; j < EDGES; ++j) {
       int v0 = s->vert[cube + j * NC], v1 = s->vert[cube + j * NC + NC * EDGES];
       int a0 = edge_arm0(j);
       dn += (Sol[nw][j] - Sol[old][j]) * (s->s[a0 * N + v0] == s->s[(a0 + 1) * N + v1]);
     }
-    if (!((uint64_t)r[1] < s->thr_eta[dn + 4])) nw = old;
+    if (!((uint64_t)r[upper ? 3 : 1] < s->thr_eta[dn + 4])) nw = old;
     s->state[cube] = nw;
     write_cube_edges(s, cube);
   }
@@ -426,9 +429,12 @@ void ora_sigma_halfsweep(ora_sim *s, uint64_t step, uint32_t sub, int arm, int i
     int nb = neighbor(L, c, arm);
     int nbspin = s->s[(arm ^ 1) * N + nb];
     int old = s->s[arm * N + c];
+    /* cells (x,y,z) and (x,y,z+L/2) are active together (L/2 is even) and
+     * share one Philox block: words 0,1 for the lower, 2,3 for the upper */
+    int upper = z >= L / 2;
     uint32_t r[4];
-    draw(s->p.seed, ORA_ST_SIGMA, (uint32_t)c, step, sub, r);
-    int nw = (int)uniform_int(r[0], 6);
+    draw(s->p.seed, ORA_ST_SIGMA, (uint32_t)(upper ? c - N / 2 : c), step, sub, r);
+    int nw = (int)uniform_int(r[upper ? 2 : 0], 6);
     if (nw == old) continue;
     int h = s->eta[arm * N + c] * ((nw == nbspin) - (old == nbspin));
     int d = 0;
@@ -437,7 +443,7 @@ void ora_sigma_halfsweep(ora_sim *s, uint64_t step, uint32_t sub, int arm, int i
       int sk = s->s[k * N + c];
       d += (nw == sk) - (old == sk);
     }
-    if ((uint64_t)r[1] < s->thr_sigma[(h + 1) * 11 + (d + 5)]) s->s[arm * N + c] = (uint8_t)nw;
+    if ((uint64_t)r[upper ? 3 : 1] < s->thr_sigma[(h + 1) * 11 + (d + 5)]) s->s[arm * N + c] = (uint8_t)nw;
   }
 }
 

@@ -1,0 +1,167 @@
+"""GPU tests of the physics and of the batched / precision forms."""
+import io
+import json
+import lzma
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from cvfbulkwater_amd import (FP32_COMPENSATED, FP64, VOLUME_DEVICE, VOLUME_HOST, Params, Simulation,
+                              vdw_energy)
+from oracle import py as O
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+REF = json.loads((GOLD / "reference_values.json").read_text())
+
+
+def load_config(name):
+    a = np.loadtxt(io.StringIO(lzma.decompress((GOLD / name).read_bytes()).decode()), dtype=np.int64)
+    return a[:, :6].T.astype(np.uint8).reshape(-1), a[:, 6:].T.astype(np.uint8).reshape(-1)
+
+
+def block_mean_se(x, nblocks=10):
+    x = np.asarray(x, float)
+    b = x[: len(x) // nblocks * nblocks].reshape(nblocks, -1).mean(axis=1)
+    return x.mean(), b.std(ddof=1) / math.sqrt(nblocks)
+
+
+def test_config2_long_run_matches_reference_data_out():
+    """Config 2: L=32, T=0.815, P=0.45 from the shipped input_config,
+    10^4 + 10^5 steps sampled every 10 (the reference's own run).  Long-run
+    density, enthalpy and N_HB within 2 sigma of SOURCE/data.out (sigma =
+    combined 10-block standard errors)."""
+    s, e = load_config("input_config.xz")
+    sim = Simulation(Params(L=32, T=0.815, P=0.45, seed=44648604, flag_config=1))
+    assert sim.set_config(s, e) == 0
+    sim.step(10000)
+    rows = []
+    for i in range(100000 // 10):
+        sim.step(1 if i == 0 else 10)
+        o = sim.observe()
+        rows.append((o.E, o.H, o.V, o.n_hb, o.m, o.r_cell))
+    ours = np.array(rows)
+    ref = np.loadtxt(io.StringIO(lzma.decompress((GOLD / "data.out.xz").read_bytes()).decode()))[:, 1:]
+    N = 32 ** 3
+    report = {}
+    for name, f in (("rho", lambda a: N / a[:, 2]), ("H/N", lambda a: a[:, 1] / N),
+                    ("N_HB/N", lambda a: a[:, 3] / N)):
+        m1, s1 = block_mean_se(f(ours))
+        m2, s2 = block_mean_se(f(ref))
+        z = (m1 - m2) / math.hypot(s1, s2)
+        report[name] = (m1, m2, z)
+        assert abs(z) < 2.0, (name, m1, m2, s1, s2)
+    print(report)
+
+
+def test_config3_cluster_form_density_vs_published_isobar():
+    """Config 3: L=32, T=0.70, P=0.45, 1 Metropolis + 1 Swendsen-Wang per
+    step, 2000 + 5000 steps: <rho> vs the published L=32 isobar value."""
+    sim = Simulation(Params(L=32, T=0.70, P=0.45, seed=7, n_cluster=1))
+    sim.step(2000)
+    rho = []
+    for i in range(500):
+        sim.step(10)
+        rho.append(32 ** 3 / sim.observe().V)
+    T_pub = {round(r[0], 4): r for r in REF["published_density_isobars_L32"]["0.45"]}
+    _, want, err = T_pub[0.7]
+    m, se = block_mean_se(rho)
+    z = (m - want) / math.hypot(se, err)
+    assert abs(z) < 2.5, (m, se, want, err)
+    c = sim.cluster_stats()
+    assert c.n_clusters > 1 and c.largest >= c.second > 0
+
+
+def test_cluster_out_degenerate_row_engine():
+    sim = Simulation(Params(L=32))
+    sim.step(3)
+    c = sim.cluster_stats()
+    line = "%d %d %d %d %f %f %d" % (0, c.largest, c.second, c.n_perc, c.mean, c.var, c.n_clusters)
+    assert line == REF["cluster_out_first_row"]
+
+
+def test_shipped_configs_through_engine():
+    for name, key in (("input_config.xz", "input_config_counts"), ("config.out.xz", "config_out_counts")):
+        s, e = load_config(name)
+        sim = Simulation(Params(L=32, flag_config=1))
+        assert sim.set_config(s, e) == 0
+        o = sim.observe()
+        assert (o.nhb2, o.n_sigma) == (REF[key]["nhb2"], REF[key]["n_sigma"])
+
+
+def batch_points(Ls, precision=FP64, nc=0):
+    return [Params(L=L, T=0.55 + 0.05 * i, P=-0.1 + 0.1 * i, seed=1000 + 7 * i, n_cluster=nc,
+                   precision=precision, volume_mode=VOLUME_DEVICE) for i, L in enumerate(Ls)]
+
+
+@pytest.mark.parametrize("nc", [0, 1])
+def test_batch_equals_sequential_single_runs(nc):
+    """A batch of uneven-L state points advanced in one launch sequence is
+    bit-identical to the same points run one at a time."""
+    Ls = [8, 16, 12, 24, 8, 20]
+    pts = batch_points(Ls, nc=nc)
+    batch = Simulation(pts)
+    singles = [Simulation(p) for p in pts]
+    for _ in range(3):
+        batch.step(40)
+        for s in singles:
+            s.step(40)
+        for i, s in enumerate(singles):
+            a, b = batch.get_config(i), s.get_config()
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
+            assert batch.volume(i) == s.volume()
+            if nc:
+                assert np.array_equal(batch.labels(i), s.labels())
+            assert batch.observe(i).nhb2 == s.observe().nhb2
+
+
+def test_batch_points_match_oracle():
+    pts = batch_points([8, 12, 16], nc=1)
+    batch = Simulation(pts)
+    oras = [O.OracleSim(L=p.L, T=p.T, P=p.P, seed=p.seed, n_cluster=1) for p in pts]
+    batch.step(30)
+    for i, o in enumerate(oras):
+        o.step(30)
+        a = batch.get_config(i)
+        b = o.get_config()
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert np.array_equal(batch.labels(i), o.labels())
+        assert batch.volume(i) == o.volume()
+
+
+@pytest.mark.parametrize("L", [32, 96])
+def test_fp32_compensated_vdw_within_1e6(L):
+    d, f = O.shell_table(L, 6.0)
+    r = np.linspace(0.92, 1.35, 2000)
+    e64 = vdw_energy(L, d, f, 6.0, r, FP64)
+    e32 = vdw_energy(L, d, f, 6.0, r, FP32_COMPENSATED)
+    ref = np.array([O.vdw_energy(x, 6.0, d, f) for x in r])
+    assert np.array_equal(e64, ref)  # device FP64 == host FP64, bit for bit
+    rel = np.abs(e32 - e64) / np.abs(e64)
+    assert rel.max() <= 1e-6, rel.max()
+
+
+def test_precision_form_keeps_lattice_trajectory():
+    """FP32-compensated vdW changes only the volume chain: sigma/eta
+    trajectories are identical to the FP64 run from the same seeds."""
+    Ls = [16, 24]
+    a = Simulation(batch_points(Ls, FP64))
+    b = Simulation(batch_points(Ls, FP32_COMPENSATED))
+    a.step(300)
+    b.step(300)
+    for i in range(len(Ls)):
+        x, y = a.get_config(i), b.get_config(i)
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+        ra, rb = a.volume(i)[0], b.volume(i)[0]
+        assert abs(ra - rb) < 0.05
+
+
+def test_host_and_device_volume_modes_agree():
+    a = Simulation(Params(L=24, volume_mode=VOLUME_HOST))
+    b = Simulation(Params(L=24, volume_mode=VOLUME_DEVICE))
+    a.step(500)
+    b.step(500)
+    assert a.volume() == b.volume()
+    assert a.observe().E == b.observe().E
