@@ -17,7 +17,7 @@ production shape).  value = site-updates/s of the batched form
 The roofline is for the dominant kernel (k_sigma, the half-lattice sigma
 sweep): algorithmic bytes per launch = (cells/2) x 12 B (4 B own word read,
 4 B neighbour word read, 4 B write), divided by the mean launch duration
-measured with HIP events on the engine's stream.
+recorded by the dispatch itself (hipExtLaunchKernelGGL start/stop events).
 
 Inputs are synthetic (random initial lattices); nothing is read from disk.
 """
@@ -117,6 +117,25 @@ def cpu_baseline(budget_s: float) -> dict:
                       + f" MC steps), {dt:.1f} s on {platform.processor() or platform.machine()}"}
 
 
+def calibrate_copy(mib: float, reps: int = 10) -> int:
+    """hipMemcpyDtoD of a known size with the runtime's copy kernel: under
+    rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch give the scale
+    between the counters and bytes actually moved."""
+    hip = C.CDLL("libamdhip64.so")
+    n = int(mib * 1024 * 1024)
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpyDtoD(b, a, C.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibrate_copy_bytes": n, "reps": reps}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,7 +147,12 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["batch", "plain"], default=None,
                     help="run one leg only (for profiling)")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
                                    "(state points shard across GPUs with no exchange, "
@@ -147,7 +171,9 @@ def main() -> int:
         prof = sim.profile_steps(args.profile_steps)
         sim.close()
         sig_ms = prof["sigma"][0]
-        sig_bytes = (cells // 2) * SIGMA_BYTES_PER_ACTIVE_CELL
+        # one launch sweeps one group's points; the groups split the cells
+        launches_per_step = prof["sigma"][1] / args.profile_steps
+        sig_bytes = (cells // 2) * SIGMA_BYTES_PER_ACTIVE_CELL * 12 / launches_per_step
         achieved = sig_bytes / (sig_ms * 1e-3) / 1e9
         out.update(batch_ms=ms, batch_cells=cells, batch_rate=cells / (ms * 1e-3), prof=prof,
                    sig_ms=sig_ms, sig_bytes=sig_bytes, achieved=achieved)

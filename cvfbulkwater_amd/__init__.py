@@ -57,6 +57,11 @@ class _Obs(C.Structure):
     ]
 
 
+class _Tuning(C.Structure):
+    _fields_ = [("streams", C.c_int32), ("block_sigma", C.c_int32), ("block_eta", C.c_int32),
+                ("block_cell", C.c_int32), ("graph_steps", C.c_int32)]
+
+
 class _CStat(C.Structure):
     _fields_ = [
         ("largest", C.c_int32), ("second", C.c_int32), ("n_perc", C.c_int32),
@@ -148,6 +153,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cvf_destroy": (None, [P]),
         "cvf_vdw_energy": (C.c_int, [I32, I32, F32P, I32P, I32, C.c_float, DP, I32, I32, DP]),
         "cvf_profile_steps": (C.c_int, [P, I64, DP, C.POINTER(C.c_int64), I32]),
+        "cvf_get_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
+        "cvf_set_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -268,6 +275,21 @@ class Simulation:
         _check(self._lib, self._lib.cvf_set_shells(self._h, pt, d.ctypes.data_as(C.POINTER(C.c_float)),
                                                    f.ctypes.data_as(C.POINTER(C.c_int32)), d.size),
                "cvf_set_shells")
+
+    TUNING_FIELDS = ("streams", "block_sigma", "block_eta", "block_cell", "graph_steps")
+
+    @property
+    def tuning(self) -> dict:
+        t = _Tuning()
+        _check(self._lib, self._lib.cvf_get_tuning(self._h, C.byref(t)), "cvf_get_tuning")
+        return {k: getattr(t, k) for k in self.TUNING_FIELDS}
+
+    @tuning.setter
+    def tuning(self, value: dict) -> None:
+        cur = self.tuning
+        cur.update(value)
+        t = _Tuning(*[int(cur[k]) for k in self.TUNING_FIELDS])
+        _check(self._lib, self._lib.cvf_set_tuning(self._h, C.byref(t)), "cvf_set_tuning")
 
     KCLASSES = ("begin", "eta", "sigma", "sw_local", "sw_union", "sw_flip")
 

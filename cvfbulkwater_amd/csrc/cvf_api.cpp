@@ -72,58 +72,75 @@ struct PointHost {
   VolParams vp;
 };
 
+// A group of state points advanced by its own stream: its own block tables,
+// step slot and captured graphs.  Groups share nothing mutable, so their
+// launch sequences overlap on separate hardware queues.
+struct Group {
+  std::vector<int> points;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  DevBuf<BlockEnt> tab_half, tab_cube, tab_cell;
+  DevBuf<int32_t> plist;
+  uint32_t nb_half = 0, nb_cube = 0, nb_cell = 0;
+  hipGraph_t graph = nullptr, graphK = nullptr;
+  hipGraphExec_t gexec = nullptr, gexecK = nullptr;
+  size_t cells = 0;
+  void drop_graphs() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (gexecK) (void)hipGraphExecDestroy(gexecK);
+    if (graphK) (void)hipGraphDestroy(graphK);
+    gexec = gexecK = nullptr;
+    graph = graphK = nullptr;
+  }
+  ~Group() {
+    drop_graphs();
+    if (done) (void)hipEventDestroy(done);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
 }  // namespace
 
 struct cvf_sim {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // main stream: uploads, observables, statistics
+  hipEvent_t start_ev = nullptr;
   int npts = 0;
   int nm = 1, nc = 0, precision = CVF_FP64, vol_device = 0;
-  int nsched = 12;
   std::vector<PointHost> ph;
   size_t total_cells = 0, total_nodes = 0;
-  uint32_t shell_cap = 0;
+  cvf_tuning tune{};
 
   DevBuf<uint32_t> words, parent, cover, nslot_nperc;
-  DevBuf<uint8_t> cov, sched;
+  DevBuf<uint8_t> cov;
   DevBuf<int32_t> csize, slot;
   DevBuf<PointDev> pts;
   DevBuf<PointState> pst;
   DevBuf<float> dist;
   DevBuf<int32_t> freq;
   DevBuf<unsigned long long> counters;
-  DevBuf<BlockEnt> tab_half, tab_cube, tab_cell;
-  uint32_t nb_half = 0, nb_cube = 0, nb_cell = 0;
+  DevBuf<uint64_t> cur_steps;          // one slot per group
+  DevBuf<BlockEnt> tab_cell_all;       // every point's cells, for observables
+  uint32_t nb_cell_all = 0;
+  std::vector<std::unique_ptr<Group>> groups;
   uint32_t max_slots = 0;
 
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
   int64_t steps = 0;
   int64_t obs_at = -1;
   std::vector<unsigned long long> obs_counters;
-  bool sw_done = false;
 
+  void drop_graphs() {
+    for (auto& g : groups) g->drop_graphs();
+  }
   ~cvf_sim() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
+    groups.clear();
+    if (start_ev) (void)hipEventDestroy(start_ev);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
 namespace {
-
-// Block table over `count(p)` work items per point, padded to a multiple of
-// the XCD count.
-template <class F>
-std::vector<BlockEnt> make_table(const std::vector<PointHost>& ph, F count) {
-  std::vector<BlockEnt> t;
-  for (size_t p = 0; p < ph.size(); ++p) {
-    const uint32_t n = count(ph[p]);
-    for (uint32_t s = 0; s < n; s += kBlock) t.push_back(BlockEnt{(int32_t)p, s});
-  }
-  while (t.size() % kNumXcd) t.push_back(BlockEnt{-1, 0});
-  return t;
-}
 
 int validate(const cvf_params& p) {
   if (p.L < 4 || p.L % 4 != 0 || p.L > kMaxL)
@@ -131,7 +148,7 @@ int validate(const cvf_params& p) {
                 p.L, kMaxL);
   if (!(p.T > 0.f)) return fail(CVF_EINVAL, "T must be > 0");
   if (p.n_metropolis < 0 || p.n_cluster < 0) return fail(CVF_EINVAL, "negative step counts");
-  if (p.n_metropolis * 12 > 255 * 12) return fail(CVF_EINVAL, "n_metropolis too large");
+  if (p.n_metropolis > 4096 || p.n_cluster > 4096) return fail(CVF_EINVAL, "step counts too large");
   if (p.flag_chessboard != 1)
     return fail(CVF_EUNSUPPORTED, "Flag_chessboard=%d is not supported (only 1)", p.flag_chessboard);
   if (p.flag_config < 0 || p.flag_config > 2)
@@ -140,6 +157,8 @@ int validate(const cvf_params& p) {
     return fail(CVF_EINVAL, "unknown precision %d", p.precision);
   if (p.precision == CVF_FP32_COMPENSATED && p.volume_mode != CVF_VOLUME_DEVICE)
     return fail(CVF_EINVAL, "the FP32-compensated vdW form runs the volume move on the device");
+  if (p.volume_mode != CVF_VOLUME_HOST && p.volume_mode != CVF_VOLUME_DEVICE)
+    return fail(CVF_EINVAL, "unknown volume mode %d", p.volume_mode);
   if (!(p.r_cutoff > 0.f)) return fail(CVF_EINVAL, "R_cutoff must be > 0");
   return CVF_OK;
 }
@@ -244,6 +263,7 @@ void init_words(const PointHost& P, std::vector<uint32_t>& w) {
 }
 
 int upload_point_words(cvf_sim* s, int pt, const std::vector<uint32_t>& w) {
+  HIPC(hipStreamSynchronize(s->stream));
   HIPC(hipMemcpyAsync(s->words.p + s->ph[pt].woff, w.data(), w.size() * 4, hipMemcpyHostToDevice,
                       s->stream));
   HIPC(hipStreamSynchronize(s->stream));
@@ -258,7 +278,7 @@ int download_point_words(cvf_sim* s, int pt, std::vector<uint32_t>& w) {
   return CVF_OK;
 }
 
-PointDev make_point_dev(const PointHost& P, uint32_t sched_off) {
+PointDev make_point_dev(const PointHost& P) {
   PointDev d;
   memset(&d, 0, sizeof d);
   d.seed = P.prm.seed;
@@ -268,16 +288,60 @@ PointDev make_point_dev(const PointHost& P, uint32_t sched_off) {
   d.woff = P.woff;
   d.loff = P.loff;
   d.shell_off = P.shell_off;
-  d.divL = make_fastdiv(P.L);
-  d.divH = make_fastdiv(P.L / 2);
   d.jeff_sign = P.thr.Jeff > 0 ? 1 : (P.thr.Jeff < 0 ? -1 : 0);
-  d.sched_off = (int32_t)sched_off;
   d.vp = P.vp;
   memcpy(d.thr_sigma, P.thr.sigma, sizeof d.thr_sigma);
   memcpy(d.thr_eta, P.thr.eta, sizeof d.thr_eta);
   d.thr_sw[0] = P.thr.sw_hb;
   d.thr_sw[1] = P.thr.sw_s;
   return d;
+}
+
+BlockEnt make_entry(const PointHost& P, int point, uint32_t start) {
+  BlockEnt e;
+  e.point = point;
+  e.start = start;
+  e.L = P.L;
+  e.N = P.N;
+  e.woff = P.woff;
+  e.loff = P.loff;
+  e.seed_lo = (uint32_t)P.prm.seed;
+  e.seed_hi = (uint32_t)(P.prm.seed >> 32);
+  e.mL = make_fastdiv(P.L).m;
+  e.mH = make_fastdiv(P.L / 2).m;
+  e.mQ = make_fastdiv(P.L / 4).m;
+  e.jeff_sign = P.thr.Jeff > 0 ? 1 : (P.thr.Jeff < 0 ? -1 : 0);
+  e.thb_lo = (uint32_t)P.thr.sw_hb;
+  e.thb_hi = (uint32_t)(P.thr.sw_hb >> 32);
+  e.ts_lo = (uint32_t)P.thr.sw_s;
+  e.ts_hi = (uint32_t)(P.thr.sw_s >> 32);
+  return e;
+}
+
+// Block table over `count(P)` work items per point of `points`, `block`
+// items per workgroup, padded to a multiple of the XCD count.
+template <class F>
+std::vector<BlockEnt> make_table(const std::vector<PointHost>& ph, const std::vector<int>& points,
+                                 uint32_t block, F count) {
+  std::vector<BlockEnt> t;
+  for (int p : points) {
+    const uint32_t n = count(ph[p]);
+    for (uint32_t st = 0; st < n; st += block) t.push_back(make_entry(ph[p], p, st));
+  }
+  while (t.size() % kNumXcd) {
+    BlockEnt pad;
+    memset(&pad, 0, sizeof pad);
+    pad.point = -1;
+    t.push_back(pad);
+  }
+  return t;
+}
+
+template <class T>
+int upload_vec(DevBuf<T>& buf, const std::vector<T>& v) {
+  HIPC(buf.alloc(v.size()));
+  HIPC(hipMemcpy(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return CVF_OK;
 }
 
 int upload_tables(cvf_sim* s) {
@@ -290,18 +354,10 @@ int upload_tables(cvf_sim* s) {
     P.vp.nshell = (int32_t)P.dist.size();
     dist.insert(dist.end(), P.dist.begin(), P.dist.end());
     freq.insert(freq.end(), P.freq.begin(), P.freq.end());
-    pd.push_back(make_point_dev(P, (uint32_t)(p * s->nsched)));
+    pd.push_back(make_point_dev(P));
   }
   if (s->dist.p == nullptr || s->dist.n < dist.size()) {
-    // a captured step graph holds the old table pointers
-    if (s->gexec) {
-      (void)hipGraphExecDestroy(s->gexec);
-      s->gexec = nullptr;
-    }
-    if (s->graph) {
-      (void)hipGraphDestroy(s->graph);
-      s->graph = nullptr;
-    }
+    s->drop_graphs();  // captured graphs hold the old table pointers
     HIPC(s->dist.alloc(std::max<size_t>(dist.size(), 64)));
     HIPC(s->freq.alloc(std::max<size_t>(freq.size(), 64)));
   }
@@ -311,33 +367,150 @@ int upload_tables(cvf_sim* s) {
   return CVF_OK;
 }
 
-int capture_step_graph(cvf_sim* s) {
-  if (s->gexec) return CVF_OK;
-  HIPC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  const Launch lh{s->tab_half.p, s->nb_half}, lc{s->tab_cube.p, s->nb_cube},
-      ll{s->tab_cell.p, s->nb_cell};
-  launch_step_begin(s->stream, s->pts.p, s->npts, s->pst.p, s->sched.p, s->nsched, s->dist.p,
-                    s->freq.p, s->vol_device, s->precision);
-  launch_eta(s->stream, lc, s->pts.p, s->pst.p, s->words.p, 0);
-  launch_eta(s->stream, lc, s->pts.p, s->pst.p, s->words.p, 1);
-  for (int sub = 0; sub < 12 * s->nm; ++sub)
-    launch_sigma(s->stream, lh, s->pts.p, s->pst.p, s->words.p, s->sched.p, (uint32_t)sub);
-  for (int jc = 0; jc < s->nc; ++jc) {
-    launch_sw_local(s->stream, ll, s->pts.p, s->pst.p, s->words.p, s->parent.p, s->cov.p,
-                    (uint32_t)jc);
-    launch_sw_union(s->stream, ll, s->pts.p, s->parent.p, s->cov.p);
-    launch_sw_flip(s->stream, ll, s->pts.p, s->pst.p, s->words.p, s->parent.p, (uint32_t)jc);
+uint32_t cells_sigma(const PointHost& P) { return P.N / 8; }  // quads of active cells
+uint32_t cells_eta(const PointHost& P) { return (P.L / 2) * (P.L / 4) * (P.L / 4); }  // cube quads
+uint32_t cells_all(const PointHost& P) { return P.N; }
+
+int valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512 || b == 1024; }
+
+// (Re)build the groups for the current tuning: points are dealt to groups
+// largest-first onto the group with the fewest cells.
+int build_groups(cvf_sim* s) {
+  HIPC(hipDeviceSynchronize());
+  s->groups.clear();
+  const int ng = std::max(1, std::min(s->tune.streams, s->npts));
+  for (int g = 0; g < ng; ++g) s->groups.emplace_back(new Group);
+  std::vector<int> order(s->npts);
+  for (int i = 0; i < s->npts; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return s->ph[a].N > s->ph[b].N; });
+  for (int p : order) {
+    Group* best = s->groups[0].get();
+    for (auto& g : s->groups)
+      if (g->cells < best->cells) best = g.get();
+    best->points.push_back(p);
+    best->cells += s->ph[p].N;
   }
-  hipError_t e = hipStreamEndCapture(s->stream, &s->graph);
-  if (e != hipSuccess) return fail(CVF_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
-  HIPC(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+  for (auto& g : s->groups) {
+    std::sort(g->points.begin(), g->points.end());
+    HIPC(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    HIPC(hipEventCreateWithFlags(&g->done, hipEventDisableTiming));
+    const auto th = make_table(s->ph, g->points, (uint32_t)s->tune.block_sigma, cells_sigma);
+    const auto tc = make_table(s->ph, g->points, (uint32_t)s->tune.block_eta, cells_eta);
+    const auto tl = make_table(s->ph, g->points, (uint32_t)s->tune.block_cell, cells_all);
+    int rc;
+    if ((rc = upload_vec(g->tab_half, th)) || (rc = upload_vec(g->tab_cube, tc)) ||
+        (rc = upload_vec(g->tab_cell, tl)))
+      return rc;
+    g->nb_half = (uint32_t)th.size();
+    g->nb_cube = (uint32_t)tc.size();
+    g->nb_cell = (uint32_t)tl.size();
+    std::vector<int32_t> pl(g->points.begin(), g->points.end());
+    if ((rc = upload_vec(g->plist, pl))) return rc;
+  }
+  HIPC(s->cur_steps.alloc(s->groups.size()));
+  HIPC(hipMemset(s->cur_steps.p, 0, s->groups.size() * 8));
+  std::vector<int> all(s->npts);
+  for (int i = 0; i < s->npts; ++i) all[i] = i;
+  const auto ta = make_table(s->ph, all, 256, cells_all);
+  int rc = upload_vec(s->tab_cell_all, ta);
+  if (rc) return rc;
+  s->nb_cell_all = (uint32_t)ta.size();
+  HIPC(hipDeviceSynchronize());
   return CVF_OK;
+}
+
+// One MC step of group g (MonteCarlo_Step, gpu_water3D.cu:1346-1733).
+void enqueue_step(cvf_sim* s, int gi, hipStream_t st) {
+  Group& g = *s->groups[gi];
+  const uint64_t* stepp = s->cur_steps.p + gi;
+  const Launch lh{g.tab_half.p, g.nb_half, (uint32_t)s->tune.block_sigma, stepp};
+  const Launch lc{g.tab_cube.p, g.nb_cube, (uint32_t)s->tune.block_eta, stepp};
+  const Launch ll{g.tab_cell.p, g.nb_cell, (uint32_t)s->tune.block_cell, stepp};
+  launch_step_begin(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, s->cur_steps.p + gi,
+                    s->dist.p, s->freq.p, s->vol_device, s->precision);
+  launch_eta(st, lc, s->pts.p, s->words.p, 0);
+  launch_eta(st, lc, s->pts.p, s->words.p, 1);
+  for (int sub = 0; sub < 12 * s->nm; ++sub) launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub);
+  for (int jc = 0; jc < s->nc; ++jc) {
+    launch_sw_local(st, ll, s->words.p, s->parent.p, s->cov.p, (uint32_t)jc);
+    launch_sw_union(st, ll, s->parent.p, s->cov.p);
+    launch_sw_flip(st, ll, s->words.p, s->parent.p, (uint32_t)jc);
+  }
+}
+
+int capture(cvf_sim* s, int gi, int nsteps, hipGraph_t* g, hipGraphExec_t* ge) {
+  hipStream_t st = s->groups[gi]->stream;
+  HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < nsteps; ++i) enqueue_step(s, gi, st);
+  hipError_t e = hipStreamEndCapture(st, g);
+  if (e != hipSuccess) return fail(CVF_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+  HIPC(hipGraphInstantiate(ge, *g, nullptr, nullptr, 0));
+  return CVF_OK;
+}
+
+// The MC step is a fixed launch sequence whose step index lives in device
+// memory, so one captured graph replays any step.
+int capture_step_graphs(cvf_sim* s) {
+  for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+    Group& g = *s->groups[gi];
+    if (g.gexec && g.gexecK) continue;
+    g.drop_graphs();
+    int rc = capture(s, (int)gi, 1, &g.graph, &g.gexec);
+    if (!rc && s->tune.graph_steps > 1) rc = capture(s, (int)gi, s->tune.graph_steps, &g.graphK, &g.gexecK);
+    if (rc) return rc;
+  }
+  return CVF_OK;
+}
+
+// group streams start after the work already on the main stream
+int fork_groups(cvf_sim* s) {
+  HIPC(hipEventRecord(s->start_ev, s->stream));
+  for (auto& g : s->groups) HIPC(hipStreamWaitEvent(g->stream, s->start_ev, 0));
+  return CVF_OK;
+}
+
+// the main stream continues after every group's work
+int join_groups(cvf_sim* s) {
+  for (auto& g : s->groups) {
+    HIPC(hipEventRecord(g->done, g->stream));
+    HIPC(hipStreamWaitEvent(s->stream, g->done, 0));
+  }
+  return CVF_OK;
+}
+
+void host_volume_moves(cvf_sim* s) {
+  for (auto& P : s->ph) {
+    double u2 = 0;
+    const double trial = volume_trial(P.vol, P.vp, (uint64_t)s->steps, &u2);
+    const double te =
+        vdw_energy_fp64(trial, P.prm.r_cutoff, P.dist.data(), P.freq.data(), (int)P.dist.size());
+    volume_accept(P.vol, P.vp, trial, te, u2);
+  }
 }
 
 int check_pt(cvf_sim* s, int32_t pt) {
   if (!s) return fail(CVF_EINVAL, "null handle");
   if (pt < 0 || pt >= s->npts) return fail(CVF_EINVAL, "state point %d out of range [0,%d)", pt, s->npts);
   return CVF_OK;
+}
+
+int check_tuning(const cvf_tuning& t) {
+  if (t.streams < 1 || t.streams > 8) return fail(CVF_EINVAL, "streams must be in [1, 8]");
+  if (!valid_block(t.block_sigma) || !valid_block(t.block_eta) || !valid_block(t.block_cell))
+    return fail(CVF_EINVAL, "block sizes must be 64, 128, 256, 512 or 1024");
+  if (t.graph_steps < 1 || t.graph_steps > 64) return fail(CVF_EINVAL, "graph_steps must be in [1, 64]");
+  return CVF_OK;
+}
+
+cvf_tuning default_tuning(const cvf_sim* s) {
+  cvf_tuning t;
+  t.streams = 1;
+  t.block_sigma = s->total_cells < (1u << 20) ? 64 : 256;
+  t.block_eta = t.block_sigma;
+  t.block_cell = 256;
+  t.graph_steps = 16;
+  return t;
 }
 
 int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
@@ -365,8 +538,8 @@ int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
   s->nc = pts[0].n_cluster;
   s->precision = pts[0].precision;
   s->vol_device = pts[0].volume_mode == CVF_VOLUME_DEVICE;
-  s->nsched = std::max(12 * s->nm, 1);
   HIPC(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  HIPC(hipEventCreateWithFlags(&s->start_ev, hipEventDisableTiming));
   size_t woff = 0;
   uint32_t maxL = 0;
   for (int i = 0; i < n; ++i) {
@@ -410,12 +583,10 @@ int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
   HIPC(s->parent.alloc(s->total_nodes));
   HIPC(hipMemset(s->parent.p, 0, s->total_nodes * 4));  // labels read as 0 before the first SW step
   HIPC(s->cov.alloc(s->total_cells));
-  HIPC(s->sched.alloc((size_t)n * s->nsched));
-  HIPC(hipMemset(s->sched.p, 0, (size_t)n * s->nsched));
   HIPC(s->pts.alloc(n));
   HIPC(s->pst.alloc(n));
   HIPC(s->counters.alloc(2 * (size_t)n));
-  uint32_t maxN = maxL * maxL * maxL;
+  const uint32_t maxN = maxL * maxL * maxL;
   HIPC(s->csize.alloc(6 * (size_t)maxN));
   HIPC(s->slot.alloc(6 * (size_t)maxN));
   s->max_slots = 3 * maxL * maxL + 1;  // a cluster of >= 2L nodes: at most 6N/(2L)
@@ -426,25 +597,11 @@ int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
   std::vector<PointState> st(n);
   for (int i = 0; i < n; ++i) {
     st[i].step = 0;
-    st[i].cur_step = 0;
     st[i].vol = s->ph[i].vol;
   }
   HIPC(hipMemcpy(s->pst.p, st.data(), n * sizeof(PointState), hipMemcpyHostToDevice));
-  // block tables
-  // sigma: one thread per pair of active cells (N/4); eta: one thread per
-  // pair of cubes of one family (N/16)
-  auto th = make_table(s->ph, [](const PointHost& P) { return P.N / 4; });
-  auto tc = make_table(s->ph, [](const PointHost& P) { return (P.L / 2) * (P.L / 2) * (P.L / 4); });
-  auto tl = make_table(s->ph, [](const PointHost& P) { return P.N; });
-  HIPC(s->tab_half.alloc(th.size()));
-  HIPC(s->tab_cube.alloc(tc.size()));
-  HIPC(s->tab_cell.alloc(tl.size()));
-  HIPC(hipMemcpy(s->tab_half.p, th.data(), th.size() * sizeof(BlockEnt), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(s->tab_cube.p, tc.data(), tc.size() * sizeof(BlockEnt), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(s->tab_cell.p, tl.data(), tl.size() * sizeof(BlockEnt), hipMemcpyHostToDevice));
-  s->nb_half = (uint32_t)th.size();
-  s->nb_cube = (uint32_t)tc.size();
-  s->nb_cell = (uint32_t)tl.size();
+  s->tune = default_tuning(s.get());
+  if ((rc = build_groups(s.get()))) return rc;
   // initial configurations
   std::vector<uint32_t> w;
   for (int i = 0; i < n; ++i) {
@@ -557,24 +714,25 @@ int cvf_get_config(cvf_sim* s, int32_t pt, uint8_t* sigma6N, uint8_t* eta6N) {
 int cvf_step(cvf_sim* s, int64_t n_steps) {
   if (!s) return fail(CVF_EINVAL, "null handle");
   if (n_steps < 0) return fail(CVF_EINVAL, "negative step count");
+  if (n_steps == 0) return CVF_OK;
   HIPC(hipSetDevice(s->device));
-  int rc = capture_step_graph(s);
+  int rc = capture_step_graphs(s);
   if (rc) return rc;
-  for (int64_t i = 0; i < n_steps; ++i) {
-    if (!s->vol_device) {
-      for (auto& P : s->ph) {
-        double u2 = 0;
-        const double trial = volume_trial(P.vol, P.vp, (uint64_t)s->steps, &u2);
-        const double te =
-            vdw_energy_fp64(trial, P.prm.r_cutoff, P.dist.data(), P.freq.data(), (int)P.dist.size());
-        volume_accept(P.vol, P.vp, trial, te, u2);
-      }
+  if ((rc = fork_groups(s))) return rc;
+  const int64_t K = s->tune.graph_steps;
+  for (int64_t i = 0; i < n_steps;) {
+    const bool bulk = K > 1 && n_steps - i >= K;
+    for (auto& g : s->groups) HIPC(hipGraphLaunch(bulk ? g->gexecK : g->gexec, g->stream));
+    const int64_t k = bulk ? K : 1;
+    // the volume chain does not read the lattice: in host-volume mode the
+    // host advances it while the device runs the lattice steps
+    for (int64_t j = 0; j < k; ++j) {
+      if (!s->vol_device) host_volume_moves(s);
+      ++s->steps;
     }
-    HIPC(hipGraphLaunch(s->gexec, s->stream));
-    ++s->steps;
+    i += k;
   }
-  if (s->nc > 0 && n_steps > 0) s->sw_done = true;
-  return CVF_OK;
+  return join_groups(s);
 }
 
 int cvf_sync(cvf_sim* s) {
@@ -608,7 +766,8 @@ int cvf_observe(cvf_sim* s, int32_t pt, cvf_obs* o) {
   HIPC(hipSetDevice(s->device));
   if (s->obs_at != s->steps) {
     launch_zero_u32(s->stream, (uint32_t*)s->counters.p, 4 * (size_t)s->npts);
-    launch_observe(s->stream, Launch{s->tab_cell.p, s->nb_cell}, s->pts.p, s->words.p, s->counters.p);
+    launch_observe(s->stream, Launch{s->tab_cell_all.p, s->nb_cell_all, 256, nullptr}, s->words.p,
+                   s->counters.p);
     s->obs_counters.resize(2 * (size_t)s->npts);
     HIPC(hipMemcpyAsync(s->obs_counters.data(), s->counters.p, 16 * (size_t)s->npts,
                         hipMemcpyDeviceToHost, s->stream));
@@ -759,61 +918,90 @@ int cvf_profile_steps(cvf_sim* s, int64_t n_steps, double* kernel_ms, int64_t* l
   HIPC(hipSetDevice(s->device));
   if (!s->vol_device && n_steps > 0)
     return fail(CVF_EUNSUPPORTED, "profiling needs the device volume mode");
-  std::vector<hipEvent_t> ev;
-  std::vector<int> cls;
-  const Launch lh{s->tab_half.p, s->nb_half}, lc{s->tab_cube.p, s->nb_cube},
-      ll{s->tab_cell.p, s->nb_cell};
-  auto mark = [&](int c) -> int {
-    hipEvent_t e;
-    HIPC(hipEventCreate(&e));
-    HIPC(hipEventRecord(e, s->stream));
-    ev.push_back(e);
-    cls.push_back(c);
-    return CVF_OK;
+  struct Rec {
+    hipEvent_t a, b;
+    int cls;
   };
-  int rc = mark(-1);
+  std::vector<Rec> recs;
+  hipError_t err = hipSuccess;
+  // each launch records its own start and end (hipExtLaunchKernelGGL); the
+  // groups run concurrently on their streams, as in cvf_step
+  auto timed = [&](int cls, auto&& fn) {
+    Rec r{nullptr, nullptr, cls};
+    if (err == hipSuccess) err = hipEventCreate(&r.a);
+    if (err == hipSuccess) err = hipEventCreate(&r.b);
+    if (err != hipSuccess) return;
+    set_launch_timing(r.a, r.b);
+    fn();
+    set_launch_timing(nullptr, nullptr);
+    recs.push_back(r);
+  };
+  int rc = fork_groups(s);
   if (rc) return rc;
   for (int64_t i = 0; i < n_steps; ++i) {
-    launch_step_begin(s->stream, s->pts.p, s->npts, s->pst.p, s->sched.p, s->nsched, s->dist.p,
-                      s->freq.p, s->vol_device, s->precision);
-    if ((rc = mark(CVF_KCLASS_BEGIN))) return rc;
-    for (int f = 0; f < 2; ++f) {
-      launch_eta(s->stream, lc, s->pts.p, s->pst.p, s->words.p, f);
-      if ((rc = mark(CVF_KCLASS_ETA))) return rc;
-    }
-    for (int sub = 0; sub < 12 * s->nm; ++sub) {
-      launch_sigma(s->stream, lh, s->pts.p, s->pst.p, s->words.p, s->sched.p, (uint32_t)sub);
-      if ((rc = mark(CVF_KCLASS_SIGMA))) return rc;
-    }
-    for (int jc = 0; jc < s->nc; ++jc) {
-      launch_sw_local(s->stream, ll, s->pts.p, s->pst.p, s->words.p, s->parent.p, s->cov.p,
-                      (uint32_t)jc);
-      if ((rc = mark(CVF_KCLASS_SW_LOCAL))) return rc;
-      launch_sw_union(s->stream, ll, s->pts.p, s->parent.p, s->cov.p);
-      if ((rc = mark(CVF_KCLASS_SW_UNION))) return rc;
-      launch_sw_flip(s->stream, ll, s->pts.p, s->pst.p, s->words.p, s->parent.p, (uint32_t)jc);
-      if ((rc = mark(CVF_KCLASS_SW_FLIP))) return rc;
+    for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+      Group& g = *s->groups[gi];
+      hipStream_t st = g.stream;
+      const uint64_t* stepp = s->cur_steps.p + gi;
+      const Launch lh{g.tab_half.p, g.nb_half, (uint32_t)s->tune.block_sigma, stepp};
+      const Launch lc{g.tab_cube.p, g.nb_cube, (uint32_t)s->tune.block_eta, stepp};
+      const Launch ll{g.tab_cell.p, g.nb_cell, (uint32_t)s->tune.block_cell, stepp};
+      timed(CVF_KCLASS_BEGIN, [&] {
+        launch_step_begin(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p,
+                          s->cur_steps.p + gi, s->dist.p, s->freq.p, s->vol_device, s->precision);
+      });
+      for (int f = 0; f < 2; ++f)
+        timed(CVF_KCLASS_ETA, [&] { launch_eta(st, lc, s->pts.p, s->words.p, f); });
+      for (int sub = 0; sub < 12 * s->nm; ++sub)
+        timed(CVF_KCLASS_SIGMA, [&] { launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub); });
+      for (int jc = 0; jc < s->nc; ++jc) {
+        timed(CVF_KCLASS_SW_LOCAL,
+              [&] { launch_sw_local(st, ll, s->words.p, s->parent.p, s->cov.p, (uint32_t)jc); });
+        timed(CVF_KCLASS_SW_UNION, [&] { launch_sw_union(st, ll, s->parent.p, s->cov.p); });
+        timed(CVF_KCLASS_SW_FLIP,
+              [&] { launch_sw_flip(st, ll, s->words.p, s->parent.p, (uint32_t)jc); });
+      }
     }
     ++s->steps;
   }
-  if (s->nc > 0 && n_steps > 0) s->sw_done = true;
+  if (err != hipSuccess) return fail(CVF_EDEVICE, "event creation failed: %s", hipGetErrorString(err));
+  if ((rc = join_groups(s))) return rc;
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(s->stream));
   std::vector<double> tot(CVF_KCLASS_COUNT, 0.0);
   std::vector<int64_t> cnt(CVF_KCLASS_COUNT, 0);
-  for (size_t k = 1; k < ev.size(); ++k) {
+  for (auto& r : recs) {
     float ms = 0;
-    HIPC(hipEventElapsedTime(&ms, ev[k - 1], ev[k]));
-    tot[cls[k]] += ms;
-    cnt[cls[k]] += 1;
+    HIPC(hipEventElapsedTime(&ms, r.a, r.b));
+    tot[r.cls] += ms;
+    cnt[r.cls] += 1;
   }
-  for (auto e : ev) (void)hipEventDestroy(e);
+  for (auto& r : recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
   for (int c = 0; c < CVF_KCLASS_COUNT; ++c) {
     kernel_ms[c] = cnt[c] ? tot[c] / (double)cnt[c] : 0.0;
     if (launches) launches[c] = cnt[c];
   }
   s->obs_at = -1;
   return CVF_OK;
+}
+
+int cvf_get_tuning(cvf_sim* s, cvf_tuning* t) {
+  if (!s || !t) return fail(CVF_EINVAL, "null argument");
+  *t = s->tune;
+  return CVF_OK;
+}
+
+int cvf_set_tuning(cvf_sim* s, const cvf_tuning* t) {
+  if (!s || !t) return fail(CVF_EINVAL, "null argument");
+  int rc = check_tuning(*t);
+  if (rc) return rc;
+  HIPC(hipSetDevice(s->device));
+  HIPC(hipStreamSynchronize(s->stream));
+  s->tune = *t;
+  return build_groups(s);
 }
 
 int cvf_vdw_energy(int32_t device_id, int32_t L, const float* dist, const int32_t* freq,

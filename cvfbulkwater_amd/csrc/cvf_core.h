@@ -81,6 +81,16 @@ __host__ __device__ __forceinline__ uint32_t uniform_int(uint32_t x, uint32_t n)
   return mulhi32(x, n);
 }
 
+// One Philox word per site update: X = q*w (35-bit); the proposal is
+// X >> 32 = floor(q u) and the acceptance uniform is X mod 2^32, the
+// fractional part of q u in 32-bit fixed point (u = w / 2^32).
+__host__ __device__ __forceinline__ void split_word(uint32_t w, uint32_t q, uint32_t& prop,
+                                                   uint32_t& acc) {
+  const uint64_t X = (uint64_t)w * q;
+  prop = (uint32_t)(X >> 32);
+  acc = (uint32_t)X;
+}
+
 // 53-bit uniform double in [0,1).
 __host__ __device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
   return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
@@ -105,14 +115,17 @@ __host__ __device__ __forceinline__ int count_eq(uint32_t w, uint32_t v) {
   return 6 - popc32(nz);
 }
 
-// Fast exact division by a runtime constant d (2 <= d), valid for t*d < 2^32.
+// Fast exact division by a runtime constant d >= 1: floor(t/d) =
+// mulhi(t, ceil(2^32/d)), exact whenever t*d < 2^32 (d = 1 is passed through).
 struct FastDiv {
   uint32_t d, m;
 };
 __host__ inline FastDiv make_fastdiv(uint32_t d) {
-  return FastDiv{d, (uint32_t)((((uint64_t)1 << 32) + d - 1) / d)};
+  return FastDiv{d, d <= 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + d - 1) / d)};
 }
-__host__ __device__ __forceinline__ uint32_t fdiv(uint32_t t, FastDiv f) { return mulhi32(t, f.m); }
+__host__ __device__ __forceinline__ uint32_t fdiv(uint32_t t, FastDiv f) {
+  return f.d <= 1 ? t : mulhi32(t, f.m);
+}
 
 // ------------------------------------------------- cube (chessboard) tables
 // 12 edges of the 2x2x2 cube with origin v1 (gpu_water3D.cu:3136-3176);

@@ -8,7 +8,6 @@
 
 namespace cvf {
 
-constexpr int kBlock = 256;
 constexpr int kNumXcd = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin
 
 // Immutable per-state-point description (device copy).
@@ -18,9 +17,8 @@ struct PointDev {
   uint32_t woff;       // offset of the point's words in the state array
   uint32_t loff;       // offset of the point's 6N labels
   uint32_t shell_off;  // offset of its shell table
-  FastDiv divL, divH;  // division by L and by L/2
   int32_t jeff_sign;   // sign of Jeff (SW covalent rule, gpu_water3D.cu:2399-2424)
-  int32_t sched_off;   // offset of its (arm, parity) schedule for the current step
+  uint32_t pad;
   VolParams vp;
   uint64_t thr_sigma[33];  // index (eta*dhb + 1)*11 + dsig + 5
   uint64_t thr_eta[9];     // index DeltaN_HB + 4
@@ -29,39 +27,46 @@ struct PointDev {
 
 // Mutable per-state-point device state.
 struct PointState {
-  uint64_t step;      // next MC step index
-  uint64_t cur_step;  // step being executed
+  uint64_t step;  // next MC step index of this point
   VolState vol;
 };
 
-// One workgroup's work: state point and first work item.
+// One workgroup's work: the state point, the first work item, and every
+// per-point scalar the kernels need, so that one scalar load of the entry
+// starts a workgroup (64 bytes).
 struct BlockEnt {
-  int32_t point;
+  int32_t point;  // -1: padding entry
   uint32_t start;
+  uint32_t L, N, woff, loff;
+  uint32_t seed_lo, seed_hi;
+  uint32_t mL, mH, mQ;  // fast-division multipliers for L, L/2, L/4
+  int32_t jeff_sign;
+  uint32_t thb_lo, thb_hi, ts_lo, ts_hi;  // SW link thresholds
 };
 
 struct Launch {
   const BlockEnt* tab;
-  uint32_t nblocks;  // padded to a multiple of kNumXcd
+  uint32_t nblocks;      // padded to a multiple of kNumXcd
+  uint32_t block;        // threads per workgroup
+  const uint64_t* step;  // the step being executed (written by k_step_begin)
 };
 
+// Per-kernel timing for profiling: while set, launches go through
+// hipExtLaunchKernelGGL with these start/stop events (nullptr to clear).
+void set_launch_timing(hipEvent_t start, hipEvent_t stop);
+
 // Launchers (all asynchronous on `st`).
-void launch_step_begin(hipStream_t st, const PointDev* pts, int npts, PointState* ps, uint8_t* sched,
-                       int nsched, const float* dist, const int32_t* freq, int vol_device,
-                       int precision);
-void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps, uint32_t* words,
-                int family);
-void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
-                  uint32_t* words, const uint8_t* sched, uint32_t sub);
+void launch_step_begin(hipStream_t st, const PointDev* pts, const int32_t* plist, int npts,
+                       PointState* ps, uint64_t* cur_step, const float* dist, const int32_t* freq,
+                       int vol_device, int precision);
+void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, int family);
+void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, uint32_t sub);
 void launch_zero_u32(hipStream_t st, uint32_t* p, size_t n);
-void launch_observe(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* words,
-                    unsigned long long* counters);
-void launch_sw_local(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
-                     const uint32_t* words, uint32_t* parent, uint8_t* cov, uint32_t jc);
-void launch_sw_union(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* parent,
-                     const uint8_t* cov);
-void launch_sw_flip(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
-                    uint32_t* words, uint32_t* parent, uint32_t jc);
+void launch_observe(hipStream_t st, Launch ln, const uint32_t* words, unsigned long long* counters);
+void launch_sw_local(hipStream_t st, Launch ln, const uint32_t* words, uint32_t* parent, uint8_t* cov,
+                     uint32_t jc);
+void launch_sw_union(hipStream_t st, Launch ln, uint32_t* parent, const uint8_t* cov);
+void launch_sw_flip(hipStream_t st, Launch ln, uint32_t* words, uint32_t* parent, uint32_t jc);
 // cluster statistics of one point (labels at `label`, 6N nodes)
 void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, int32_t* size);
 void launch_perc_slots(hipStream_t st, const uint32_t* label, uint32_t nn, const int32_t* size,

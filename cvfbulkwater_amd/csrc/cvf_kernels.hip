@@ -7,12 +7,14 @@
 //                -> sigma(sub 0) ... sigma(sub 12*n_metropolis - 1)
 //                -> [sw_local -> sw_union -> sw_flip] x n_cluster
 // which the host captures once into a hipGraph and replays.  Every kernel
-// takes a block table: entry b names the state point and first work item of
-// workgroup b, so one launch advances any number of state points of uneven
-// L.  Workgroup ids are remapped so that each XCD gets a contiguous range of
-// the table (each XCD has its own L2; a point's words then stay in one L2
-// from launch to launch).
+// takes a block table whose entry b carries the state point, the first work
+// item and all per-point scalars of workgroup b, so one launch advances any
+// number of state points of uneven L and a workgroup starts from a single
+// scalar load.  Workgroup ids are remapped so that each XCD gets a
+// contiguous range of the table (each XCD has its own L2).
 #include "cvf_kernels.h"
+
+#include <hip/hip_ext.h>
 
 namespace cvf {
 
@@ -23,15 +25,33 @@ __device__ __forceinline__ uint32_t xcd_logical_block(uint32_t b, uint32_t nb) {
   return (b % kNumXcd) * per + b / kNumXcd;
 }
 
-__device__ __forceinline__ uint64_t load_step(const PointState* ps, int p) {
-  return ps[p].cur_step;
+__device__ __forceinline__ uint64_t seed_of(const BlockEnt& e) {
+  return ((uint64_t)e.seed_hi << 32) | e.seed_lo;
+}
+
+__device__ __forceinline__ uint32_t neighbor_up(uint32_t c, uint32_t coord, uint32_t stride,
+                                                uint32_t L) {
+  return coord == L - 1 ? c - (L - 1) * stride : c + stride;
+}
+
+// neighbour of cell c (coordinate `coord` along the arm's axis, cell stride
+// `stride` along it) across arm `arm`
+__device__ __forceinline__ uint32_t arm_neighbor(uint32_t c, uint32_t coord, uint32_t stride,
+                                                 uint32_t L, int arm) {
+  if (arm & 1) return coord == 0 ? c + (L - 1) * stride : c - stride;
+  return coord == L - 1 ? c - (L - 1) * stride : c + stride;
+}
+
+__device__ __forceinline__ void cell_coords(const BlockEnt& e, uint32_t t, uint32_t& x, uint32_t& y,
+                                            uint32_t& z) {
+  const FastDiv dL{e.L, e.mL};
+  const uint32_t r1 = fdiv(t, dL);
+  x = t - r1 * e.L;
+  z = fdiv(r1, dL);
+  y = r1 - z * e.L;
 }
 
 // ------------------------------------------------------------ step begin
-// One wavefront per state point: reads and advances the point's step
-// counter, draws the (arm, parity) schedule of the step's half-sweeps
-// (gpu_water3D.cu:1393-1394) and, in device-volume mode, performs the
-// volume move (volume_step, 1740-1781).
 __device__ __forceinline__ void two_sum_acc(float& s, float& c, float v) {
   // Neumaier's compensated step
   const float t = s + v;
@@ -110,20 +130,21 @@ __device__ double vdw_fp32_wave(double r, const float* dist, const int32_t* freq
   return ((double)s + (double)c) * (double)N;
 }
 
+// One wavefront per state point of a group: performs the point's volume move
+// in device-volume mode (volume_step, gpu_water3D.cu:1740-1781), advances the
+// point's step counter and publishes the step being executed for the
+// group's kernels.  All points of a handle advance in lockstep.
 __global__ __launch_bounds__(64) void k_step_begin(const PointDev* __restrict__ pts,
+                                                   const int32_t* __restrict__ plist,
                                                    PointState* __restrict__ ps,
-                                                   uint8_t* __restrict__ sched, int nsched,
+                                                   uint64_t* __restrict__ cur_step,
                                                    const float* __restrict__ dist,
                                                    const int32_t* __restrict__ freq, int vol_device,
                                                    int precision) {
-  const int p = blockIdx.x;
+  const int p = plist[blockIdx.x];
   const int lane = threadIdx.x;
   const PointDev& P = pts[p];
   const uint64_t step = ps[p].step;
-  for (int s = lane; s < nsched; s += 64) {
-    const U4 r = draw(P.seed, kStSched, 0, step, (uint32_t)s);
-    sched[P.sched_off + s] = (uint8_t)(uniform_int(r.x, 6) | ((r.y >> 31) << 3));
-  }
   if (vol_device) {
     VolState vs = ps[p].vol;
     double u2 = 0;
@@ -142,8 +163,8 @@ __global__ __launch_bounds__(64) void k_step_begin(const PointDev* __restrict__ 
   }
   __syncthreads();
   if (lane == 0) {
-    ps[p].cur_step = step;
     ps[p].step = step + 1;
+    if (blockIdx.x == 0) *cur_step = step;
   }
 }
 
@@ -155,8 +176,7 @@ __global__ __launch_bounds__(64) void k_step_begin(const PointDev* __restrict__ 
 // families are independent given sigma, so running the families one after
 // the other equals the reference's simultaneous update.
 __device__ __forceinline__ void eta_cube(uint32_t* __restrict__ W, const uint64_t* thr, uint32_t L,
-                                         uint32_t ox, uint32_t oy, uint32_t oz, uint32_t ru,
-                                         uint32_t ra) {
+                                         uint32_t ox, uint32_t oy, uint32_t oz, uint32_t rw) {
   const uint32_t x1 = ox + 1 == L ? 0 : ox + 1;
   const uint32_t y1 = oy + 1 == L ? 0 : oy + 1;
   const uint32_t z1 = oz + 1 == L ? 0 : oz + 1;
@@ -175,9 +195,11 @@ __device__ __forceinline__ void eta_cube(uint32_t* __restrict__ W, const uint64_
     D |= (uint32_t)(sigma_of(w[A], 2 * ax) == sigma_of(w[B], 2 * ax + 1)) << jj;
     E |= eta_of(w[A], 2 * ax) << jj;
   }
-  const uint32_t M = sol_mask(uniform_int(ru, 9));
+  uint32_t prop, acc;
+  split_word(rw, 9, prop, acc);
+  const uint32_t M = sol_mask(prop);
   const int dn = popc32(M & D) - popc32(E & D);
-  if ((uint64_t)ra < thr[dn + 4] && M != E) {
+  if ((uint64_t)acc < thr[dn + 4] && M != E) {
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
@@ -191,32 +213,31 @@ __device__ __forceinline__ void eta_cube(uint32_t* __restrict__ W, const uint64_
   }
 }
 
-// One thread per pair of cubes of the family with origins (ox,oy,oz) and
-// (ox,oy,oz+L/2): both use one Philox block (words x,y and z,w).
-__global__ __launch_bounds__(kBlock) void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb,
-                                                const PointDev* __restrict__ pts,
-                                                const PointState* __restrict__ ps,
-                                                uint32_t* __restrict__ words, int family) {
+// One thread per quad of cubes of the family with origins
+// (ox, oy + a L/2, oz + b L/2), a,b in {0,1}: one Philox block, word a + 2b.
+__global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
+                      const PointDev* __restrict__ pts, uint32_t* __restrict__ words, int family) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
-  const PointDev& P = pts[e.point];
   __shared__ uint64_t thr[9];
-  if (threadIdx.x < 9) thr[threadIdx.x] = P.thr_eta[threadIdx.x];
+  if (threadIdx.x < 9) thr[threadIdx.x] = pts[e.point].thr_eta[threadIdx.x];
   __syncthreads();
-  const uint32_t hL = P.hL, L = P.L;
+  const uint32_t L = e.L, hL = L / 2, qL = L / 4;
   const uint32_t t = e.start + threadIdx.x;
-  if (t >= hL * hL * (L / 4)) return;
-  const uint64_t step = load_step(ps, e.point);
-  const uint32_t r1 = fdiv(t, P.divH);
+  if (t >= hL * qL * qL) return;
+  const uint64_t step = *stepp;
+  const uint32_t r1 = fdiv(t, FastDiv{hL, e.mH});  // t = i + hL (j + qL k), j,k < L/4
   const uint32_t i = t - r1 * hL;
-  const uint32_t k = fdiv(r1, P.divH);
-  const uint32_t j = r1 - k * hL;
+  const uint32_t k = fdiv(r1, FastDiv{qL, e.mQ});
+  const uint32_t j = r1 - k * qL;
   const uint32_t ox = 2 * i + family, oy = 2 * j + family, oz = 2 * k + family;
   const uint32_t cube_ref = i + hL * j + hL * hL * oz;  // gpu_water3D.cu:3405
-  const U4 r = draw(P.seed, kStEta, cube_ref, step, 0);
-  uint32_t* W = words + P.woff;
-  eta_cube(W, thr, L, ox, oy, oz, r.x, r.y);
-  eta_cube(W, thr, L, ox, oy, oz + L / 2, r.z, r.w);
+  const U4 r = draw(seed_of(e), kStEta, cube_ref, step, 0);
+  uint32_t* W = words + e.woff;
+  eta_cube(W, thr, L, ox, oy, oz, r.x);
+  eta_cube(W, thr, L, ox, oy + hL, oz, r.y);
+  eta_cube(W, thr, L, ox, oy, oz + hL, r.z);
+  eta_cube(W, thr, L, ox, oy + hL, oz + hL, r.w);
 }
 
 // --------------------------------------------------------- sigma half-sweep
@@ -226,11 +247,12 @@ __global__ __launch_bounds__(kBlock) void k_eta(const BlockEnt* __restrict__ tab
 // facing arm of the neighbour across arm a; that neighbour has the other
 // parity along the axis, so no word read in a launch is written in it.
 __device__ __forceinline__ void sigma_update(uint32_t* __restrict__ W, const uint64_t* thr, uint32_t c,
-                                             uint32_t nbc, int arm, uint32_t ru, uint32_t ra) {
+                                             uint32_t nbc, int arm, uint32_t rw) {
   const uint32_t w = W[c];
   const uint32_t wn = W[nbc];
   const uint32_t sa = sigma_of(w, arm);
-  const uint32_t sp = uniform_int(ru, 6);
+  uint32_t sp, ra;
+  split_word(rw, 6, sp, ra);
   if (sp == sa) return;
   const uint32_t snb = sigma_of(wn, arm ^ 1);
   const int h = eta_of(w, arm) ? ((int)(sp == snb) - (int)(sa == snb)) : 0;
@@ -238,71 +260,72 @@ __device__ __forceinline__ void sigma_update(uint32_t* __restrict__ W, const uin
   if ((uint64_t)ra < thr[(h + 1) * 11 + d + 5]) W[c] = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
 }
 
-// One thread per pair of active cells (x,y,z) and (x,y,z+L/2), z < L/2: both
-// have the same parity along every axis (L/2 is even) and share one Philox
-// block (words x,y for the lower cell, z,w for the upper).
-__global__ __launch_bounds__(kBlock) void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb,
-                                                  const PointDev* __restrict__ pts,
-                                                  const PointState* __restrict__ ps,
-                                                  uint32_t* __restrict__ words,
-                                                  const uint8_t* __restrict__ sched, uint32_t sub) {
+// One thread per quad of active cells (x, y + a L/2, z + b L/2), a,b in
+// {0,1}, y,z < L/2: the four have the same parity along every axis (L/2 is
+// even) and share one Philox block keyed on the a=b=0 cell (word a + 2b).
+// The half-sweep's (arm, parity) is drawn in the kernel from uniform values
+// (gpu_water3D.cu:1393-1394 draws them on the host).
+__global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
+                        const PointDev* __restrict__ pts, uint32_t* __restrict__ words, uint32_t sub) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
-  const PointDev& P = pts[e.point];
   __shared__ uint64_t thr[33];
-  if (threadIdx.x < 33) thr[threadIdx.x] = P.thr_sigma[threadIdx.x];
+  if (threadIdx.x < 33) thr[threadIdx.x] = pts[e.point].thr_sigma[threadIdx.x];
   __syncthreads();
-  const uint32_t L = P.L, hL = P.hL, N = P.N;
+  const uint32_t L = e.L, hL = L / 2, N = e.N;
   const uint32_t t = e.start + threadIdx.x;
-  if (t >= N / 4) return;
-  const uint64_t step = load_step(ps, e.point);
-  const uint32_t sc = sched[P.sched_off + sub];
-  const int arm = (int)(sc & 7u);
-  const uint32_t q = (sc >> 3) ? 0u : 1u;  // kept parity along the axis
+  if (t >= N / 8) return;
+  const uint64_t step = *stepp;
+  const uint64_t seed = seed_of(e);
+  const U4 sc = draw(seed, kStSched, 0, step, sub);
+  const int arm = (int)uniform_int(sc.x, 6);
+  const uint32_t q = (sc.y >> 31) ? 0u : 1u;  // kept parity along the axis
   const int ax = arm >> 1;
-  uint32_t x, y, z;  // lower cell, z < L/2
-  if (ax == 0) {
-    const uint32_t r1 = fdiv(t, P.divH);
+  const FastDiv dL{L, e.mL}, dH{hL, e.mH};
+  uint32_t x, y, z;  // quad origin, y,z < L/2
+  if (ax == 0) {  // t = hx + hL (y + hL z)
+    const uint32_t r1 = fdiv(t, dH);
     x = 2 * (t - r1 * hL) + q;
-    z = fdiv(r1, P.divL);
-    y = r1 - z * L;
-  } else if (ax == 1) {
-    const uint32_t r1 = fdiv(t, P.divL);
+    z = fdiv(r1, dH);
+    y = r1 - z * hL;
+  } else if (ax == 1) {  // t = x + L (z + hL hy)
+    const uint32_t r1 = fdiv(t, dL);
     x = t - r1 * L;
-    z = fdiv(r1, P.divH);
-    y = 2 * (r1 - z * hL) + q;
-  } else {
-    const uint32_t r1 = fdiv(t, P.divL);
+    const uint32_t hy = fdiv(r1, dH);
+    z = r1 - hy * hL;
+    y = 2 * hy + q;
+  } else {  // t = x + L (y + hL hz)
+    const uint32_t r1 = fdiv(t, dL);
     x = t - r1 * L;
-    const uint32_t hz = fdiv(r1, P.divL);
-    y = r1 - hz * L;
+    const uint32_t hz = fdiv(r1, dH);
+    y = r1 - hz * hL;
     z = 2 * hz + q;
   }
-  const uint32_t c = x + L * (y + L * z);
-  const uint32_t c2 = c + N / 2;
-  uint32_t nb1, nb2;
-  if (ax == 2) {
-    const uint32_t LL = L * L, z2 = z + hL;
-    if (arm & 1) {
-      nb1 = z == 0 ? c + (L - 1) * LL : c - LL;
-      nb2 = c2 - LL;  // z2 >= L/2 > 0
-    } else {
-      nb1 = c + LL;  // z < L/2 <= L-1
-      nb2 = z2 == L - 1 ? c2 - (L - 1) * LL : c2 + LL;
-    }
+  const uint32_t dy = hL * L, dz = N / 2;
+  const uint32_t c0 = x + L * (y + L * z);
+  const uint32_t c[4] = {c0, c0 + dy, c0 + dz, c0 + dy + dz};
+  uint32_t nbc[4];
+  if (ax == 0) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) nbc[m] = arm_neighbor(c[m], x, 1, L, arm);
+  } else if (ax == 1) {
+    nbc[0] = arm_neighbor(c[0], y, L, L, arm);
+    nbc[1] = arm_neighbor(c[1], y + hL, L, L, arm);
+    nbc[2] = nbc[0] + dz;
+    nbc[3] = nbc[1] + dz;
   } else {
-    const uint32_t coord = ax == 0 ? x : y;
-    const uint32_t stride = ax == 0 ? 1u : L;
-    if (arm & 1)
-      nb1 = coord == 0 ? c + (L - 1) * stride : c - stride;
-    else
-      nb1 = coord == L - 1 ? c - (L - 1) * stride : c + stride;
-    nb2 = nb1 + N / 2;
+    const uint32_t LL = L * L;
+    nbc[0] = arm_neighbor(c[0], z, LL, L, arm);
+    nbc[2] = arm_neighbor(c[2], z + hL, LL, L, arm);
+    nbc[1] = nbc[0] + dy;
+    nbc[3] = nbc[2] + dy;
   }
-  uint32_t* W = words + P.woff;
-  const U4 r = draw(P.seed, kStSigma, c, step, sub);
-  sigma_update(W, thr, c, nb1, arm, r.x, r.y);
-  sigma_update(W, thr, c2, nb2, arm, r.z, r.w);
+  uint32_t* W = words + e.woff;
+  const U4 r = draw(seed, kStSigma, c0, step, sub);
+  sigma_update(W, thr, c[0], nbc[0], arm, r.x);
+  sigma_update(W, thr, c[1], nbc[1], arm, r.y);
+  sigma_update(W, thr, c[2], nbc[2], arm, r.z);
+  sigma_update(W, thr, c[3], nbc[3], arm, r.w);
 }
 
 // ----------------------------------------------------------- observables
@@ -315,27 +338,21 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
     p[i] = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_observe(const BlockEnt* __restrict__ tab, uint32_t nb,
-                                                    const PointDev* __restrict__ pts,
-                                                    const uint32_t* __restrict__ words,
-                                                    unsigned long long* __restrict__ counters) {
+__global__ void k_observe(const BlockEnt* __restrict__ tab, uint32_t nb,
+                          const uint32_t* __restrict__ words,
+                          unsigned long long* __restrict__ counters) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
-  const PointDev& P = pts[e.point];
-  const uint32_t L = P.L;
+  const uint32_t L = e.L;
   const uint32_t t = e.start + threadIdx.x;
   uint32_t nhb2 = 0, nsig = 0;
-  if (t < P.N) {
-    const uint32_t r1 = fdiv(t, P.divL);
-    const uint32_t x = t - r1 * L;
-    const uint32_t z = fdiv(r1, P.divL);
-    const uint32_t y = r1 - z * L;
-    const uint32_t* W = words + P.woff;
+  if (t < e.N) {
+    uint32_t x, y, z;
+    cell_coords(e, t, x, y, z);
+    const uint32_t* W = words + e.woff;
     const uint32_t w = W[t];
-    const uint32_t nx = x == L - 1 ? t - (L - 1) : t + 1;
-    const uint32_t ny = y == L - 1 ? t - (L - 1) * L : t + L;
-    const uint32_t nz = z == L - 1 ? t - (L - 1) * L * L : t + L * L;
-    const uint32_t nbs[3] = {nx, ny, nz};
+    const uint32_t nbs[3] = {neighbor_up(t, x, 1, L), neighbor_up(t, y, L, L),
+                             neighbor_up(t, z, L * L, L)};
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
       const uint32_t wn = W[nbs[ax]];
@@ -394,55 +411,45 @@ __device__ void uf_unite(uint32_t* par, uint32_t a, uint32_t b) {
   }
 }
 
-__device__ __forceinline__ void cell_coords(const PointDev& P, uint32_t t, uint32_t& x, uint32_t& y,
-                                            uint32_t& z) {
-  const uint32_t r1 = fdiv(t, P.divL);
-  x = t - r1 * P.L;
-  z = fdiv(r1, P.divL);
-  y = r1 - z * P.L;
-}
-
 // Links of one cell: 3 covalent (arm 0|2|4 of c with arm 1|3|5 of the +x|+y|+z
 // neighbour) and 15 intra-molecule (arm pairs k<l in order).  Writes the
 // intra-cell forest into `parent` and the active covalent links into cov[c].
-__global__ __launch_bounds__(kBlock) void k_sw_local(const BlockEnt* __restrict__ tab, uint32_t nb,
-                                                     const PointDev* __restrict__ pts,
-                                                     const PointState* __restrict__ ps,
-                                                     const uint32_t* __restrict__ words,
-                                                     uint32_t* __restrict__ parent,
-                                                     uint8_t* __restrict__ cov, uint32_t jc) {
+__global__ void k_sw_local(const BlockEnt* __restrict__ tab, uint32_t nb,
+                           const uint64_t* __restrict__ stepp, const uint32_t* __restrict__ words,
+                           uint32_t* __restrict__ parent, uint8_t* __restrict__ cov, uint32_t jc) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
-  const PointDev& P = pts[e.point];
   const uint32_t t = e.start + threadIdx.x;
-  if (t >= P.N) return;
-  const uint32_t L = P.L, N = P.N;
-  const uint64_t step = load_step(ps, e.point);
+  if (t >= e.N) return;
+  const uint32_t L = e.L, N = e.N;
+  const uint64_t step = *stepp;
   uint32_t x, y, z;
-  cell_coords(P, t, x, y, z);
-  const uint32_t* W = words + P.woff;
+  cell_coords(e, t, x, y, z);
+  const uint32_t* W = words + e.woff;
   const uint32_t w = W[t];
-  const uint32_t nbs[3] = {x == L - 1 ? t - (L - 1) : t + 1, y == L - 1 ? t - (L - 1) * L : t + L,
-                           z == L - 1 ? t - (L - 1) * L * L : t + L * L};
+  const uint32_t nbs[3] = {neighbor_up(t, x, 1, L), neighbor_up(t, y, L, L),
+                           neighbor_up(t, z, L * L, L)};
   uint32_t rw[20];
+  const uint64_t seed = seed_of(e);
 #pragma unroll
   for (int g = 0; g < 5; ++g) {
-    const U4 r = draw(P.seed, kStSwBond, t, step, jc * 5 + g);
+    const U4 r = draw(seed, kStSwBond, t, step, jc * 5 + g);
     rw[4 * g] = r.x;
     rw[4 * g + 1] = r.y;
     rw[4 * g + 2] = r.z;
     rw[4 * g + 3] = r.w;
   }
-  const uint64_t thb = P.thr_sw[0], ts = P.thr_sw[1];
+  const uint64_t thb = ((uint64_t)e.thb_hi << 32) | e.thb_lo;
+  const uint64_t ts = ((uint64_t)e.ts_hi << 32) | e.ts_lo;
   uint32_t cmask = 0;
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
     const uint32_t wn = W[nbs[ax]];
     const uint32_t s0 = sigma_of(w, 2 * ax), s1 = sigma_of(wn, 2 * ax + 1);
     bool act = false;
-    if (P.jeff_sign > 0)
+    if (e.jeff_sign > 0)
       act = eta_of(w, 2 * ax) && s0 == s1 && (uint64_t)rw[ax] < thb;
-    else if (P.jeff_sign < 0)
+    else if (e.jeff_sign < 0)
       act = s0 != s1 && (uint64_t)rw[ax] < thb;
     cmask |= (uint32_t)act << ax;
   }
@@ -462,33 +469,30 @@ __global__ __launch_bounds__(kBlock) void k_sw_local(const BlockEnt* __restrict_
       }
       ++link;
     }
-  uint32_t* par = parent + P.loff;
+  uint32_t* par = parent + e.loff;
 #pragma unroll
   for (int a = 0; a < 6; ++a) {
     uint32_t r = a;
     while (lab[r] != r) r = lab[r];
     par[a * N + t] = r * N + t;
   }
-  cov[P.woff + t] = (uint8_t)cmask;
+  cov[e.woff + t] = (uint8_t)cmask;
 }
 
-__global__ __launch_bounds__(kBlock) void k_sw_union(const BlockEnt* __restrict__ tab, uint32_t nb,
-                                                     const PointDev* __restrict__ pts,
-                                                     uint32_t* __restrict__ parent,
-                                                     const uint8_t* __restrict__ cov) {
+__global__ void k_sw_union(const BlockEnt* __restrict__ tab, uint32_t nb, uint32_t* __restrict__ parent,
+                           const uint8_t* __restrict__ cov) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
-  const PointDev& P = pts[e.point];
   const uint32_t t = e.start + threadIdx.x;
-  if (t >= P.N) return;
-  const uint32_t cm = cov[P.woff + t];
+  if (t >= e.N) return;
+  const uint32_t cm = cov[e.woff + t];
   if (!cm) return;
-  const uint32_t L = P.L, N = P.N;
+  const uint32_t L = e.L, N = e.N;
   uint32_t x, y, z;
-  cell_coords(P, t, x, y, z);
-  const uint32_t nbs[3] = {x == L - 1 ? t - (L - 1) : t + 1, y == L - 1 ? t - (L - 1) * L : t + L,
-                           z == L - 1 ? t - (L - 1) * L * L : t + L * L};
-  uint32_t* par = parent + P.loff;
+  cell_coords(e, t, x, y, z);
+  const uint32_t nbs[3] = {neighbor_up(t, x, 1, L), neighbor_up(t, y, L, L),
+                           neighbor_up(t, z, L * L, L)};
+  uint32_t* par = parent + e.loff;
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax)
     if (cm & (1u << ax)) uf_unite(par, (2 * ax) * N + t, (2 * ax + 1) * N + nbs[ax]);
@@ -496,20 +500,18 @@ __global__ __launch_bounds__(kBlock) void k_sw_union(const BlockEnt* __restrict_
 
 // Final labels (full path compression) and the cluster rotation of
 // gpu_update_cluster (gpu_water3D.cu:2531-2542): sigma += floor(6u(label)).
-__global__ __launch_bounds__(kBlock) void k_sw_flip(const BlockEnt* __restrict__ tab, uint32_t nb,
-                                                    const PointDev* __restrict__ pts,
-                                                    const PointState* __restrict__ ps,
-                                                    uint32_t* __restrict__ words,
-                                                    uint32_t* __restrict__ parent, uint32_t jc) {
+__global__ void k_sw_flip(const BlockEnt* __restrict__ tab, uint32_t nb,
+                          const uint64_t* __restrict__ stepp, uint32_t* __restrict__ words,
+                          uint32_t* __restrict__ parent, uint32_t jc) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
-  const PointDev& P = pts[e.point];
   const uint32_t t = e.start + threadIdx.x;
-  if (t >= P.N) return;
-  const uint32_t N = P.N;
-  const uint64_t step = load_step(ps, e.point);
-  uint32_t* par = parent + P.loff;
-  uint32_t* W = words + P.woff;
+  if (t >= e.N) return;
+  const uint32_t N = e.N;
+  const uint64_t step = *stepp;
+  const uint64_t seed = seed_of(e);
+  uint32_t* par = parent + e.loff;
+  uint32_t* W = words + e.woff;
   uint32_t w = W[t];
   uint32_t nw = w & ~kSigmaBits;
 #pragma unroll
@@ -517,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_sw_flip(const BlockEnt* __restrict__
     const uint32_t node = a * N + t;
     const uint32_t root = uf_find(par, node);
     __hip_atomic_store(par + node, root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const U4 r = draw(P.seed, kStSwFlip, root, step, jc);
+    const U4 r = draw(seed, kStSwFlip, root, step, jc);
     uint32_t s = sigma_of(w, a) + uniform_int(r.x, 6);
     s = s >= 6 ? s - 6 : s;
     nw |= s << (3 * a);
@@ -599,71 +601,75 @@ __global__ __launch_bounds__(64) void k_vdw_eval(const float* __restrict__ dist,
 // ------------------------------------------------------------ launchers
 static inline dim3 grid1(size_t n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
 
-void launch_step_begin(hipStream_t st, const PointDev* pts, int npts, PointState* ps, uint8_t* sched,
-                       int nsched, const float* dist, const int32_t* freq, int vol_device,
-                       int precision) {
-  hipLaunchKernelGGL(k_step_begin, dim3(npts), dim3(64), 0, st, pts, ps, sched, nsched, dist, freq,
-                     vol_device, precision);
+// When set, the next launches record their own execution time into the two
+// events (hipExtLaunchKernelGGL timestamps the dispatch itself).
+static thread_local hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
+void set_launch_timing(hipEvent_t start, hipEvent_t stop) {
+  g_t0 = start;
+  g_t1 = stop;
 }
-void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps, uint32_t* words,
-                int family) {
-  hipLaunchKernelGGL(k_eta, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts, ps, words,
-                     family);
+
+template <typename K, typename... A>
+static void launch(K kernel, dim3 grid, dim3 block, hipStream_t st, A... args) {
+  if (g_t0)
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, g_t0, g_t1, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
 }
-void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
-                  uint32_t* words, const uint8_t* sched, uint32_t sub) {
-  hipLaunchKernelGGL(k_sigma, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts, ps,
-                     words, sched, sub);
+
+void launch_step_begin(hipStream_t st, const PointDev* pts, const int32_t* plist, int npts,
+                       PointState* ps, uint64_t* cur_step, const float* dist, const int32_t* freq,
+                       int vol_device, int precision) {
+  launch(k_step_begin, dim3(npts), dim3(64), st, pts, plist, ps, cur_step, dist, freq, vol_device,
+         precision);
+}
+void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, int family) {
+  launch(k_eta, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, pts, words, family);
+}
+void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, uint32_t sub) {
+  launch(k_sigma, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, pts, words, sub);
 }
 void launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
-  size_t blocks = (n + kBlock - 1) / kBlock;
+  size_t blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)blocks), dim3(kBlock), 0, st, p, n);
+  launch(k_zero_u32, dim3((unsigned)blocks), dim3(256), st, p, n);
 }
-void launch_observe(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* words,
-                    unsigned long long* counters) {
-  hipLaunchKernelGGL(k_observe, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts, words,
-                     counters);
+void launch_observe(hipStream_t st, Launch ln, const uint32_t* words, unsigned long long* counters) {
+  launch(k_observe, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, words, counters);
 }
-void launch_sw_local(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
-                     const uint32_t* words, uint32_t* parent, uint8_t* cov, uint32_t jc) {
-  hipLaunchKernelGGL(k_sw_local, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts, ps,
-                     words, parent, cov, jc);
+void launch_sw_local(hipStream_t st, Launch ln, const uint32_t* words, uint32_t* parent, uint8_t* cov,
+                     uint32_t jc) {
+  launch(k_sw_local, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, words, parent,
+         cov, jc);
 }
-void launch_sw_union(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* parent,
-                     const uint8_t* cov) {
-  hipLaunchKernelGGL(k_sw_union, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts,
-                     parent, cov);
+void launch_sw_union(hipStream_t st, Launch ln, uint32_t* parent, const uint8_t* cov) {
+  launch(k_sw_union, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, parent, cov);
 }
-void launch_sw_flip(hipStream_t st, Launch ln, const PointDev* pts, const PointState* ps,
-                    uint32_t* words, uint32_t* parent, uint32_t jc) {
-  hipLaunchKernelGGL(k_sw_flip, dim3(ln.nblocks), dim3(kBlock), 0, st, ln.tab, ln.nblocks, pts, ps,
-                     words, parent, jc);
+void launch_sw_flip(hipStream_t st, Launch ln, uint32_t* words, uint32_t* parent, uint32_t jc) {
+  launch(k_sw_flip, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, words, parent,
+         jc);
 }
 void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, int32_t* size) {
-  hipLaunchKernelGGL(k_cluster_sizes, grid1(nn, kBlock), dim3(kBlock), 0, st, label, nn, size);
+  launch(k_cluster_sizes, grid1(nn, 256), dim3(256), st, label, nn, size);
 }
 void launch_perc_slots(hipStream_t st, const uint32_t* label, uint32_t nn, const int32_t* size,
                        int32_t minsize, int32_t* slot, uint32_t* nslot, uint32_t max_slots) {
-  hipLaunchKernelGGL(k_perc_slots, grid1(nn, kBlock), dim3(kBlock), 0, st, label, nn, size, minsize,
-                     slot, nslot, max_slots);
+  launch(k_perc_slots, grid1(nn, 256), dim3(256), st, label, nn, size, minsize, slot, nslot, max_slots);
 }
 void launch_perc_cover(hipStream_t st, const uint32_t* label, uint32_t nn, uint32_t L,
                        const int32_t* size, int32_t minsize, const int32_t* slot, uint32_t* cover,
                        uint32_t max_slots) {
-  hipLaunchKernelGGL(k_perc_cover, grid1(nn, kBlock), dim3(kBlock), 0, st, label, nn, L, size,
-                     minsize, slot, cover, max_slots);
+  launch(k_perc_cover, grid1(nn, 256), dim3(256), st, label, nn, L, size, minsize, slot, cover,
+         max_slots);
 }
 void launch_perc_count(hipStream_t st, const uint32_t* cover, const uint32_t* nslot, uint32_t L,
                        uint32_t max_slots, uint32_t* nperc) {
-  hipLaunchKernelGGL(k_perc_count, grid1(max_slots, kBlock), dim3(kBlock), 0, st, cover, nslot, L,
-                     max_slots, nperc);
+  launch(k_perc_count, grid1(max_slots, 256), dim3(256), st, cover, nslot, L, max_slots, nperc);
 }
 void launch_vdw_eval(hipStream_t st, const float* dist, const int32_t* freq, int nshell,
                      float r_cutoff, int N, const double* r, int n_r, int precision, double* out) {
-  hipLaunchKernelGGL(k_vdw_eval, dim3(n_r), dim3(64), 0, st, dist, freq, nshell, r_cutoff, N, r,
-                     precision, out);
+  launch(k_vdw_eval, dim3(n_r), dim3(64), st, dist, freq, nshell, r_cutoff, N, r, precision, out);
 }
 
 }  // namespace cvf

@@ -130,6 +130,22 @@ int cvf_sync(cvf_sim *s);
 const char *cvf_last_error(void);
 void cvf_destroy(cvf_sim *s);
 
+/* Launch configuration of a handle.  It changes speed only: every choice
+ * gives bit-identical trajectories (the random numbers are keyed on sites,
+ * not on threads).  No reference counterpart: the reference's launch shape
+ * is fixed at N/512 blocks of 512 threads (gpu_water3D.cu:68, 517-518). */
+typedef struct cvf_tuning {
+  int32_t streams;      /* groups of state points, each on its own stream (1..8) */
+  int32_t block_sigma;  /* workgroup size of the sigma half-sweep (64..1024, power of 2) */
+  int32_t block_eta;    /* workgroup size of the eta cube sweep */
+  int32_t block_cell;   /* workgroup size of the per-cell kernels (SW, observables) */
+  int32_t graph_steps;  /* MC steps per replayed hipGraph (1..64) */
+} cvf_tuning;
+
+int cvf_get_tuning(cvf_sim *s, cvf_tuning *t);
+/* Rebuilds the groups, block tables and graphs; synchronises the handle. */
+int cvf_set_tuning(cvf_sim *s, const cvf_tuning *t);
+
 /* Kernel classes of one MC step (profiling). */
 enum {
   CVF_KCLASS_BEGIN = 0,   /* step counter, schedule, device volume move */
@@ -143,7 +159,8 @@ enum {
 
 /* Instrumented cvf_step for measurement (no reference counterpart; the
  * reference's timers are commented out, gpu_water3D.cu:1180-1192): runs
- * n_steps MC steps with individual launches, a HIP event after each, and
+ * n_steps MC steps with individual launches, each timed by start/stop HIP
+ * events recorded by the dispatch itself (hipExtLaunchKernelGGL), and
  * returns the mean duration in ms of one launch of each kernel class and
  * the launch counts.  Needs the device volume mode. */
 int cvf_profile_steps(cvf_sim *s, int64_t n_steps, double *kernel_ms, int64_t *launches,

@@ -79,6 +79,15 @@ static double u53(uint32_t a, uint32_t b) {
   return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
 }
 
+/* One 32-bit word per site update: with X = q*w (35-bit), the proposal is
+ * X >> 32 = floor(q*u) and the acceptance uniform is the fractional part
+ * X mod 2^32 (resolution q / 2^32), u = w / 2^32. */
+static void split_word(uint32_t w, uint32_t q, uint32_t *prop, uint32_t *acc) {
+  uint64_t X = (uint64_t)w * q;
+  *prop = (uint32_t)(X >> 32);
+  *acc = (uint32_t)X;
+}
+
 static uint64_t thr_from_p(double p) {
   if (!(p < 1.0)) return (uint64_t)1 << 32;
   if (p <= 0.0) return 0;
@@ -388,19 +397,24 @@ void ora_eta_sweep(ora_sim *s, uint64_t step) {
   int N = s->N, NC = s->NC;
   for (int cube = 0; cube < NC; ++cube) {
     int old = s->state[cube];
-    /* cubes with origin z and z + L/2 share one Philox block: the lower
-     * cube uses words 0,1, the upper one words 2,3 (site = lower cube) */
-    int upper = cube >= NC / 2;
-    uint32_t r[4];
-    draw(s->p.seed, ORA_ST_ETA, (uint32_t)(upper ? cube - NC / 2 : cube), step, 0, r);
-    int nw = (int)uniform_int(r[upper ? 2 : 0], 9);
+    /* the four cubes with origins (ox, oy + a L/2, oz + b L/2), a,b in {0,1},
+     * share one Philox block keyed on the a=b=0 cube; cube a,b uses word
+     * a + 2b (cube index = x/2 + (L/2)(y/2) + (L^2/4) z, 3405) */
+    int L = s->L, hL = L / 2;
+    int oy2 = (cube / hL) % hL, oz = cube / (hL * hL);
+    int ay = oy2 >= L / 4, bz = oz >= hL;
+    int site = cube - ay * hL * (L / 4) - bz * hL * hL * hL;
+    uint32_t r[4], prop, acc;
+    draw(s->p.seed, ORA_ST_ETA, (uint32_t)site, step, 0, r);
+    split_word(r[ay + 2 * bz], 9, &prop, &acc);
+    int nw = (int)prop;
     int dn = 0;
     for (int j = 0; j < EDGES; ++j) {
       int v0 = s->vert[cube + j * NC], v1 = s->vert[cube + j * NC + NC * EDGES];
       int a0 = edge_arm0(j);
       dn += (Sol[nw][j] - Sol[old][j]) * (s->s[a0 * N + v0] == s->s[(a0 + 1) * N + v1]);
     }
-    if (!((uint64_t)r[upper ? 3 : 1] < s->thr_eta[dn + 4])) nw = old;
+    if (!((uint64_t)acc < s->thr_eta[dn + 4])) nw = old;
     s->state[cube] = nw;
     write_cube_edges(s, cube);
   }
@@ -429,12 +443,15 @@ void ora_sigma_halfsweep(ora_sim *s, uint64_t step, uint32_t sub, int arm, int i
     int nb = neighbor(L, c, arm);
     int nbspin = s->s[(arm ^ 1) * N + nb];
     int old = s->s[arm * N + c];
-    /* cells (x,y,z) and (x,y,z+L/2) are active together (L/2 is even) and
-     * share one Philox block: words 0,1 for the lower, 2,3 for the upper */
-    int upper = z >= L / 2;
-    uint32_t r[4];
-    draw(s->p.seed, ORA_ST_SIGMA, (uint32_t)(upper ? c - N / 2 : c), step, sub, r);
-    int nw = (int)uniform_int(r[upper ? 2 : 0], 6);
+    /* the four cells (x, y + a L/2, z + b L/2), a,b in {0,1}, are active
+     * together (L/2 is even) and share one Philox block keyed on the a=b=0
+     * cell; cell a,b uses word a + 2b */
+    int ay = y >= L / 2, bz = z >= L / 2;
+    int site = c - ay * (L / 2) * L - bz * (N / 2);
+    uint32_t r[4], prop, acc;
+    draw(s->p.seed, ORA_ST_SIGMA, (uint32_t)site, step, sub, r);
+    split_word(r[ay + 2 * bz], 6, &prop, &acc);
+    int nw = (int)prop;
     if (nw == old) continue;
     int h = s->eta[arm * N + c] * ((nw == nbspin) - (old == nbspin));
     int d = 0;
@@ -443,7 +460,7 @@ void ora_sigma_halfsweep(ora_sim *s, uint64_t step, uint32_t sub, int arm, int i
       int sk = s->s[k * N + c];
       d += (nw == sk) - (old == sk);
     }
-    if ((uint64_t)r[upper ? 3 : 1] < s->thr_sigma[(h + 1) * 11 + (d + 5)]) s->s[arm * N + c] = (uint8_t)nw;
+    if ((uint64_t)acc < s->thr_sigma[(h + 1) * 11 + (d + 5)]) s->s[arm * N + c] = (uint8_t)nw;
   }
 }
 

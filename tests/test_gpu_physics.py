@@ -165,3 +165,27 @@ def test_host_and_device_volume_modes_agree():
     b.step(500)
     assert a.volume() == b.volume()
     assert a.observe().E == b.observe().E
+
+
+@pytest.mark.parametrize("tuning", [
+    {"streams": 1, "block_sigma": 256, "block_eta": 256, "block_cell": 256, "graph_steps": 1},
+    {"streams": 3, "block_sigma": 64, "block_eta": 128, "block_cell": 64, "graph_steps": 5},
+    {"streams": 8, "block_sigma": 1024, "block_eta": 512, "block_cell": 1024, "graph_steps": 16},
+])
+def test_tuning_does_not_change_results(tuning):
+    """Launch configuration (streams, workgroup sizes, steps per graph) never
+    changes the trajectory: random numbers are keyed on sites."""
+    pts = batch_points([8, 16, 12, 24, 8, 20], nc=1)
+    ref = Simulation(pts)
+    alt = Simulation(pts)
+    alt.tuning = tuning
+    assert alt.tuning == tuning
+    ref.step(37)
+    alt.step(20)
+    alt.tuning = {"streams": 2}
+    alt.step(17)
+    for i in range(len(pts)):
+        a, b = ref.get_config(i), alt.get_config(i)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert np.array_equal(ref.labels(i), alt.labels(i))
+        assert ref.volume(i) == alt.volume(i)

@@ -356,9 +356,11 @@ def test_sigma_acceptance_is_boltzmann():
             if (ind and coord % 2) or (not ind and not coord % 2):
                 assert s1[arm * N + c] == s0[arm * N + c]
                 continue
-            up = z >= L // 2  # (x,y,z) and (x,y,z+L/2) share one Philox block
-            r = O.draw(seed, O.ST_SIGMA, c - N // 2 if up else c, step, 0)
-            prop = (int(r[2 if up else 0]) * 6) >> 32
+            # (x, y + a L/2, z + b L/2) share one Philox block, word a + 2b;
+            # proposal floor(6w/2^32), acceptance (6w) mod 2^32
+            a_, b_ = int(y >= L // 2), int(z >= L // 2)
+            r = O.draw(seed, O.ST_SIGMA, c - a_ * (L // 2) * L - b_ * (N // 2), step, 0)
+            prop = (int(r[a_ + 2 * b_]) * 6) >> 32
             if prop == s0[arm * N + c]:
                 continue
             t = s0.copy()
@@ -421,8 +423,10 @@ def test_eta_acceptance_is_boltzmann():
         eta_invariants(L, s1, e1)
         base = hamiltonian_counts(L, s0, e0)[0]
         for cube, edges in cubes:
-            up = cube >= N // 8  # cubes at oz and oz+L/2 share one Philox block
-            prop = (int(O.draw(seed, O.ST_ETA, cube - N // 8 if up else cube, step, 0)[2 if up else 0]) * 9) >> 32
+            hL = L // 2  # cubes (ox, oy + a L/2, oz + b L/2) share one Philox block
+            a_, b_ = int((cube // hL) % hL >= L // 4), int(cube // (hL * hL) >= hL)
+            site = cube - a_ * hL * (L // 4) - b_ * hL ** 3
+            prop = (int(O.draw(seed, O.ST_ETA, site, step, 0)[a_ + 2 * b_]) * 9) >> 32
             acc = st1[cube] == prop
             assert acc or st1[cube] == st0[cube]
             e = e0.copy()
