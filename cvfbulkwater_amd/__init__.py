@@ -59,7 +59,8 @@ class _Obs(C.Structure):
 
 class _Tuning(C.Structure):
     _fields_ = [("streams", C.c_int32), ("block_sigma", C.c_int32), ("block_eta", C.c_int32),
-                ("block_cell", C.c_int32), ("graph_steps", C.c_int32)]
+                ("block_cell", C.c_int32), ("graph_steps", C.c_int32), ("fused", C.c_int32),
+                ("slab_planes", C.c_int32), ("block_chunk", C.c_int32)]
 
 
 class _CStat(C.Structure):
@@ -276,7 +277,8 @@ class Simulation:
                                                    f.ctypes.data_as(C.POINTER(C.c_int32)), d.size),
                "cvf_set_shells")
 
-    TUNING_FIELDS = ("streams", "block_sigma", "block_eta", "block_cell", "graph_steps")
+    TUNING_FIELDS = ("streams", "block_sigma", "block_eta", "block_cell", "graph_steps", "fused",
+                     "slab_planes", "block_chunk")
 
     @property
     def tuning(self) -> dict:
@@ -291,13 +293,14 @@ class Simulation:
         t = _Tuning(*[int(cur[k]) for k in self.TUNING_FIELDS])
         _check(self._lib, self._lib.cvf_set_tuning(self._h, C.byref(t)), "cvf_set_tuning")
 
-    KCLASSES = ("begin", "eta", "sigma", "sw_local", "sw_union", "sw_flip")
+    KCLASSES = ("begin", "eta", "sigma", "sw_local", "sw_union", "sw_flip", "chunk")
 
     def profile_steps(self, n: int) -> dict:
         """Instrumented steps: mean ms per launch and launch count per kernel class."""
-        ms = (C.c_double * 6)()
-        cnt = (C.c_int64 * 6)()
-        _check(self._lib, self._lib.cvf_profile_steps(self._h, n, ms, cnt, 6), "cvf_profile_steps")
+        k = len(self.KCLASSES)
+        ms = (C.c_double * k)()
+        cnt = (C.c_int64 * k)()
+        _check(self._lib, self._lib.cvf_profile_steps(self._h, n, ms, cnt, k), "cvf_profile_steps")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KCLASSES)}
 
     # -- measurement

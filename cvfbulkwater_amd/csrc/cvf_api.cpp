@@ -79,9 +79,12 @@ struct Group {
   std::vector<int> points;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  DevBuf<BlockEnt> tab_half, tab_cube, tab_cell;
+  DevBuf<BlockEnt> tab_half, tab_cube, tab_cell, tab_slab;
   DevBuf<int32_t> plist;
-  uint32_t nb_half = 0, nb_cube = 0, nb_cell = 0;
+  uint32_t nb_half = 0, nb_cube = 0, nb_cell = 0, nb_slab = 0;
+  bool fused = false;        // eta + sigma through the fused slab kernel
+  uint32_t chunk_lds = 0;    // dynamic LDS bytes of a fused launch
+  size_t sigma_cells = 0;    // cells swept per fused launch
   hipGraph_t graph = nullptr, graphK = nullptr;
   hipGraphExec_t gexec = nullptr, gexecK = nullptr;
   size_t cells = 0;
@@ -112,7 +115,7 @@ struct cvf_sim {
   size_t total_cells = 0, total_nodes = 0;
   cvf_tuning tune{};
 
-  DevBuf<uint32_t> words, parent, cover, nslot_nperc;
+  DevBuf<uint32_t> words, words2, parent, cover, nslot_nperc;  // words2: ping-pong partner
   DevBuf<uint8_t> cov;
   DevBuf<int32_t> csize, slot;
   DevBuf<PointDev> pts;
@@ -315,7 +318,29 @@ BlockEnt make_entry(const PointHost& P, int point, uint32_t start) {
   e.thb_hi = (uint32_t)(P.thr.sw_hb >> 32);
   e.ts_lo = (uint32_t)P.thr.sw_s;
   e.ts_hi = (uint32_t)(P.thr.sw_s >> 32);
+  const uint32_t LL = P.L * P.L;
+  e.mLL = make_fastdiv(LL).m;
+  e.mP8 = make_fastdiv(LL / 8).m;
+  e.mP4 = make_fastdiv(LL / 4).m;
+  e.mC = make_fastdiv((P.L / 4) * (P.L / 4)).m;
+  e.T = 0;
+  e.pad[0] = e.pad[1] = e.pad[2] = 0;
   return e;
+}
+
+// LDS words of a fused launch for slabs of T planes: T + one plane per
+// half-sweep of the chunk (worst-case halo) + one per side for the cubes.
+uint32_t slab_lds_bytes(uint32_t L, uint32_t T) { return (T + kChunk + 2) * L * L * 4; }
+
+// Planes per slab: the requested count if it divides L and fits, else the
+// largest divisor of L up to 8 that fits in LDS; 0 if none does.
+uint32_t choose_slab(uint32_t L, int requested) {
+  if (requested > 0 && L % (uint32_t)requested == 0 &&
+      slab_lds_bytes(L, (uint32_t)requested) <= chunk_lds_limit())
+    return (uint32_t)requested;
+  for (uint32_t T = 8; T >= 1; --T)
+    if (L % T == 0 && slab_lds_bytes(L, T) <= chunk_lds_limit()) return T;
+  return 0;
 }
 
 // Block table over `count(P)` work items per point of `points`, `block`
@@ -407,6 +432,36 @@ int build_groups(cvf_sim* s) {
     g->nb_cell = (uint32_t)tl.size();
     std::vector<int32_t> pl(g->points.begin(), g->points.end());
     if ((rc = upload_vec(g->plist, pl))) return rc;
+    // fused path when every point's slab fits in LDS
+    g->fused = s->tune.fused != 0;
+    std::vector<BlockEnt> ts;
+    g->chunk_lds = 0;
+    g->sigma_cells = 0;
+    for (int p : g->points) {
+      const PointHost& P = s->ph[p];
+      const uint32_t T = choose_slab(P.L, s->tune.slab_planes);
+      if (!T) {
+        g->fused = false;
+        break;
+      }
+      g->chunk_lds = std::max(g->chunk_lds, slab_lds_bytes(P.L, T));
+      g->sigma_cells += P.N;
+      for (uint32_t z0 = 0; z0 < P.L; z0 += T) {
+        BlockEnt en = make_entry(P, p, z0);
+        en.T = T;
+        ts.push_back(en);
+      }
+    }
+    if (g->fused) {
+      while (ts.size() % kNumXcd) {
+        BlockEnt pad;
+        memset(&pad, 0, sizeof pad);
+        pad.point = -1;
+        ts.push_back(pad);
+      }
+      if ((rc = upload_vec(g->tab_slab, ts))) return rc;
+      g->nb_slab = (uint32_t)ts.size();
+    }
   }
   HIPC(s->cur_steps.alloc(s->groups.size()));
   HIPC(hipMemset(s->cur_steps.p, 0, s->groups.size() * 8));
@@ -429,9 +484,22 @@ void enqueue_step(cvf_sim* s, int gi, hipStream_t st) {
   const Launch ll{g.tab_cell.p, g.nb_cell, (uint32_t)s->tune.block_cell, stepp};
   launch_step_begin(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, s->cur_steps.p + gi,
                     s->dist.p, s->freq.p, s->vol_device, s->precision);
-  launch_eta(st, lc, s->pts.p, s->words.p, 0);
-  launch_eta(st, lc, s->pts.p, s->words.p, 1);
-  for (int sub = 0; sub < 12 * s->nm; ++sub) launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub);
+  if (g.fused && s->nm > 0) {
+    // 12*n_metropolis half-sweeps in chunks of kChunk, ping-pong between the
+    // two word buffers (an even number of chunks ends in `words`)
+    const Launch lz{g.tab_slab.p, g.nb_slab, (uint32_t)s->tune.block_chunk, stepp};
+    const int nchunks = 12 * s->nm / kChunk;
+    for (int c = 0; c < nchunks; ++c) {
+      const uint32_t* src = (c & 1) ? s->words2.p : s->words.p;
+      uint32_t* dst = (c & 1) ? s->words.p : s->words2.p;
+      launch_chunk(st, lz, s->pts.p, src, dst, (uint32_t)(c * kChunk), kChunk, c == 0, g.chunk_lds);
+    }
+  } else {
+    launch_eta(st, lc, s->pts.p, s->words.p, 0);
+    launch_eta(st, lc, s->pts.p, s->words.p, 1);
+    for (int sub = 0; sub < 12 * s->nm; ++sub)
+      launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub);
+  }
   for (int jc = 0; jc < s->nc; ++jc) {
     launch_sw_local(st, ll, s->words.p, s->parent.p, s->cov.p, (uint32_t)jc);
     launch_sw_union(st, ll, s->parent.p, s->cov.p);
@@ -500,6 +568,9 @@ int check_tuning(const cvf_tuning& t) {
   if (!valid_block(t.block_sigma) || !valid_block(t.block_eta) || !valid_block(t.block_cell))
     return fail(CVF_EINVAL, "block sizes must be 64, 128, 256, 512 or 1024");
   if (t.graph_steps < 1 || t.graph_steps > 64) return fail(CVF_EINVAL, "graph_steps must be in [1, 64]");
+  if (!valid_block(t.block_chunk)) return fail(CVF_EINVAL, "block_chunk must be a power of 2 in [64, 1024]");
+  if (t.slab_planes < 0 || t.slab_planes > 64) return fail(CVF_EINVAL, "slab_planes must be in [0, 64]");
+  if (t.fused != 0 && t.fused != 1) return fail(CVF_EINVAL, "fused must be 0 or 1");
   return CVF_OK;
 }
 
@@ -510,6 +581,9 @@ cvf_tuning default_tuning(const cvf_sim* s) {
   t.block_eta = t.block_sigma;
   t.block_cell = 256;
   t.graph_steps = 16;
+  t.fused = 1;
+  t.slab_planes = 0;
+  t.block_chunk = 512;
   return t;
 }
 
@@ -580,6 +654,7 @@ int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
   if (loff >= 0xFFFFFFFFull) return fail(CVF_ENOMEM, "batch too large for 32-bit node offsets");
   // device buffers
   HIPC(s->words.alloc(s->total_cells));
+  HIPC(s->words2.alloc(s->total_cells));
   HIPC(s->parent.alloc(s->total_nodes));
   HIPC(hipMemset(s->parent.p, 0, s->total_nodes * 4));  // labels read as 0 before the first SW step
   HIPC(s->cov.alloc(s->total_cells));
@@ -950,10 +1025,23 @@ int cvf_profile_steps(cvf_sim* s, int64_t n_steps, double* kernel_ms, int64_t* l
         launch_step_begin(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p,
                           s->cur_steps.p + gi, s->dist.p, s->freq.p, s->vol_device, s->precision);
       });
-      for (int f = 0; f < 2; ++f)
-        timed(CVF_KCLASS_ETA, [&] { launch_eta(st, lc, s->pts.p, s->words.p, f); });
-      for (int sub = 0; sub < 12 * s->nm; ++sub)
-        timed(CVF_KCLASS_SIGMA, [&] { launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub); });
+      if (g.fused && s->nm > 0) {
+        const Launch lz{g.tab_slab.p, g.nb_slab, (uint32_t)s->tune.block_chunk, stepp};
+        const int nchunks = 12 * s->nm / kChunk;
+        for (int c = 0; c < nchunks; ++c) {
+          const uint32_t* src = (c & 1) ? s->words2.p : s->words.p;
+          uint32_t* dst = (c & 1) ? s->words.p : s->words2.p;
+          timed(CVF_KCLASS_CHUNK, [&] {
+            launch_chunk(st, lz, s->pts.p, src, dst, (uint32_t)(c * kChunk), kChunk, c == 0,
+                         g.chunk_lds);
+          });
+        }
+      } else {
+        for (int f = 0; f < 2; ++f)
+          timed(CVF_KCLASS_ETA, [&] { launch_eta(st, lc, s->pts.p, s->words.p, f); });
+        for (int sub = 0; sub < 12 * s->nm; ++sub)
+          timed(CVF_KCLASS_SIGMA, [&] { launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub); });
+      }
       for (int jc = 0; jc < s->nc; ++jc) {
         timed(CVF_KCLASS_SW_LOCAL,
               [&] { launch_sw_local(st, ll, s->words.p, s->parent.p, s->cov.p, (uint32_t)jc); });

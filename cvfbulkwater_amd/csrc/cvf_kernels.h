@@ -33,16 +33,21 @@ struct PointState {
 
 // One workgroup's work: the state point, the first work item, and every
 // per-point scalar the kernels need, so that one scalar load of the entry
-// starts a workgroup (64 bytes).
+// starts a workgroup (96 bytes).
 struct BlockEnt {
   int32_t point;  // -1: padding entry
-  uint32_t start;
+  uint32_t start;  // first work item (or first plane of a slab)
   uint32_t L, N, woff, loff;
   uint32_t seed_lo, seed_hi;
   uint32_t mL, mH, mQ;  // fast-division multipliers for L, L/2, L/4
   int32_t jeff_sign;
   uint32_t thb_lo, thb_hi, ts_lo, ts_hi;  // SW link thresholds
+  uint32_t mLL, mP8, mP4, mC;  // multipliers for L^2, L^2/8, L^2/4, L^2/16
+  uint32_t T;                  // slab thickness (planes) of the fused kernel
+  uint32_t pad[3];
 };
+
+constexpr int kChunk = 6;  // half-sweeps per fused kernel
 
 struct Launch {
   const BlockEnt* tab;
@@ -61,6 +66,12 @@ void launch_step_begin(hipStream_t st, const PointDev* pts, const int32_t* plist
                        int vol_device, int precision);
 void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, int family);
 void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, uint32_t sub);
+// Fused z-slab kernel: optional eta sweep + half-sweeps k0 .. k0+nk-1 of the
+// step on LDS-staged slabs, reading `src` and writing `dst`.
+void launch_chunk(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* src, uint32_t* dst,
+                  uint32_t k0, uint32_t nk, int with_eta, uint32_t lds_bytes);
+// largest dynamic LDS a fused launch may use
+uint32_t chunk_lds_limit();
 void launch_zero_u32(hipStream_t st, uint32_t* p, size_t n);
 void launch_observe(hipStream_t st, Launch ln, const uint32_t* words, unsigned long long* counters);
 void launch_sw_local(hipStream_t st, Launch ln, const uint32_t* words, uint32_t* parent, uint8_t* cov,

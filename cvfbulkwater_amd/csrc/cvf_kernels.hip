@@ -214,7 +214,7 @@ __device__ __forceinline__ void eta_cube(uint32_t* __restrict__ W, const uint64_
 }
 
 // One thread per quad of cubes of the family with origins
-// (ox, oy + a L/2, oz + b L/2), a,b in {0,1}: one Philox block, word a + 2b.
+// (ox + b L/2, oy + a L/2, oz), a,b in {0,1}: one Philox block, word a + 2b.
 __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
                       const PointDev* __restrict__ pts, uint32_t* __restrict__ words, int family) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
@@ -224,20 +224,21 @@ __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint6
   __syncthreads();
   const uint32_t L = e.L, hL = L / 2, qL = L / 4;
   const uint32_t t = e.start + threadIdx.x;
-  if (t >= hL * qL * qL) return;
+  if (t >= qL * qL * hL) return;
   const uint64_t step = *stepp;
-  const uint32_t r1 = fdiv(t, FastDiv{hL, e.mH});  // t = i + hL (j + qL k), j,k < L/4
-  const uint32_t i = t - r1 * hL;
-  const uint32_t k = fdiv(r1, FastDiv{qL, e.mQ});
+  const FastDiv dQ{qL, e.mQ};
+  const uint32_t r1 = fdiv(t, dQ);  // t = i + qL (j + qL k), i,j < L/4, k < L/2
+  const uint32_t i = t - r1 * qL;
+  const uint32_t k = fdiv(r1, dQ);
   const uint32_t j = r1 - k * qL;
   const uint32_t ox = 2 * i + family, oy = 2 * j + family, oz = 2 * k + family;
-  const uint32_t cube_ref = i + hL * j + hL * hL * oz;  // gpu_water3D.cu:3405
+  const uint32_t cube_ref = ox / 2 + hL * (oy / 2) + hL * hL * oz;  // gpu_water3D.cu:3405
   const U4 r = draw(seed_of(e), kStEta, cube_ref, step, 0);
   uint32_t* W = words + e.woff;
   eta_cube(W, thr, L, ox, oy, oz, r.x);
   eta_cube(W, thr, L, ox, oy + hL, oz, r.y);
-  eta_cube(W, thr, L, ox, oy, oz + hL, r.z);
-  eta_cube(W, thr, L, ox, oy + hL, oz + hL, r.w);
+  eta_cube(W, thr, L, ox + hL, oy, oz, r.z);
+  eta_cube(W, thr, L, ox + hL, oy + hL, oz, r.w);
 }
 
 // --------------------------------------------------------- sigma half-sweep
@@ -260,8 +261,8 @@ __device__ __forceinline__ void sigma_update(uint32_t* __restrict__ W, const uin
   if ((uint64_t)ra < thr[(h + 1) * 11 + d + 5]) W[c] = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
 }
 
-// One thread per quad of active cells (x, y + a L/2, z + b L/2), a,b in
-// {0,1}, y,z < L/2: the four have the same parity along every axis (L/2 is
+// One thread per quad of active cells (x + b L/2, y + a L/2, z), a,b in
+// {0,1}, x,y < L/2: the four have the same parity along every axis (L/2 is
 // even) and share one Philox block keyed on the a=b=0 cell (word a + 2b).
 // The half-sweep's (arm, parity) is drawn in the kernel from uniform values
 // (gpu_water3D.cu:1393-1394 draws them on the host).
@@ -272,7 +273,7 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   __shared__ uint64_t thr[33];
   if (threadIdx.x < 33) thr[threadIdx.x] = pts[e.point].thr_sigma[threadIdx.x];
   __syncthreads();
-  const uint32_t L = e.L, hL = L / 2, N = e.N;
+  const uint32_t L = e.L, hL = L / 2, qL = L / 4, N = e.N;
   const uint32_t t = e.start + threadIdx.x;
   if (t >= N / 8) return;
   const uint64_t step = *stepp;
@@ -281,44 +282,42 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   const int arm = (int)uniform_int(sc.x, 6);
   const uint32_t q = (sc.y >> 31) ? 0u : 1u;  // kept parity along the axis
   const int ax = arm >> 1;
-  const FastDiv dL{L, e.mL}, dH{hL, e.mH};
-  uint32_t x, y, z;  // quad origin, y,z < L/2
-  if (ax == 0) {  // t = hx + hL (y + hL z)
-    const uint32_t r1 = fdiv(t, dH);
-    x = 2 * (t - r1 * hL) + q;
+  const FastDiv dH{hL, e.mH}, dQ{qL, e.mQ};
+  uint32_t x, y, z;  // quad origin, x,y < L/2
+  if (ax == 0) {  // t = hx + qL (y + hL z)
+    const uint32_t r1 = fdiv(t, dQ);
+    x = 2 * (t - r1 * qL) + q;
     z = fdiv(r1, dH);
     y = r1 - z * hL;
-  } else if (ax == 1) {  // t = x + L (z + hL hy)
-    const uint32_t r1 = fdiv(t, dL);
-    x = t - r1 * L;
-    const uint32_t hy = fdiv(r1, dH);
-    z = r1 - hy * hL;
-    y = 2 * hy + q;
-  } else {  // t = x + L (y + hL hz)
-    const uint32_t r1 = fdiv(t, dL);
-    x = t - r1 * L;
+  } else if (ax == 1) {  // t = x + hL (hy + qL z)
+    const uint32_t r1 = fdiv(t, dH);
+    x = t - r1 * hL;
+    z = fdiv(r1, dQ);
+    y = 2 * (r1 - z * qL) + q;
+  } else {  // t = x + hL (y + hL hz)
+    const uint32_t r1 = fdiv(t, dH);
+    x = t - r1 * hL;
     const uint32_t hz = fdiv(r1, dH);
     y = r1 - hz * hL;
     z = 2 * hz + q;
   }
-  const uint32_t dy = hL * L, dz = N / 2;
+  const uint32_t dy = hL * L, dx = hL;
   const uint32_t c0 = x + L * (y + L * z);
-  const uint32_t c[4] = {c0, c0 + dy, c0 + dz, c0 + dy + dz};
+  const uint32_t c[4] = {c0, c0 + dy, c0 + dx, c0 + dx + dy};
   uint32_t nbc[4];
   if (ax == 0) {
-#pragma unroll
-    for (int m = 0; m < 4; ++m) nbc[m] = arm_neighbor(c[m], x, 1, L, arm);
+    nbc[0] = arm_neighbor(c[0], x, 1, L, arm);
+    nbc[2] = arm_neighbor(c[2], x + hL, 1, L, arm);
+    nbc[1] = nbc[0] + dy;
+    nbc[3] = nbc[2] + dy;
   } else if (ax == 1) {
     nbc[0] = arm_neighbor(c[0], y, L, L, arm);
     nbc[1] = arm_neighbor(c[1], y + hL, L, L, arm);
-    nbc[2] = nbc[0] + dz;
-    nbc[3] = nbc[1] + dz;
+    nbc[2] = nbc[0] + dx;
+    nbc[3] = nbc[1] + dx;
   } else {
-    const uint32_t LL = L * L;
-    nbc[0] = arm_neighbor(c[0], z, LL, L, arm);
-    nbc[2] = arm_neighbor(c[2], z + hL, LL, L, arm);
-    nbc[1] = nbc[0] + dy;
-    nbc[3] = nbc[2] + dy;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) nbc[m] = arm_neighbor(c[m], z, L * L, L, arm);
   }
   uint32_t* W = words + e.woff;
   const U4 r = draw(seed, kStSigma, c0, step, sub);
@@ -326,6 +325,228 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   sigma_update(W, thr, c[1], nbc[1], arm, r.y);
   sigma_update(W, thr, c[2], nbc[2], arm, r.z);
   sigma_update(W, thr, c[3], nbc[3], arm, r.w);
+}
+
+// Cube update on LDS planes: Wp is the cube's origin plane (L x L words,
+// periodic in x and y), the top face is the next plane.
+__device__ __forceinline__ void eta_cube_plane(uint32_t* Wp, const uint64_t* thr, uint32_t L, uint32_t LL,
+                                               uint32_t ox, uint32_t oy, uint32_t rw) {
+  const uint32_t x1 = ox + 1 == L ? 0 : ox + 1;
+  const uint32_t y1 = oy + 1 == L ? 0 : oy + 1;
+  uint32_t idx[8], w[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint32_t x = (b & 1) ? x1 : ox, y = (b & 2) ? y1 : oy;
+    idx[b] = ((b & 4) ? LL : 0u) + x + L * y;
+  }
+#pragma unroll
+  for (int b = 0; b < 8; ++b) w[b] = Wp[idx[b]];
+  uint32_t D = 0, E = 0;
+#pragma unroll
+  for (int jj = 0; jj < kEdges; ++jj) {
+    const int A = edge_start(jj), ax = edge_axis(jj), B = A | (1 << ax);
+    D |= (uint32_t)(sigma_of(w[A], 2 * ax) == sigma_of(w[B], 2 * ax + 1)) << jj;
+    E |= eta_of(w[A], 2 * ax) << jj;
+  }
+  uint32_t prop, acc;
+  split_word(rw, 9, prop, acc);
+  const uint32_t M = sol_mask(prop);
+  const int dn = popc32(M & D) - popc32(E & D);
+  if ((uint64_t)acc < thr[dn + 4] && M != E) {
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
+      const uint32_t ax_arm = bx, ay_arm = 2 + by, az_arm = 4 + bz;
+      const uint32_t own = (1u << ax_arm) | (1u << ay_arm) | (1u << az_arm);
+      const uint32_t bits = (((M >> corner_edge(b, 0)) & 1u) << ax_arm) |
+                            (((M >> corner_edge(b, 1)) & 1u) << ay_arm) |
+                            (((M >> corner_edge(b, 2)) & 1u) << az_arm);
+      Wp[idx[b]] = (w[b] & ~(own << kEtaShift)) | (bits << kEtaShift);
+    }
+  }
+}
+
+// LDS form of sigma_update: cell word at `wc`, facing neighbour word `wn`.
+__device__ __forceinline__ void sigma_update_lds(uint32_t* wc, uint32_t wn, const uint64_t* thr, int arm,
+                                                 uint32_t rw) {
+  const uint32_t w = *wc;
+  const uint32_t sa = sigma_of(w, arm);
+  uint32_t sp, ra;
+  split_word(rw, 6, sp, ra);
+  if (sp == sa) return;
+  const uint32_t snb = sigma_of(wn, arm ^ 1);
+  const int h = eta_of(w, arm) ? ((int)(sp == snb) - (int)(sa == snb)) : 0;
+  const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
+  if ((uint64_t)ra < thr[(h + 1) * 11 + d + 5]) *wc = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
+}
+
+// in-plane (x or y arm) update of the quad with origin (x, y) on plane Wp
+__device__ __forceinline__ void sigma_quad_plane(uint32_t* Wp, const uint64_t* thr, uint32_t L, uint32_t x,
+                                                 uint32_t y, int arm, int ax, U4 r) {
+  const uint32_t hL = L / 2;
+  const uint32_t xs[4] = {x, x, x + hL, x + hL}, ys[4] = {y, y + hL, y, y + hL};
+  const uint32_t rw[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    uint32_t nx = xs[m], ny = ys[m];
+    if (ax == 0)
+      nx = (arm & 1) ? (nx == 0 ? L - 1 : nx - 1) : (nx == L - 1 ? 0 : nx + 1);
+    else
+      ny = (arm & 1) ? (ny == 0 ? L - 1 : ny - 1) : (ny == L - 1 ? 0 : ny + 1);
+    sigma_update_lds(Wp + xs[m] + L * ys[m], Wp[nx + L * ny], thr, arm, rw[m]);
+  }
+}
+
+// z-arm update of the quad with origin (x, y) on plane Wp: the neighbour is
+// on the plane above (arm 4) or below (arm 5)
+__device__ __forceinline__ void sigma_quad_z(uint32_t* Wp, int LL, const uint64_t* thr, uint32_t L, uint32_t x,
+                                             uint32_t y, int arm, U4 r) {
+  const uint32_t hL = L / 2;
+  const int dz = (arm & 1) ? -LL : LL;
+  const uint32_t xs[4] = {x, x, x + hL, x + hL}, ys[4] = {y, y + hL, y, y + hL};
+  const uint32_t rw[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    uint32_t* wc = Wp + xs[m] + L * ys[m];
+    sigma_update_lds(wc, wc[dz], thr, arm, rw[m]);
+  }
+}
+
+// ------------------------------------------------------ fused slab kernel
+// One workgroup per z-slab of T full x-y planes of one state point.  The
+// slab plus a z-halo is staged in LDS, the optional eta sweep and kChunk
+// consecutive sigma half-sweeps run there, and the slab's own planes are
+// written to the other buffer of a ping-pong pair.  The halo is sized from
+// the chunk's own schedule: walking the chunk backwards, every +z (-z)
+// half-sweep widens the region that must be valid beforehand by one plane
+// upward (downward), x/y half-sweeps stay in-plane (full periodic planes),
+// and the eta sweep needs one more plane each side for its cubes.  Halo
+// planes are recomputed redundantly; each workgroup computes exactly the
+// updates the unfused kernels would make on those cells, with the same
+// site-keyed random numbers, so the result is identical.
+__device__ __forceinline__ uint32_t wrap_plane(int z, int L) {
+  while (z < 0) z += L;
+  while (z >= L) z -= L;
+  return (uint32_t)z;
+}
+
+__global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
+                        const PointDev* __restrict__ pts, const uint32_t* __restrict__ src,
+                        uint32_t* __restrict__ dst, uint32_t k0, uint32_t nk, int with_eta) {
+  extern __shared__ uint32_t win[];
+  __shared__ uint64_t thr_s[33];
+  __shared__ uint64_t thr_e[9];
+  __shared__ uint32_t sch[kChunk];
+  const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
+  if (e.point < 0) return;
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  const PointDev& P = pts[e.point];
+  if (tid < 33) thr_s[tid] = P.thr_sigma[tid];
+  if (tid < 9) thr_e[tid] = P.thr_eta[tid];
+  const uint64_t step = *stepp;
+  const uint64_t seed = seed_of(e);
+  if (tid < nk) {
+    const U4 r = draw(seed, kStSched, 0, step, k0 + tid);
+    sch[tid] = uniform_int(r.x, 6) | ((r.y >> 31) << 3);
+  }
+  __syncthreads();
+  const int L = (int)e.L, hL = L / 2, qL = L / 4, LL = L * L, T = (int)e.T, z0 = (int)e.start;
+  const FastDiv dH{(uint32_t)hL, e.mH}, dQ{(uint32_t)qL, e.mQ}, dLL{(uint32_t)LL, e.mLL};
+  const FastDiv dP8{(uint32_t)(LL / 8), e.mP8}, dP4{(uint32_t)(LL / 4), e.mP4},
+      dC{(uint32_t)(qL * qL), e.mC};
+  // planes relative to z0: window [wlo, whi)
+  int h_lo = 0, h_hi = 0;
+  for (uint32_t k = 0; k < nk; ++k) {
+    const uint32_t arm = sch[k] & 7u;
+    h_hi += arm == 4;
+    h_lo += arm == 5;
+  }
+  const int ext = with_eta ? 1 : 0;
+  const int wlo = -h_lo - ext, whi = T + h_hi + ext, np = whi - wlo;
+  // stage the window
+  const uint32_t* S = src + e.woff;
+  for (uint32_t i = tid; i < (uint32_t)(np * LL); i += nt) {
+    const uint32_t wz = fdiv(i, dLL);
+    const uint32_t rem = i - wz * LL;
+    win[i] = S[wrap_plane(z0 + wlo + (int)wz, L) * LL + rem];
+  }
+  __syncthreads();
+  if (with_eta) {
+    // cubes with both planes inside the window, family by family
+    for (int fam = 0; fam < 2; ++fam) {
+      // first window plane whose global z has the family's parity
+      const int first = ((z0 + wlo) & 1) == fam ? 0 : 1;
+      const int nplanes = first < np - 1 ? (np - 1 - first + 1) / 2 : 0;
+      const uint32_t work = (uint32_t)nplanes * (uint32_t)(qL * qL);
+      for (uint32_t it = tid; it < work; it += nt) {
+        const uint32_t pi = fdiv(it, dC);
+        const uint32_t u = it - pi * (uint32_t)(qL * qL);
+        const uint32_t jj = fdiv(u, dQ), ii = u - jj * qL;
+        const int wz = first + 2 * (int)pi;
+        const uint32_t ox = 2 * ii + fam, oy = 2 * jj + fam;
+        const uint32_t gz = wrap_plane(z0 + wlo + wz, L);
+        const uint32_t cube_ref = ox / 2 + hL * (oy / 2) + hL * hL * gz;  // gpu_water3D.cu:3405
+        const U4 r = draw(seed, kStEta, cube_ref, step, 0);
+        uint32_t* Wp = win + wz * LL;  // cube origin plane; the cube spans wz, wz+1
+        eta_cube_plane(Wp, thr_e, (uint32_t)L, (uint32_t)LL, ox, oy, r.x);
+        eta_cube_plane(Wp, thr_e, (uint32_t)L, (uint32_t)LL, ox, oy + hL, r.y);
+        eta_cube_plane(Wp, thr_e, (uint32_t)L, (uint32_t)LL, ox + hL, oy, r.z);
+        eta_cube_plane(Wp, thr_e, (uint32_t)L, (uint32_t)LL, ox + hL, oy + hL, r.w);
+      }
+      __syncthreads();
+    }
+  }
+  int rem_lo = h_lo, rem_hi = h_hi;
+  for (uint32_t k = 0; k < nk; ++k) {
+    const int arm = (int)(sch[k] & 7u);
+    const uint32_t q = (sch[k] >> 3) ? 0u : 1u;
+    const int ax = arm >> 1;
+    rem_hi -= arm == 4;
+    rem_lo -= arm == 5;
+    const int rlo = -rem_lo, rhi = T + rem_hi;  // planes updated by this half-sweep
+    const uint32_t sub = k0 + k;
+    if (ax < 2) {
+      const uint32_t work = (uint32_t)(rhi - rlo) * (uint32_t)(LL / 8);
+      for (uint32_t it = tid; it < work; it += nt) {
+        const uint32_t pi = fdiv(it, dP8);
+        const uint32_t u = it - pi * (uint32_t)(LL / 8);
+        uint32_t x, y;
+        if (ax == 0) {  // u = hx + qL y
+          y = fdiv(u, dQ);
+          x = 2 * (u - y * qL) + q;
+        } else {  // u = x + hL hy
+          const uint32_t hy = fdiv(u, dH);
+          x = u - hy * hL;
+          y = 2 * hy + q;
+        }
+        const int zr = rlo + (int)pi;
+        const uint32_t wz = (uint32_t)(zr - wlo);
+        const uint32_t gz = (uint32_t)(z0 + zr + 2 * L) % (uint32_t)L;
+        sigma_quad_plane(win + wz * LL, thr_s, (uint32_t)L, x, y, arm, ax,
+                         draw(seed, kStSigma, x + L * y + LL * gz, step, sub));
+      }
+    } else {
+      // planes whose global z has the kept parity
+      const int first = rlo + ((((z0 + rlo) & 1) == (int)q) ? 0 : 1);
+      const int nplanes = first < rhi ? (rhi - first + 1) / 2 : 0;
+      const uint32_t work = (uint32_t)nplanes * (uint32_t)(LL / 4);
+      for (uint32_t it = tid; it < work; it += nt) {
+        const uint32_t pi = fdiv(it, dP4);
+        const uint32_t u = it - pi * (uint32_t)(LL / 4);
+        const uint32_t y = fdiv(u, dH), x = u - y * hL;
+        const int zr = first + 2 * (int)pi;
+        const uint32_t wz = (uint32_t)(zr - wlo);
+        const uint32_t gz = (uint32_t)(z0 + zr + 2 * L) % (uint32_t)L;
+        sigma_quad_z(win + wz * LL, (int)LL, thr_s, (uint32_t)L, x, y, arm,
+                     draw(seed, kStSigma, x + L * y + LL * gz, step, sub));
+      }
+    }
+    __syncthreads();
+  }
+  // the slab's own planes go to the other buffer
+  uint32_t* D = dst + e.woff + (uint32_t)z0 * LL;
+  const uint32_t* Wi = win + (uint32_t)(-wlo) * LL;
+  for (uint32_t i = tid; i < (uint32_t)(T * LL); i += nt) D[i] = Wi[i];
 }
 
 // ----------------------------------------------------------- observables
@@ -628,6 +849,25 @@ void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words,
 }
 void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, uint32_t sub) {
   launch(k_sigma, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, pts, words, sub);
+}
+uint32_t chunk_lds_limit() {
+  // gfx950: 160 KB of LDS per CU; keep room for the static arrays
+  return 160 * 1024 - 1024;
+}
+void launch_chunk(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* src, uint32_t* dst,
+                  uint32_t k0, uint32_t nk, int with_eta, uint32_t lds_bytes) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chunk),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)chunk_lds_limit());
+    attr = true;
+  }
+  if (g_t0)
+    hipExtLaunchKernelGGL(k_chunk, dim3(ln.nblocks), dim3(ln.block), lds_bytes, st, g_t0, g_t1, 0,
+                          ln.tab, ln.nblocks, ln.step, pts, src, dst, k0, nk, with_eta);
+  else
+    hipLaunchKernelGGL(k_chunk, dim3(ln.nblocks), dim3(ln.block), lds_bytes, st, ln.tab, ln.nblocks,
+                       ln.step, pts, src, dst, k0, nk, with_eta);
 }
 void launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
   size_t blocks = (n + 255) / 256;

@@ -140,6 +140,10 @@ typedef struct cvf_tuning {
   int32_t block_eta;    /* workgroup size of the eta cube sweep */
   int32_t block_cell;   /* workgroup size of the per-cell kernels (SW, observables) */
   int32_t graph_steps;  /* MC steps per replayed hipGraph (1..64) */
+  int32_t fused;        /* 1: eta sweep + 6 half-sweeps per kernel on LDS-staged z-slabs
+                           where the slab fits in LDS; 0: one kernel per half-sweep */
+  int32_t slab_planes;  /* planes per slab of the fused kernel; 0 = largest that fits */
+  int32_t block_chunk;  /* workgroup size of the fused kernel */
 } cvf_tuning;
 
 int cvf_get_tuning(cvf_sim *s, cvf_tuning *t);
@@ -154,7 +158,8 @@ enum {
   CVF_KCLASS_SW_LOCAL = 3,
   CVF_KCLASS_SW_UNION = 4,
   CVF_KCLASS_SW_FLIP = 5,
-  CVF_KCLASS_COUNT = 6
+  CVF_KCLASS_CHUNK = 6,   /* fused slab kernel (eta sweep + 6 sigma half-sweeps) */
+  CVF_KCLASS_COUNT = 7
 };
 
 /* Instrumented cvf_step for measurement (no reference counterpart; the

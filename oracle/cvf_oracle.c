@@ -397,16 +397,16 @@ void ora_eta_sweep(ora_sim *s, uint64_t step) {
   int N = s->N, NC = s->NC;
   for (int cube = 0; cube < NC; ++cube) {
     int old = s->state[cube];
-    /* the four cubes with origins (ox, oy + a L/2, oz + b L/2), a,b in {0,1},
-     * share one Philox block keyed on the a=b=0 cube; cube a,b uses word
-     * a + 2b (cube index = x/2 + (L/2)(y/2) + (L^2/4) z, 3405) */
+    /* the four cubes with origins (ox + b L/2, oy + a L/2, oz), a,b in
+     * {0,1}, share one Philox block keyed on the a=b=0 cube; cube a,b uses
+     * word a + 2b (cube index = x/2 + (L/2)(y/2) + (L^2/4) z, 3405) */
     int L = s->L, hL = L / 2;
-    int oy2 = (cube / hL) % hL, oz = cube / (hL * hL);
-    int ay = oy2 >= L / 4, bz = oz >= hL;
-    int site = cube - ay * hL * (L / 4) - bz * hL * hL * hL;
+    int ox2 = cube % hL, oy2 = (cube / hL) % hL;
+    int ay = oy2 >= L / 4, bx = ox2 >= L / 4;
+    int site = cube - ay * hL * (L / 4) - bx * (L / 4);
     uint32_t r[4], prop, acc;
     draw(s->p.seed, ORA_ST_ETA, (uint32_t)site, step, 0, r);
-    split_word(r[ay + 2 * bz], 9, &prop, &acc);
+    split_word(r[ay + 2 * bx], 9, &prop, &acc);
     int nw = (int)prop;
     int dn = 0;
     for (int j = 0; j < EDGES; ++j) {
@@ -443,14 +443,14 @@ void ora_sigma_halfsweep(ora_sim *s, uint64_t step, uint32_t sub, int arm, int i
     int nb = neighbor(L, c, arm);
     int nbspin = s->s[(arm ^ 1) * N + nb];
     int old = s->s[arm * N + c];
-    /* the four cells (x, y + a L/2, z + b L/2), a,b in {0,1}, are active
+    /* the four cells (x + b L/2, y + a L/2, z), a,b in {0,1}, are active
      * together (L/2 is even) and share one Philox block keyed on the a=b=0
      * cell; cell a,b uses word a + 2b */
-    int ay = y >= L / 2, bz = z >= L / 2;
-    int site = c - ay * (L / 2) * L - bz * (N / 2);
+    int ay = y >= L / 2, bx = x >= L / 2;
+    int site = c - ay * (L / 2) * L - bx * (L / 2);
     uint32_t r[4], prop, acc;
     draw(s->p.seed, ORA_ST_SIGMA, (uint32_t)site, step, sub, r);
-    split_word(r[ay + 2 * bz], 6, &prop, &acc);
+    split_word(r[ay + 2 * bx], 6, &prop, &acc);
     int nw = (int)prop;
     if (nw == old) continue;
     int h = s->eta[arm * N + c] * ((nw == nbspin) - (old == nbspin));

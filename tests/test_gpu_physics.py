@@ -168,9 +168,15 @@ def test_host_and_device_volume_modes_agree():
 
 
 @pytest.mark.parametrize("tuning", [
-    {"streams": 1, "block_sigma": 256, "block_eta": 256, "block_cell": 256, "graph_steps": 1},
-    {"streams": 3, "block_sigma": 64, "block_eta": 128, "block_cell": 64, "graph_steps": 5},
-    {"streams": 8, "block_sigma": 1024, "block_eta": 512, "block_cell": 1024, "graph_steps": 16},
+    {"streams": 1, "block_sigma": 256, "block_eta": 256, "block_cell": 256, "graph_steps": 1,
+     "fused": 0},
+    {"streams": 3, "block_sigma": 64, "block_eta": 128, "block_cell": 64, "graph_steps": 5,
+     "fused": 0},
+    {"streams": 8, "block_sigma": 1024, "block_eta": 512, "block_cell": 1024, "graph_steps": 16,
+     "fused": 0},
+    {"streams": 2, "fused": 1, "slab_planes": 4, "block_chunk": 256},
+    {"streams": 1, "fused": 1, "slab_planes": 2, "block_chunk": 1024, "graph_steps": 3},
+    {"streams": 4, "fused": 1, "slab_planes": 1, "block_chunk": 64},
 ])
 def test_tuning_does_not_change_results(tuning):
     """Launch configuration (streams, workgroup sizes, steps per graph) never
@@ -179,10 +185,10 @@ def test_tuning_does_not_change_results(tuning):
     ref = Simulation(pts)
     alt = Simulation(pts)
     alt.tuning = tuning
-    assert alt.tuning == tuning
+    assert {k: alt.tuning[k] for k in tuning} == tuning
     ref.step(37)
     alt.step(20)
-    alt.tuning = {"streams": 2}
+    alt.tuning = {"streams": 2, "fused": 1 - alt.tuning["fused"]}
     alt.step(17)
     for i in range(len(pts)):
         a, b = ref.get_config(i), alt.get_config(i)

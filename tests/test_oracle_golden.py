@@ -356,10 +356,10 @@ def test_sigma_acceptance_is_boltzmann():
             if (ind and coord % 2) or (not ind and not coord % 2):
                 assert s1[arm * N + c] == s0[arm * N + c]
                 continue
-            # (x, y + a L/2, z + b L/2) share one Philox block, word a + 2b;
+            # (x + b L/2, y + a L/2, z) share one Philox block, word a + 2b;
             # proposal floor(6w/2^32), acceptance (6w) mod 2^32
-            a_, b_ = int(y >= L // 2), int(z >= L // 2)
-            r = O.draw(seed, O.ST_SIGMA, c - a_ * (L // 2) * L - b_ * (N // 2), step, 0)
+            a_, b_ = int(y >= L // 2), int(x >= L // 2)
+            r = O.draw(seed, O.ST_SIGMA, c - a_ * (L // 2) * L - b_ * (L // 2), step, 0)
             prop = (int(r[a_ + 2 * b_]) * 6) >> 32
             if prop == s0[arm * N + c]:
                 continue
@@ -423,9 +423,9 @@ def test_eta_acceptance_is_boltzmann():
         eta_invariants(L, s1, e1)
         base = hamiltonian_counts(L, s0, e0)[0]
         for cube, edges in cubes:
-            hL = L // 2  # cubes (ox, oy + a L/2, oz + b L/2) share one Philox block
-            a_, b_ = int((cube // hL) % hL >= L // 4), int(cube // (hL * hL) >= hL)
-            site = cube - a_ * hL * (L // 4) - b_ * hL ** 3
+            hL = L // 2  # cubes (ox + b L/2, oy + a L/2, oz) share one Philox block
+            a_, b_ = int((cube // hL) % hL >= L // 4), int(cube % hL >= L // 4)
+            site = cube - a_ * hL * (L // 4) - b_ * (L // 4)
             prop = (int(O.draw(seed, O.ST_ETA, site, step, 0)[a_ + 2 * b_]) * 9) >> 32
             acc = st1[cube] == prop
             assert acc or st1[cube] == st0[cube]
