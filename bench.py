@@ -145,6 +145,8 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=2000)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--tune-steps", type=int, default=20,
+                    help="MC steps per autotuner trial (0: default launch configuration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["batch", "plain"], default=None,
                     help="run one leg only (for profiling)")
@@ -168,6 +170,7 @@ def main() -> int:
         pts = batch_params(fp32=True)
         cells = sum(p.L ** 3 for p in pts)
         sim = Simulation(pts)
+        tuning = sim.autotune(args.tune_steps) if args.tune_steps else sim.tuning
         ms = time_steps(sim, args.steps, args.warmup, ev)
         prof = sim.profile_steps(args.profile_steps)
         sim.close()
@@ -182,14 +185,15 @@ def main() -> int:
             per_step = prof["sigma"][1] / args.profile_steps
             k_bytes = (cells // 2) * SIGMA_BYTES_PER_ACTIVE_CELL * 12 / per_step
         achieved = k_bytes / (k_ms * 1e-3) / 1e9
-        out.update(batch_ms=ms, batch_cells=cells, batch_rate=cells / (ms * 1e-3), prof=prof,
+        out.update(batch_ms=ms, batch_cells=cells, batch_rate=cells / (ms * 1e-3), prof=prof, tuning=tuning,
                    kname=kname, sig_ms=k_ms, sig_bytes=k_bytes, achieved=achieved)
     if args.only in (None, "plain"):
         sim = Simulation(Params(L=32, T=0.815, P=0.45, seed=44648604, precision=FP64,
                                 volume_mode=VOLUME_HOST))
+        ptuning = sim.autotune(args.tune_steps) if args.tune_steps else sim.tuning
         pms = time_steps(sim, args.plain_steps, args.warmup, ev)
         sim.close()
-        out.update(plain_ms=pms, plain_rate=32 ** 3 / (pms * 1e-3))
+        out.update(plain_ms=pms, plain_rate=32 ** 3 / (pms * 1e-3), plain_tuning=ptuning)
 
     line = {"metric": "site-updates/s per GPU, batched mixed-prec vs plain; achieved HBM GB/s",
             "unit": "site-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -201,7 +205,8 @@ def main() -> int:
                                       "T in [0.55,1.00] x P in [-0.10,0.80], 1 Metropolis + 1 eta "
                                       "sweep + 1 volume move per step, FP32-compensated vdW, "
                                       "hipGraph step", "cells": out["batch_cells"],
-                          "parallelism": "one GPU, state points batched"}
+                          "parallelism": "one GPU, state points batched",
+                          "tuning": out["tuning"]}
         line["roofline"] = {"bound": "hbm", "kernel": out["kname"], "achieved": out["achieved"],
                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": out["achieved"] / HBM_PEAK_GBS, "traffic": None,
@@ -210,7 +215,8 @@ def main() -> int:
     if "plain_rate" in out:
         line["plain"] = {"workload": "plain drop-in form: L=32, T=0.815, P=0.45, FP64 vdW, host "
                                      "volume move", "value": out["plain_rate"],
-                         "ms_per_step": out["plain_ms"], "steps": args.plain_steps}
+                         "ms_per_step": out["plain_ms"], "steps": args.plain_steps,
+                         "tuning": out["plain_tuning"]}
         if "batch_rate" in out:
             line["batched_over_plain"] = out["batch_rate"] / out["plain_rate"]
         else:

@@ -24,7 +24,7 @@ import numpy as np
 __all__ = [
     "CVFError", "Params", "Obs", "ClusterStats", "Simulation", "load_library",
     "FP64", "FP32_COMPENSATED", "VOLUME_HOST", "VOLUME_DEVICE", "vdw_energy",
-    "LIB_PATH", "EXE_PATH",
+    "LIB_PATH", "EXE_PATH", "Correlator",
 ]
 
 PKG = Path(__file__).resolve().parent
@@ -155,6 +155,12 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cvf_vdw_energy": (C.c_int, [I32, I32, F32P, I32P, I32, C.c_float, DP, I32, I32, DP]),
         "cvf_profile_steps": (C.c_int, [P, I64, DP, C.POINTER(C.c_int64), I32]),
         "cvf_get_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
+        "cvf_autotune": (C.c_int, [P, I32, C.POINTER(_Tuning)]),
+        "cvf_sigma_histogram": (C.c_int, [P, I32, C.POINTER(C.c_int64)]),
+        "cvf_corr_create": (C.c_void_p, []),
+        "cvf_corr_add": (None, [C.c_void_p, C.c_double]),
+        "cvf_corr_result": (C.c_int, [C.c_void_p, DP, DP, I32]),
+        "cvf_corr_destroy": (None, [C.c_void_p]),
         "cvf_set_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
     }
     for name, (res, args) in sig.items():
@@ -293,6 +299,12 @@ class Simulation:
         t = _Tuning(*[int(cur[k]) for k in self.TUNING_FIELDS])
         _check(self._lib, self._lib.cvf_set_tuning(self._h, C.byref(t)), "cvf_set_tuning")
 
+    def autotune(self, steps_per_trial: int = 20) -> dict:
+        """Time candidate launch configurations and keep the fastest (state unchanged)."""
+        t = _Tuning()
+        _check(self._lib, self._lib.cvf_autotune(self._h, steps_per_trial, C.byref(t)), "cvf_autotune")
+        return {k: getattr(t, k) for k in self.TUNING_FIELDS}
+
     KCLASSES = ("begin", "eta", "sigma", "sw_local", "sw_union", "sw_flip", "chunk")
 
     def profile_steps(self, n: int) -> dict:
@@ -302,6 +314,11 @@ class Simulation:
         cnt = (C.c_int64 * k)()
         _check(self._lib, self._lib.cvf_profile_steps(self._h, n, ms, cnt, k), "cvf_profile_steps")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KCLASSES)}
+
+    def sigma_histogram(self, pt: int = 0) -> np.ndarray:
+        out = (C.c_int64 * 6)()
+        _check(self._lib, self._lib.cvf_sigma_histogram(self._h, pt, out), "cvf_sigma_histogram")
+        return np.array(list(out), np.int64)
 
     # -- measurement
     def observe(self, pt: int = 0) -> Obs:
@@ -328,3 +345,29 @@ def vdw_energy(L: int, dist: np.ndarray, freq: np.ndarray, r_cutoff: float, r: n
                                    rr.ctypes.data_as(C.POINTER(C.c_double)), rr.size, precision,
                                    out.ctypes.data_as(C.POINTER(C.c_double))), "cvf_vdw_energy")
     return out
+
+
+class Correlator:
+    """Multi-tau correlator of the Flag_correlation mode (host code in libcvf.so)."""
+
+    def __init__(self):
+        self._lib = load_library()
+        self._h = self._lib.cvf_corr_create()
+
+    def add(self, w: float) -> None:
+        self._lib.cvf_corr_add(self._h, float(w))
+
+    def result(self) -> tuple[np.ndarray, np.ndarray]:
+        t = np.empty(1024)
+        f = np.empty(1024)
+        n = self._lib.cvf_corr_result(self._h, t.ctypes.data_as(C.POINTER(C.c_double)),
+                                      f.ctypes.data_as(C.POINTER(C.c_double)), 1024)
+        if n < 0:
+            raise CVFError("cvf_corr_result failed")
+        return t[:n].copy(), f[:n].copy()
+
+    def __del__(self):
+        try:
+            self._lib.cvf_corr_destroy(self._h)
+        except Exception:
+            pass

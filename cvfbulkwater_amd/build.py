@@ -33,7 +33,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function",
 ]
 
-LIB_SOURCES = ["cvf_kernels.hip", "cvf_api.cpp", "cvf_host.cpp"]
+LIB_SOURCES = ["cvf_kernels.hip", "cvf_api.cpp", "cvf_host.cpp", "cvf_corr.cpp"]
 EXE_SOURCES = ["cvf_water3d.cpp"]
 
 

@@ -872,6 +872,23 @@ int cvf_observe(cvf_sim* s, int32_t pt, cvf_obs* o) {
   return CVF_OK;
 }
 
+int cvf_sigma_histogram(cvf_sim* s, int32_t pt, int64_t counts6[6]) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  if (!counts6) return fail(CVF_EINVAL, "null output");
+  HIPC(hipSetDevice(s->device));
+  const PointHost& P = s->ph[pt];
+  DevBuf<unsigned long long> h;
+  HIPC(h.alloc(6));
+  launch_zero_u32(s->stream, (uint32_t*)h.p, 12);
+  launch_sigma_hist(s->stream, s->words.p + P.woff, P.N, h.p);
+  unsigned long long out[6];
+  HIPC(hipMemcpyAsync(out, h.p, sizeof out, hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipStreamSynchronize(s->stream));
+  for (int i = 0; i < 6; ++i) counts6[i] = (int64_t)out[i];
+  return CVF_OK;
+}
+
 int cvf_get_labels(cvf_sim* s, int32_t pt, int32_t* labels6N) {
   int rc = check_pt(s, pt);
   if (rc) return rc;
@@ -1090,6 +1107,140 @@ int cvf_set_tuning(cvf_sim* s, const cvf_tuning* t) {
   HIPC(hipStreamSynchronize(s->stream));
   s->tune = *t;
   return build_groups(s);
+}
+
+namespace {
+
+std::string tune_key(cvf_sim* s) {
+  hipDeviceProp_t prop;
+  std::string arch = hipGetDeviceProperties(&prop, s->device) == hipSuccess ? prop.gcnArchName : "?";
+  std::vector<uint32_t> Ls;
+  for (auto& P : s->ph) Ls.push_back(P.L);
+  std::sort(Ls.begin(), Ls.end());
+  std::string k = arch.substr(0, arch.find(':')) + "|";
+  for (size_t i = 0; i < Ls.size();) {
+    size_t j = i;
+    while (j < Ls.size() && Ls[j] == Ls[i]) ++j;
+    k += std::to_string(Ls[i]) + "x" + std::to_string(j - i) + ",";
+    i = j;
+  }
+  k += "|nm" + std::to_string(s->nm) + "|nc" + std::to_string(s->nc) + "|p" +
+       std::to_string(s->precision) + "|v" + std::to_string(s->vol_device);
+  return k;
+}
+
+bool cache_lookup(const std::string& key, cvf_tuning* t) {
+  const char* path = getenv("CVF_TUNE_CACHE");
+  if (!path || !*path) return false;
+  FILE* f = fopen(path, "r");
+  if (!f) return false;
+  char line[1024];
+  bool found = false;
+  while (fgets(line, sizeof line, f)) {
+    char k[768];
+    cvf_tuning c;
+    if (sscanf(line, "%767s %d %d %d %d %d %d %d %d", k, &c.streams, &c.block_sigma, &c.block_eta,
+               &c.block_cell, &c.graph_steps, &c.fused, &c.slab_planes, &c.block_chunk) == 9 &&
+        key == k) {
+      *t = c;
+      found = true;
+    }
+  }
+  fclose(f);
+  return found;
+}
+
+void cache_store(const std::string& key, const cvf_tuning& c) {
+  const char* path = getenv("CVF_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE* f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s %d %d %d %d %d %d %d %d\n", key.c_str(), c.streams, c.block_sigma, c.block_eta,
+          c.block_cell, c.graph_steps, c.fused, c.slab_planes, c.block_chunk);
+  fclose(f);
+}
+
+}  // namespace
+
+extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* chosen) {
+  if (!s || steps_per_trial < 1) return fail(CVF_EINVAL, "bad arguments");
+  HIPC(hipSetDevice(s->device));
+  const std::string key = tune_key(s);
+  cvf_tuning best = s->tune;
+  int rc;
+  if (cache_lookup(key, &best) && check_tuning(best) == CVF_OK) {
+    if ((rc = cvf_set_tuning(s, &best))) return rc;
+    if (chosen) *chosen = best;
+    return CVF_OK;
+  }
+  // snapshot of everything a step changes
+  HIPC(hipStreamSynchronize(s->stream));
+  DevBuf<uint32_t> w0, l0;
+  DevBuf<PointState> p0;
+  HIPC(w0.alloc(s->total_cells));
+  HIPC(l0.alloc(s->total_nodes));
+  HIPC(p0.alloc(s->npts));
+  HIPC(hipMemcpy(w0.p, s->words.p, s->total_cells * 4, hipMemcpyDeviceToDevice));
+  HIPC(hipMemcpy(l0.p, s->parent.p, s->total_nodes * 4, hipMemcpyDeviceToDevice));
+  HIPC(hipMemcpy(p0.p, s->pst.p, s->npts * sizeof(PointState), hipMemcpyDeviceToDevice));
+  std::vector<VolState> hv;
+  for (auto& P : s->ph) hv.push_back(P.vol);
+  const int64_t steps0 = s->steps;
+  // candidates
+  std::vector<cvf_tuning> cand;
+  const cvf_tuning base = default_tuning(s);
+  for (int streams : {1, 2, 4}) {
+    if (streams > s->npts) continue;
+    for (int slab : {0, 4, 2})
+      for (int bc : {256, 512, 1024}) {
+        cvf_tuning t = base;
+        t.streams = streams;
+        t.fused = 1;
+        t.slab_planes = slab;
+        t.block_chunk = bc;
+        cand.push_back(t);
+      }
+    for (int bs : {64, 256}) {
+      cvf_tuning t = base;
+      t.streams = streams;
+      t.fused = 0;
+      t.block_sigma = t.block_eta = bs;
+      cand.push_back(t);
+    }
+  }
+  hipEvent_t a, b;
+  HIPC(hipEventCreate(&a));
+  HIPC(hipEventCreate(&b));
+  double best_ms = 1e30;
+  for (const auto& t : cand) {
+    if ((rc = cvf_set_tuning(s, &t))) break;
+    if ((rc = cvf_step(s, 2))) break;
+    HIPC(hipEventRecord(a, s->stream));
+    if ((rc = cvf_step(s, steps_per_trial))) break;
+    HIPC(hipEventRecord(b, s->stream));
+    HIPC(hipEventSynchronize(b));
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, a, b));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  // restore the snapshot and apply the winner
+  HIPC(hipStreamSynchronize(s->stream));
+  HIPC(hipMemcpy(s->words.p, w0.p, s->total_cells * 4, hipMemcpyDeviceToDevice));
+  HIPC(hipMemcpy(s->parent.p, l0.p, s->total_nodes * 4, hipMemcpyDeviceToDevice));
+  HIPC(hipMemcpy(s->pst.p, p0.p, s->npts * sizeof(PointState), hipMemcpyDeviceToDevice));
+  for (size_t i = 0; i < s->ph.size(); ++i) s->ph[i].vol = hv[i];
+  s->steps = steps0;
+  s->obs_at = -1;
+  if (rc) return rc;
+  if ((rc = cvf_set_tuning(s, &best))) return rc;
+  cache_store(key, best);
+  if (chosen) *chosen = best;
+  return CVF_OK;
 }
 
 int cvf_vdw_energy(int32_t device_id, int32_t L, const float* dist, const int32_t* freq,

@@ -78,6 +78,7 @@ void launch_sw_local(hipStream_t st, Launch ln, const uint32_t* words, uint32_t*
                      uint32_t jc);
 void launch_sw_union(hipStream_t st, Launch ln, uint32_t* parent, const uint8_t* cov);
 void launch_sw_flip(hipStream_t st, Launch ln, uint32_t* words, uint32_t* parent, uint32_t jc);
+void launch_sigma_hist(hipStream_t st, const uint32_t* W, uint32_t N, unsigned long long* counts);
 // cluster statistics of one point (labels at `label`, 6N nodes)
 void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, int32_t* size);
 void launch_perc_slots(hipStream_t st, const uint32_t* label, uint32_t nn, const int32_t* size,

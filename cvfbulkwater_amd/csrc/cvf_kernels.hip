@@ -596,6 +596,30 @@ __global__ void k_observe(const BlockEnt* __restrict__ tab, uint32_t nb,
   }
 }
 
+// Counts of the six sigma values over the 6N arms of one point.
+__global__ void k_sigma_hist(const uint32_t* __restrict__ W, uint32_t N,
+                             unsigned long long* __restrict__ counts) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t c[6] = {0, 0, 0, 0, 0, 0};
+  if (t < N) {
+    const uint32_t w = W[t];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      const uint32_t v = sigma_of(w, a);
+#pragma unroll
+      for (int s = 0; s < 6; ++s) c[s] += v == (uint32_t)s;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 6; ++s)
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) c[s] += __shfl_xor(c[s], m, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+      if (c[s]) atomicAdd(&counts[s], (unsigned long long)c[s]);
+}
+
 // --------------------------------------------------------- Swendsen-Wang
 // Bond activation of gpu_create_cluster (gpu_water3D.cu:2383-2438), then
 // union-find with minimum-root linking.  Every link only ever points a
@@ -889,6 +913,9 @@ void launch_sw_union(hipStream_t st, Launch ln, uint32_t* parent, const uint8_t*
 void launch_sw_flip(hipStream_t st, Launch ln, uint32_t* words, uint32_t* parent, uint32_t jc) {
   launch(k_sw_flip, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, words, parent,
          jc);
+}
+void launch_sigma_hist(hipStream_t st, const uint32_t* W, uint32_t N, unsigned long long* counts) {
+  launch(k_sigma_hist, grid1(N, 256), dim3(256), st, W, N, counts);
 }
 void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, int32_t* size) {
   launch(k_cluster_sizes, grid1(nn, 256), dim3(256), st, label, nn, size);

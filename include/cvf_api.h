@@ -150,6 +150,15 @@ int cvf_get_tuning(cvf_sim *s, cvf_tuning *t);
 /* Rebuilds the groups, block tables and graphs; synchronises the handle. */
 int cvf_set_tuning(cvf_sim *s, const cvf_tuning *t);
 
+/* Project-written autotuner: times candidate tunings (fused slab kernel
+ * with several slab thicknesses and workgroup sizes, unfused kernels, 1/2/4
+ * streams) for `steps_per_trial` MC steps each, from a snapshot of the
+ * state that is restored afterwards, and applies the fastest.  The result
+ * is cached in the file named by CVF_TUNE_CACHE (if set), keyed by the
+ * device architecture and the batch shape.  The simulation state and step
+ * count are unchanged on return. */
+int cvf_autotune(cvf_sim *s, int32_t steps_per_trial, cvf_tuning *chosen);
+
 /* Kernel classes of one MC step (profiling). */
 enum {
   CVF_KCLASS_BEGIN = 0,   /* step counter, schedule, device volume move */
@@ -170,6 +179,22 @@ enum {
  * the launch counts.  Needs the device volume mode. */
 int cvf_profile_steps(cvf_sim *s, int64_t n_steps, double *kernel_ms, int64_t *launches,
                       int32_t n_classes);
+
+/* Histogram of the 6N sigma values of point pt (the counts behind the
+ * correlator observable M = max_s n_s / 6N, gpu_water3D.cu:1254-1266). */
+int cvf_sigma_histogram(cvf_sim *s, int32_t pt, int64_t counts6[6]);
+
+/* Multi-tau correlator of the Flag_correlation = 1 mode (gpu_water3D.cu:
+ * 161-191, 2983-3128; Ramirez, Sinha, Bhatt, Harmandaris, Schmid, J. Chem.
+ * Phys. 133, 154103 (2010)): 32 levels of 16 points, blocking by 2.
+ * cvf_corr_result returns the lag times and the normalised correlation
+ * C(t)/C(0) written to correlation.dat (printCorr, 3114-3128); the return
+ * value is the number of points (<= cap) or < 0. */
+typedef struct cvf_correlator cvf_correlator;
+cvf_correlator *cvf_corr_create(void);
+void cvf_corr_add(cvf_correlator *c, double w);
+int cvf_corr_result(cvf_correlator *c, double *t, double *f, int32_t cap);
+void cvf_corr_destroy(cvf_correlator *c);
 
 /* E_vdW of a shell table at r (calculate_energy_vdW, 1784-1805), evaluated
  * on the device: FP64, or FP32 terms with a compensated wavefront sum. */

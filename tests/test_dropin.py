@@ -13,7 +13,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from cvfbulkwater_amd import EXE_PATH
+from cvfbulkwater_amd import EXE_PATH, Correlator
 from oracle import py as O
 
 GOLD = Path(__file__).resolve().parent / "golden"
@@ -30,7 +30,7 @@ Sampling_Interval           {interval}
 R_cutoff			  6.0
 Flag_config                 {flag_config}
 Flag_distances              {flag_dist}
-Flag_correlation            0
+Flag_correlation            {flag_corr}
 Flag_chessboard             1
 Device_ID                   0
 """
@@ -38,7 +38,7 @@ Device_ID                   0
 
 def write_input(d: Path, **kw):
     v = dict(P=0.45, T=0.815, seed=44648604, nc=0, neq=100, nloops=300, interval=10, flag_config=1,
-             flag_dist=0)
+             flag_dist=0, flag_corr=0)
     v.update(kw)
     (d / "input_data").write_text(INPUT_DATA.format(**v))
     if v["flag_config"] == 1:
@@ -159,3 +159,23 @@ def test_dropin_distances_mismatch(tmp_path):
     write_input(tmp_path, neq=1, nloops=1, flag_dist=1)
     r = run(tmp_path, check=False)
     assert r.returncode == 1 and "X rank mismatch" in r.stderr
+
+
+@pytest.mark.gpu
+def test_dropin_correlation_dat_matches_oracle(tmp_path):
+    """Flag_correlation=1 at L=16: after every production step the fraction
+    of sigma in the most populated state feeds the multi-tau correlator;
+    correlation.dat equals the same correlator fed from the oracle run."""
+    write_input(tmp_path, T=0.70, neq=10, nloops=200, flag_config=0, flag_corr=1)
+    run(tmp_path, "-L", "16")
+    got = (tmp_path / "correlation.dat").read_text()
+    ora = O.OracleSim(L=16, T=0.70, P=0.45, seed=44648604)
+    ora.step(10)
+    c = Correlator()
+    for _ in range(200):
+        ora.step(1)
+        s, _e = ora.get_config()
+        c.add(np.bincount(s, minlength=6).max() / s.size)
+    t, f = c.result()
+    assert got == "".join("%.0f\t%f\n" % (a, b) for a, b in zip(t, f))
+    assert len(t) > 16

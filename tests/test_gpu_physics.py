@@ -195,3 +195,33 @@ def test_tuning_does_not_change_results(tuning):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
         assert np.array_equal(ref.labels(i), alt.labels(i))
         assert ref.volume(i) == alt.volume(i)
+
+
+def test_autotune_keeps_state_and_trajectory():
+    """cvf_autotune times candidate tunings from a snapshot, restores it and
+    applies the winner: the run continues exactly as without tuning."""
+    pts = batch_points([8, 16, 24, 12], nc=1)
+    ref = Simulation(pts)
+    alt = Simulation(pts)
+    ref.step(15)
+    alt.step(15)
+    chosen = alt.autotune(4)
+    assert alt.steps == 15 and alt.tuning == chosen
+    for i in range(len(pts)):
+        assert np.array_equal(ref.get_config(i)[0], alt.get_config(i)[0])
+        assert ref.volume(i) == alt.volume(i)
+    ref.step(20)
+    alt.step(20)
+    for i in range(len(pts)):
+        a, b = ref.get_config(i), alt.get_config(i)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert np.array_equal(ref.labels(i), alt.labels(i))
+        assert ref.volume(i) == alt.volume(i)
+
+
+def test_sigma_histogram_matches_config():
+    sim = Simulation(batch_points([8, 16, 12]))
+    sim.step(25)
+    for i in range(3):
+        s, _e = sim.get_config(i)
+        assert np.array_equal(sim.sigma_histogram(i), np.bincount(s, minlength=6))
