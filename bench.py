@@ -137,6 +137,67 @@ def calibrate_copy(mib: float, reps: int = 10) -> int:
     return 0
 
 
+def precision_form(steps: int = 5000, interval: int = 10, ev: HipEvents | None = None) -> dict:
+    """Mixed-precision large lattice: L = 96, T = 0.65, P = 0.30, 1 Metropolis
+    + 1 Swendsen-Wang + 1 volume move per step, run once with FP32-compensated
+    vdW and once FP64 everywhere from the same seed, sampled every `interval`
+    steps.  Returns, per observable (E, H, V, density, N_HB), the largest
+    per-sample relative difference and the relative differences of the
+    long-run mean and variance (FP64 accumulators in both runs)."""
+    import numpy as np
+
+    from cvfbulkwater_amd import FP32_COMPENSATED, FP64, VOLUME_DEVICE, Params, Simulation
+
+    L = 96
+    N = L ** 3
+    series, ms = {}, {}
+    for name, prec in (("fp32c", FP32_COMPENSATED), ("fp64", FP64)):
+        sim = Simulation(Params(L=L, T=0.65, P=0.30, seed=20240611, n_metropolis=1, n_cluster=1,
+                                precision=prec, volume_mode=VOLUME_DEVICE))
+        rows = []
+        for _ in range(steps // interval):
+            sim.step(interval)
+            o = sim.observe()
+            rows.append((o.E, o.H, o.V, N / o.V, o.n_hb))
+        if ev is not None:
+            ms[name] = time_steps(sim, 200, 0, ev)
+        sim.close()
+        series[name] = np.array(rows)
+    a, b = series["fp32c"], series["fp64"]
+    errors = {}
+    for k, name in enumerate(("E", "H", "V", "density", "N_HB")):
+        x, y = a[:, k], b[:, k]
+        errors[name] = {
+            "max_rel": float(np.max(np.abs(x - y) / np.abs(y))),
+            "mean_rel": float(abs(x.mean() - y.mean()) / abs(y.mean())),
+            "var_rel": float(abs(x.var() - y.var()) / y.var()) if y.var() > 0 else 0.0,
+            "mean_fp64": float(y.mean()),
+        }
+    out = {"workload": f"L={L}, T=0.65, P=0.30, 1 Metropolis + 1 Swendsen-Wang + 1 volume move "
+                       f"per step, {steps} steps sampled every {interval}, FP32-compensated vdW vs "
+                       "FP64 everywhere, same seed", "errors": errors}
+    if ms:
+        out["ms_per_step"] = ms
+    return out
+
+
+def plain_per_L(steps: int, warmup: int, ev: HipEvents) -> dict:
+    """The batch run as 128 sequential plain drop-in runs (one state point
+    per handle, FP64, host volume move): each L's plain step time times its
+    32 points."""
+    from cvfbulkwater_amd import FP64, VOLUME_HOST, Params, Simulation
+
+    per = {}
+    for L in (16, 24, 32, 48):
+        sim = Simulation(Params(L=L, T=0.815, P=0.45, seed=44648604, precision=FP64,
+                                volume_mode=VOLUME_HOST))
+        per[L] = time_steps(sim, steps, warmup, ev)
+        sim.close()
+    cells = 32 * sum(L ** 3 for L in per)
+    total_ms = 32 * sum(per.values())
+    return {"ms_per_step_by_L": per, "value": cells / (total_ms * 1e-3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,7 +209,8 @@ def main() -> int:
     ap.add_argument("--tune-steps", type=int, default=20,
                     help="MC steps per autotuner trial (0: default launch configuration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["batch", "plain"], default=None,
+    ap.add_argument("--precision-steps", type=int, default=5000)
+    ap.add_argument("--only", choices=["batch", "plain", "precision"], default=None,
                     help="run one leg only (for profiling)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (calibrates the "
@@ -193,7 +255,10 @@ def main() -> int:
         ptuning = sim.autotune(args.tune_steps) if args.tune_steps else sim.tuning
         pms = time_steps(sim, args.plain_steps, args.warmup, ev)
         sim.close()
-        out.update(plain_ms=pms, plain_rate=32 ** 3 / (pms * 1e-3), plain_tuning=ptuning)
+        out.update(plain_ms=pms, plain_rate=32 ** 3 / (pms * 1e-3), plain_tuning=ptuning,
+                   seq=plain_per_L(max(args.plain_steps // 4, 100), args.warmup, ev))
+    if args.only in (None, "precision"):
+        out["precision"] = precision_form(args.precision_steps, 10, ev)
 
     line = {"metric": "site-updates/s per GPU, batched mixed-prec vs plain; achieved HBM GB/s",
             "unit": "site-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -217,10 +282,18 @@ def main() -> int:
                                      "volume move", "value": out["plain_rate"],
                          "ms_per_step": out["plain_ms"], "steps": args.plain_steps,
                          "tuning": out["plain_tuning"]}
+        line["plain_sequential_128"] = {
+            "workload": "the 128 batch state points as sequential plain runs (FP64, host volume "
+                        "move), from the per-L plain step times", **out["seq"]}
         if "batch_rate" in out:
             line["batched_over_plain"] = out["batch_rate"] / out["plain_rate"]
+            line["batched_over_sequential_plain"] = out["batch_rate"] / out["seq"]["value"]
         else:
             line.update(value=out["plain_rate"], ms_per_step=out["plain_ms"])
+    if "precision" in out:
+        line["precision_form"] = out["precision"]
+        if "value" not in line:
+            line.update(value=None, ms_per_step=out["precision"].get("ms_per_step", {}).get("fp32c"))
     if not args.no_cpu_baseline and args.only is None:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
         line["cpu_baseline"]["cores"] = 1

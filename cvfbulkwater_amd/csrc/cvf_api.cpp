@@ -152,8 +152,8 @@ int validate(const cvf_params& p) {
   if (!(p.T > 0.f)) return fail(CVF_EINVAL, "T must be > 0");
   if (p.n_metropolis < 0 || p.n_cluster < 0) return fail(CVF_EINVAL, "negative step counts");
   if (p.n_metropolis > 4096 || p.n_cluster > 4096) return fail(CVF_EINVAL, "step counts too large");
-  if (p.flag_chessboard != 1)
-    return fail(CVF_EUNSUPPORTED, "Flag_chessboard=%d is not supported (only 1)", p.flag_chessboard);
+  if (p.flag_chessboard != 0 && p.flag_chessboard != 1)
+    return fail(CVF_EINVAL, "Error: Wrong value for flag_chessboard = %d (0 or 1)", p.flag_chessboard);
   if (p.flag_config < 0 || p.flag_config > 2)
     return fail(CVF_EINVAL, "Error: Wrong value for flag_config = %d", p.flag_config);
   if (p.precision != CVF_FP64 && p.precision != CVF_FP32_COMPENSATED)
@@ -241,6 +241,48 @@ void for_each_cube(uint32_t L, F f) {
         }
 }
 
+// Per-cell HB counters of the non-chessboard mode (gpu_water3D.cu:845-862):
+// the number of arms whose sigma equals the facing arm of the neighbour,
+// weighted by eta, into bits 24-31.
+void set_nhb(uint32_t L, std::vector<uint32_t>& w) {
+  const uint32_t N = L * L * L;
+  std::vector<uint32_t> n(N);
+  for (uint32_t c = 0; c < N; ++c) {
+    const uint32_t x = c % L, y = (c / L) % L, z = c / (L * L);
+    const uint32_t nb[6] = {(x + 1) % L + L * (y + L * z), (x + L - 1) % L + L * (y + L * z),
+                            x + L * ((y + 1) % L + L * z), x + L * ((y + L - 1) % L + L * z),
+                            x + L * (y + L * ((z + 1) % L)), x + L * (y + L * ((z + L - 1) % L))};
+    uint32_t acc = 0;
+    for (int a = 0; a < 6; ++a)
+      acc += eta_of(w[c], a) * (sigma_of(w[c], a) == sigma_of(w[nb[a]], a ^ 1));
+    n[c] = acc & 0xFF;
+  }
+  for (uint32_t c = 0; c < N; ++c) w[c] = (w[c] & kLowBits) | (n[c] << kNhbShift);
+}
+
+// Capped HB count of measure() without the chessboard (gpu_water3D.cu:
+// 2064-2099): cells in index order, forward x, y, z bonds; a bond counts for
+// both cells while both counters are below NHBMAX = 4.  Returns 2 N_HB.
+int64_t greedy_nhb2(uint32_t L, const std::vector<uint32_t>& w) {
+  const uint32_t N = L * L * L;
+  std::vector<uint8_t> cnt(N, 0);
+  int64_t total = 0;
+  for (uint32_t c = 0; c < N; ++c) {
+    const uint32_t x = c % L, y = (c / L) % L, z = c / (L * L);
+    const uint32_t nb[3] = {(x + 1) % L + L * (y + L * z), x + L * ((y + 1) % L + L * z),
+                            x + L * (y + L * ((z + 1) % L))};
+    for (int ax = 0; ax < 3; ++ax) {
+      const uint32_t j = nb[ax];
+      if (sigma_of(w[c], 2 * ax) == sigma_of(w[j], 2 * ax + 1) && cnt[c] < 4 && cnt[j] < 4) {
+        ++cnt[c];
+        ++cnt[j];
+        total += 2;
+      }
+    }
+  }
+  return total;
+}
+
 // Initial state of one point (gpu_water3D.cu:639-826 with Philox streams).
 void init_words(const PointHost& P, std::vector<uint32_t>& w) {
   const uint32_t L = P.L, N = P.N;
@@ -258,6 +300,12 @@ void init_words(const PointHost& P, std::vector<uint32_t>& w) {
       for (int a = 0; a < 6; ++a) s |= uniform_int(rw[a], 6) << (3 * a);
       w[c] = s;
     }
+  }
+  if (!P.prm.flag_chessboard) {
+    // every bond active (gpu_water3D.cu:721-724)
+    for (uint32_t c = 0; c < N; ++c) w[c] |= 0x3Fu << kEtaShift;
+    set_nhb(L, w);
+    return;
   }
   for_each_cube(L, [&](uint32_t ox, uint32_t oy, uint32_t oz, uint32_t cube) {
     const U4 r = draw(seed, kStInitCube, cube, 0, 0);
@@ -324,7 +372,8 @@ BlockEnt make_entry(const PointHost& P, int point, uint32_t start) {
   e.mP4 = make_fastdiv(LL / 4).m;
   e.mC = make_fastdiv((P.L / 4) * (P.L / 4)).m;
   e.T = 0;
-  e.pad[0] = e.pad[1] = e.pad[2] = 0;
+  e.flags = P.prm.flag_chessboard ? 0u : kEntNoChess;
+  e.pad[0] = e.pad[1] = 0;
   return e;
 }
 
@@ -393,7 +442,10 @@ int upload_tables(cvf_sim* s) {
 }
 
 uint32_t cells_sigma(const PointHost& P) { return P.N / 8; }  // quads of active cells
-uint32_t cells_eta(const PointHost& P) { return (P.L / 2) * (P.L / 4) * (P.L / 4); }  // cube quads
+// cube quads; no cube moves without the chessboard (gpu_water3D.cu:1348)
+uint32_t cells_eta(const PointHost& P) {
+  return P.prm.flag_chessboard ? (P.L / 2) * (P.L / 4) * (P.L / 4) : 0;
+}
 uint32_t cells_all(const PointHost& P) { return P.N; }
 
 int valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512 || b == 1024; }
@@ -440,7 +492,7 @@ int build_groups(cvf_sim* s) {
     for (int p : g->points) {
       const PointHost& P = s->ph[p];
       const uint32_t T = choose_slab(P.L, s->tune.slab_planes);
-      if (!T) {
+      if (!T || !P.prm.flag_chessboard) {
         g->fused = false;
         break;
       }
@@ -495,8 +547,10 @@ void enqueue_step(cvf_sim* s, int gi, hipStream_t st) {
       launch_chunk(st, lz, s->pts.p, src, dst, (uint32_t)(c * kChunk), kChunk, c == 0, g.chunk_lds);
     }
   } else {
-    launch_eta(st, lc, s->pts.p, s->words.p, 0);
-    launch_eta(st, lc, s->pts.p, s->words.p, 1);
+    if (g.nb_cube) {
+      launch_eta(st, lc, s->pts.p, s->words.p, 0);
+      launch_eta(st, lc, s->pts.p, s->words.p, 1);
+    }
     for (int sub = 0; sub < 12 * s->nm; ++sub)
       launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub);
   }
@@ -750,6 +804,11 @@ int cvf_set_config(cvf_sim* s, int32_t pt, const uint8_t* sigma6N, const uint8_t
     }
     w[c] = pack_cell(s6, e6);
   }
+  if (!P.prm.flag_chessboard) {
+    // eta kept as read, no cube decoding (gpu_water3D.cu:794)
+    set_nhb(L, w);
+    return upload_point_words(s, pt, w);
+  }
   int bad = 0;
   std::vector<uint32_t> masks;
   std::vector<uint32_t> org;
@@ -853,7 +912,14 @@ int cvf_observe(cvf_sim* s, int32_t pt, cvf_obs* o) {
   rc = cvf_get_volume(s, pt, &r_cell, &V, &E_vdw);
   if (rc) return rc;
   const PointHost& P = s->ph[pt];
-  const int64_t nhb2 = (int64_t)s->obs_counters[2 * pt], nsig = (int64_t)s->obs_counters[2 * pt + 1];
+  int64_t nhb2 = (int64_t)s->obs_counters[2 * pt];
+  const int64_t nsig = (int64_t)s->obs_counters[2 * pt + 1];
+  if (!P.prm.flag_chessboard) {
+    // the capped count is sequential over cells: taken on the host
+    std::vector<uint32_t> w;
+    if ((rc = download_point_words(s, pt, w))) return rc;
+    nhb2 = greedy_nhb2(P.L, w);
+  }
   // measure(), gpu_water3D.cu:2138-2152
   const double n_hb = 0.5 * (double)nhb2;
   double e = E_vdw + n_hb * (double)(-P.prm.J_hb);
@@ -1054,7 +1120,7 @@ int cvf_profile_steps(cvf_sim* s, int64_t n_steps, double* kernel_ms, int64_t* l
           });
         }
       } else {
-        for (int f = 0; f < 2; ++f)
+        for (int f = 0; f < 2 && g.nb_cube; ++f)
           timed(CVF_KCLASS_ETA, [&] { launch_eta(st, lc, s->pts.p, s->words.p, f); });
         for (int sub = 0; sub < 12 * s->nm; ++sub)
           timed(CVF_KCLASS_SIGMA, [&] { launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub); });
@@ -1191,7 +1257,7 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
   const cvf_tuning base = default_tuning(s);
   for (int streams : {1, 2, 4}) {
     if (streams > s->npts) continue;
-    for (int slab : {0, 4, 2})
+    for (int slab : {0, 4, 2, 1})
       for (int bc : {256, 512, 1024}) {
         cvf_tuning t = base;
         t.streams = streams;

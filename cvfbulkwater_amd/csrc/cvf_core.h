@@ -26,6 +26,9 @@ constexpr int kEdges = 12;
 constexpr int kNSol = 9;
 constexpr uint32_t kSigmaBits = 0x3FFFFu;
 constexpr uint32_t kEtaShift = 18;
+// bits 24-31: per-cell HB counter of the non-chessboard mode (uint8 arithmetic)
+constexpr uint32_t kNhbShift = 24;
+constexpr uint32_t kLowBits = 0x00FFFFFFu;
 constexpr uint32_t kFieldLsb = 0x9249u;  // bit 0 of each 3-bit sigma field
 constexpr int kMaxL = 252;               // fast division exact for L <= 255
 

@@ -44,8 +44,11 @@ struct BlockEnt {
   uint32_t thb_lo, thb_hi, ts_lo, ts_hi;  // SW link thresholds
   uint32_t mLL, mP8, mP4, mC;  // multipliers for L^2, L^2/8, L^2/4, L^2/16
   uint32_t T;                  // slab thickness (planes) of the fused kernel
-  uint32_t pad[3];
+  uint32_t flags;              // kEntNoChess: the point runs without the chessboard
+  uint32_t pad[2];
 };
+
+constexpr uint32_t kEntNoChess = 1;
 
 constexpr int kChunk = 6;  // half-sweeps per fused kernel
 

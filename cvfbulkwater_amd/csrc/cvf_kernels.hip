@@ -261,6 +261,32 @@ __device__ __forceinline__ void sigma_update(uint32_t* __restrict__ W, const uin
   if ((uint64_t)ra < thr[(h + 1) * 11 + d + 5]) W[c] = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
 }
 
+// Non-chessboard rule (gpu_water3D.cu:1966-1984, 2004-2007): with dN =
+// delta(new, nb) - delta(old, nb), the HB term is
+// [n_c + dN <= 4][n_nb + dN <= 4] delta(new, nb) - [n_c <= 4][n_nb <= 4]
+// delta(old, nb), with the uint8 counters n in bits 24-31; an accepted move
+// adds dN to both counters.  The neighbour has the other parity along the
+// arm's axis, so no other thread of the launch reads or writes its word.
+__device__ __forceinline__ void sigma_update_nc(uint32_t* __restrict__ W, const uint64_t* thr, uint32_t c,
+                                                uint32_t nbc, int arm, uint32_t rw) {
+  const uint32_t w = W[c];
+  const uint32_t wn = W[nbc];
+  const uint32_t sa = sigma_of(w, arm);
+  uint32_t sp, ra;
+  split_word(rw, 6, sp, ra);
+  if (sp == sa) return;
+  const uint32_t snb = sigma_of(wn, arm ^ 1);
+  const int dn = (int)(sp == snb) - (int)(sa == snb);
+  const int ni = (int)(w >> kNhbShift), nj = (int)(wn >> kNhbShift);
+  const int h = ((ni + dn <= 4) && (nj + dn <= 4) && sp == snb) - ((ni <= 4) && (nj <= 4) && sa == snb);
+  const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
+  if ((uint64_t)ra < thr[(h + 1) * 11 + d + 5]) {
+    W[c] = (w & kLowBits & ~(7u << (3 * arm))) | (sp << (3 * arm)) |
+           ((uint32_t)((ni + dn) & 0xFF) << kNhbShift);
+    if (dn) W[nbc] = (wn & kLowBits) | ((uint32_t)((nj + dn) & 0xFF) << kNhbShift);
+  }
+}
+
 // One thread per quad of active cells (x + b L/2, y + a L/2, z), a,b in
 // {0,1}, x,y < L/2: the four have the same parity along every axis (L/2 is
 // even) and share one Philox block keyed on the a=b=0 cell (word a + 2b).
@@ -321,6 +347,13 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   }
   uint32_t* W = words + e.woff;
   const U4 r = draw(seed, kStSigma, c0, step, sub);
+  if (e.flags & kEntNoChess) {
+    sigma_update_nc(W, thr, c[0], nbc[0], arm, r.x);
+    sigma_update_nc(W, thr, c[1], nbc[1], arm, r.y);
+    sigma_update_nc(W, thr, c[2], nbc[2], arm, r.z);
+    sigma_update_nc(W, thr, c[3], nbc[3], arm, r.w);
+    return;
+  }
   sigma_update(W, thr, c[0], nbc[0], arm, r.x);
   sigma_update(W, thr, c[1], nbc[1], arm, r.y);
   sigma_update(W, thr, c[2], nbc[2], arm, r.z);

@@ -27,6 +27,7 @@ struct ora_sim {
   int L, N, NC;
   uint8_t *s;        /* sigma [6][N] */
   uint8_t *eta;      /* eta   [6][N] */
+  uint8_t *nhb;      /* nhb [N], non-chessboard mode (gpu_water3D.cu:125, 845-862) */
   int32_t *state;    /* cube state [N/4] */
   int32_t *vert;     /* [2][12][N/4] cube edge end cells */
   int32_t *label;    /* [6N] */
@@ -279,6 +280,21 @@ static void compute_thresholds(ora_sim *s) {
   s->thr_s = thr_from_p(1.0 - exp(-s->beta * fabs((double)p->J_sig)));
 }
 
+/* Initial HB counters, gpu_water3D.cu:845-862: per cell, the number of
+ * arms whose sigma equals the facing arm of the neighbour, weighted by eta
+ * (uint8 arithmetic). */
+static void compute_nhb(ora_sim *s) {
+  int L = s->L, N = s->N;
+  for (int c = 0; c < N; ++c) {
+    uint8_t acc = 0;
+    for (int a = 0; a < ARMS; ++a) {
+      int nb = neighbor(L, c, a);
+      acc = (uint8_t)(acc + s->eta[a * N + c] * (s->s[a * N + c] == s->s[(a ^ 1) * N + nb]));
+    }
+    s->nhb[c] = acc;
+  }
+}
+
 ora_sim *ora_create(const ora_params *p) {
   if (p->L < 4 || p->L % 4 != 0) return NULL;
   init_solutions();
@@ -289,6 +305,7 @@ ora_sim *ora_create(const ora_params *p) {
   s->NC = s->N / 4;
   s->s = (uint8_t *)calloc((size_t)ARMS * s->N, 1);
   s->eta = (uint8_t *)calloc((size_t)ARMS * s->N, 1);
+  s->nhb = (uint8_t *)calloc((size_t)s->N, 1);
   s->state = (int32_t *)calloc((size_t)s->NC, sizeof(int32_t));
   s->vert = (int32_t *)calloc((size_t)2 * EDGES * s->NC, sizeof(int32_t));
   s->label = (int32_t *)calloc((size_t)ARMS * s->N, sizeof(int32_t));
@@ -312,13 +329,19 @@ ora_sim *ora_create(const ora_params *p) {
       for (int a = 0; a < ARMS; ++a) s->s[a * N + c] = (uint8_t)uniform_int(w[a], 6);
     }
   }
-  /* random initial cube states (gpu_Chessboard_start_cubes, 3259-3312) */
-  for (int cube = 0; cube < s->NC; ++cube) {
-    uint32_t r[4];
-    draw(p->seed, ORA_ST_INIT_CUBE, (uint32_t)cube, 0, 0, r);
-    s->state[cube] = (int32_t)uniform_int(r[0], 9);
-    write_cube_edges(s, cube);
+  if (p->flag_chessboard) {
+    /* random initial cube states (gpu_Chessboard_start_cubes, 3259-3312) */
+    for (int cube = 0; cube < s->NC; ++cube) {
+      uint32_t r[4];
+      draw(p->seed, ORA_ST_INIT_CUBE, (uint32_t)cube, 0, 0, r);
+      s->state[cube] = (int32_t)uniform_int(r[0], 9);
+      write_cube_edges(s, cube);
+    }
+  } else {
+    /* every bond active (gpu_water3D.cu:721-724) */
+    memset(s->eta, 1, (size_t)ARMS * N);
   }
+  compute_nhb(s);
   /* volume chain (gpu_water3D.cu:150, 481, 1128) */
   s->r_cell = 1.0001;
   s->V = N * s->r_cell * s->r_cell * s->r_cell;
@@ -329,7 +352,7 @@ ora_sim *ora_create(const ora_params *p) {
 
 void ora_destroy(ora_sim *s) {
   if (!s) return;
-  free(s->s); free(s->eta); free(s->state); free(s->vert); free(s->label); free(s->uf);
+  free(s->s); free(s->eta); free(s->nhb); free(s->state); free(s->vert); free(s->label); free(s->uf);
   free(s);
 }
 
@@ -344,6 +367,11 @@ int ora_set_config(ora_sim *s, const uint8_t *sigma6N, const uint8_t *eta6N) {
     if (sigma6N[i] > 5 || eta6N[i] > 1) return -1;
   memcpy(s->s, sigma6N, (size_t)ARMS * N);
   memcpy(s->eta, eta6N, (size_t)ARMS * N);
+  if (!s->p.flag_chessboard) {
+    /* eta is kept as read (no cube decoding, gpu_water3D.cu:794) */
+    compute_nhb(s);
+    return 0;
+  }
   int bad = 0;
   for (int cube = 0; cube < s->NC; ++cube) {
     s->state[cube] = cube_state_from_eta(s, cube);
@@ -358,6 +386,7 @@ int ora_set_config(ora_sim *s, const uint8_t *sigma6N, const uint8_t *eta6N) {
     }
     write_cube_edges(s, cube);
   }
+  compute_nhb(s);
   return bad;
 }
 
@@ -369,6 +398,8 @@ void ora_get_config(const ora_sim *s, uint8_t *sigma6N, uint8_t *eta6N) {
 void ora_get_cube_states(const ora_sim *s, int32_t *states) {
   memcpy(states, s->state, sizeof(int32_t) * (size_t)s->NC);
 }
+
+void ora_get_nhb(const ora_sim *s, uint8_t *nhbN) { memcpy(nhbN, s->nhb, (size_t)s->N); }
 
 uint64_t ora_get_step(const ora_sim *s) { return s->step; }
 void ora_set_step(ora_sim *s, uint64_t step) { s->step = step; }
@@ -429,9 +460,13 @@ void ora_schedule(uint64_t seed, uint64_t step, uint32_t sub, int *arm, int *ind
   *indices = (int)(r[1] >> 31);
 }
 
-/* gpu_update (gpu_water3D.cu:1890-2008), chessboard branch: one arm of the
- * cells whose coordinate along the arm's axis is even (indices=1) or odd
- * (indices=0). */
+/* gpu_update (gpu_water3D.cu:1890-2008): one arm of the cells whose
+ * coordinate along the arm's axis is even (indices=1) or odd (indices=0).
+ * Chessboard branch: the HB term is eta * (delta(new,nb) - delta(old,nb)).
+ * Non-chessboard branch (1981-1984, 2004-2007): with dN = delta(new,nb) -
+ * delta(old,nb), the HB term is [nhb_i+dN <= 4][nhb_j+dN <= 4] delta(new,nb)
+ * - [nhb_i <= 4][nhb_j <= 4] delta(old,nb), and an accepted move adds dN to
+ * the uint8 counters of both cells. */
 void ora_sigma_halfsweep(ora_sim *s, uint64_t step, uint32_t sub, int arm, int indices) {
   int L = s->L, N = s->N;
   for (int c = 0; c < N; ++c) {
@@ -453,14 +488,25 @@ void ora_sigma_halfsweep(ora_sim *s, uint64_t step, uint32_t sub, int arm, int i
     split_word(r[ay + 2 * bx], 6, &prop, &acc);
     int nw = (int)prop;
     if (nw == old) continue;
-    int h = s->eta[arm * N + c] * ((nw == nbspin) - (old == nbspin));
+    int dn = (nw == nbspin) - (old == nbspin);
+    int h;
+    if (s->p.flag_chessboard) {
+      h = s->eta[arm * N + c] * dn;
+    } else {
+      int ni = s->nhb[c], nj = s->nhb[nb];
+      h = ((ni + dn <= 4) && (nj + dn <= 4)) * (nw == nbspin) - ((ni <= 4) && (nj <= 4)) * (old == nbspin);
+    }
     int d = 0;
     for (int k = 0; k < ARMS; ++k) {
       if (k == arm) continue;
       int sk = s->s[k * N + c];
       d += (nw == sk) - (old == sk);
     }
-    if ((uint64_t)acc < s->thr_sigma[(h + 1) * 11 + (d + 5)]) s->s[arm * N + c] = (uint8_t)nw;
+    if ((uint64_t)acc < s->thr_sigma[(h + 1) * 11 + (d + 5)]) {
+      s->s[arm * N + c] = (uint8_t)nw;
+      s->nhb[c] = (uint8_t)(s->nhb[c] + dn);
+      s->nhb[nb] = (uint8_t)(s->nhb[nb] + dn);
+    }
   }
 }
 
@@ -546,7 +592,7 @@ void ora_sw_step(ora_sim *s, uint64_t step, uint32_t jc) {
 
 void ora_step(ora_sim *s) {
   uint64_t t = s->step;
-  ora_eta_sweep(s, t);
+  if (s->p.flag_chessboard) ora_eta_sweep(s, t);
   ora_volume_move(s, t);
   for (int j = 0; j < s->p.n_metropolis; ++j)
     for (int k = 0; k < 2 * ARMS; ++k) {
@@ -564,8 +610,10 @@ void ora_steps(ora_sim *s, int64_t n) {
 }
 
 /* ------------------------------------------------------------ observables */
-/* measure(), gpu_water3D.cu:2014-2160, chessboard branch, same FP64
- * accumulation order. */
+/* measure(), gpu_water3D.cu:2014-2160, same FP64 accumulation order.
+ * Chessboard branch: every HB counts with its two eta.  Non-chessboard
+ * branch (2064-2099): cells in index order, forward x, y, z bonds; an HB
+ * counts for both cells only while both counters are below NHBMAX = 4. */
 void ora_measure(const ora_sim *s, ora_obs *o) {
   int L = s->L, N = s->N;
   const uint8_t *S = s->s, *A = s->eta;
@@ -577,8 +625,13 @@ void ora_measure(const ora_sim *s, ora_obs *o) {
       int a0 = 2 * ax, a1 = a0 + 1;
       int nb = neighbor(L, i, a0);
       if (S[a0 * N + i] == S[a1 * N + nb]) {
-        cnt[i] += A[a0 * N + i];
-        cnt[nb] += A[a1 * N + nb];
+        if (s->p.flag_chessboard) {
+          cnt[i] += A[a0 * N + i];
+          cnt[nb] += A[a1 * N + nb];
+        } else if (cnt[i] < 4 && cnt[nb] < 4) {
+          ++cnt[i];
+          ++cnt[nb];
+        }
       }
     }
     for (int j = 0; j < ARMS - 1; ++j)

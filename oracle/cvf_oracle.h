@@ -24,6 +24,10 @@
  *   sigma/eta/volume  exact single-variable transition probabilities vs.
  *                     empirical frequencies; long-run means vs SOURCE/data.out
  *   SW labels         BFS on small lattices (canonical minimum-node labels)
+ *   non-chessboard    nhb bookkeeping invariant, exact single-variable
+ *                     transition probabilities of the NHBMAX-gated rule, the
+ *                     greedy capped N_HB count vs a brute-force restatement
+ *                     on small lattices
  * The exact per-seed trajectories of the reference depend on cuRAND XORWOW
  * streams and cannot be reproduced: the oracle's trajectories are pinned to
  * the reference only statistically (parity unpinned bitwise at that
@@ -51,6 +55,7 @@ typedef struct ora_params {
   float r_cutoff;
   float J_hb, J_sig, v_hb;
   int32_t flag_config;     /* 0 random, 1 from arrays (ora_set_config), 2 ordered */
+  int32_t flag_chessboard; /* 1: eta cube moves; 0: eta fixed, HB gated by NHBMAX */
 } ora_params;
 
 typedef struct ora_obs {
@@ -83,6 +88,8 @@ void ora_destroy(ora_sim *s);
 int ora_set_config(ora_sim *s, const uint8_t *sigma6N, const uint8_t *eta6N);
 void ora_get_config(const ora_sim *s, uint8_t *sigma6N, uint8_t *eta6N);
 void ora_get_cube_states(const ora_sim *s, int32_t *states);
+/* per-cell HB counter nhb[N] of the non-chessboard mode */
+void ora_get_nhb(const ora_sim *s, uint8_t *nhbN);
 
 /* One full MC step of schedule R (gpu_water3D.cu:1346-1733). */
 void ora_step(ora_sim *s);

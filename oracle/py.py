@@ -23,6 +23,7 @@ class _Params(C.Structure):
         ("L", C.c_int32), ("P", C.c_float), ("T", C.c_float), ("seed", C.c_uint64),
         ("n_metropolis", C.c_int32), ("n_cluster", C.c_int32), ("r_cutoff", C.c_float),
         ("J_hb", C.c_float), ("J_sig", C.c_float), ("v_hb", C.c_float), ("flag_config", C.c_int32),
+        ("flag_chessboard", C.c_int32),
     ]
 
 
@@ -66,6 +67,7 @@ def lib() -> C.CDLL:
             "ora_set_config": (C.c_int, [P, U8P, U8P]),
             "ora_get_config": (None, [P, U8P, U8P]),
             "ora_get_cube_states": (None, [P, I32P]),
+            "ora_get_nhb": (None, [P, U8P]),
             "ora_step": (None, [P]),
             "ora_steps": (None, [P, C.c_int64]),
             "ora_eta_sweep": (None, [P, C.c_uint64]),
@@ -143,10 +145,11 @@ def cluster_stats_labels(L: int, labels: np.ndarray):
 
 class OracleSim:
     def __init__(self, L=32, P=0.45, T=0.815, seed=44648604, n_metropolis=1, n_cluster=0,
-                 r_cutoff=6.0, J_hb=0.5, J_sig=0.08, v_hb=0.6, flag_config=0):
+                 r_cutoff=6.0, J_hb=0.5, J_sig=0.08, v_hb=0.6, flag_config=0, flag_chessboard=1):
         self.L = L
         self.N = L ** 3
-        p = _Params(L, P, T, seed, n_metropolis, n_cluster, r_cutoff, J_hb, J_sig, v_hb, flag_config)
+        p = _Params(L, P, T, seed, n_metropolis, n_cluster, r_cutoff, J_hb, J_sig, v_hb, flag_config,
+                    flag_chessboard)
         self._h = lib().ora_create(C.byref(p))
         if not self._h:
             raise ValueError("ora_create failed")
@@ -178,6 +181,11 @@ class OracleSim:
     def cube_states(self):
         out = np.empty(self.N // 4, np.int32)
         lib().ora_get_cube_states(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
+
+    def nhb(self):
+        out = np.empty(self.N, np.uint8)
+        lib().ora_get_nhb(self._h, out.ctypes.data_as(C.POINTER(C.c_uint8)))
         return out
 
     def step(self, n: int = 1):

@@ -130,3 +130,54 @@ def test_config_roundtrip_and_normalisation():
         eng.step(1)
         ora.step(1)
     assert_same_state(eng, ora, "after 50 steps")
+
+
+@pytest.mark.parametrize("L,nc,flag_config", [(8, 0, 0), (16, 0, 2), (12, 1, 0), (8, 1, 2), (32, 0, 0)])
+def test_nonchessboard_mode_every_step(L, nc, flag_config):
+    """Flag_chessboard = 0: eta fixed, HB gated by the per-cell counters;
+    sigma, eta, labels and the capped N_HB identical to the oracle."""
+    kw = dict(L=L, T=0.6, P=0.45, seed=77, n_cluster=nc, flag_config=flag_config)
+    eng = Simulation(Params(flag_chessboard=0, **kw))
+    ora = O.OracleSim(flag_chessboard=0, **kw)
+    assert_same_obs(eng, ora, "init")
+    for t in range(60 if L < 32 else 20):
+        eng.step(1)
+        ora.step(1)
+        assert_same_state(eng, ora, f"step {t}")
+        if nc:
+            assert np.array_equal(eng.labels(), ora.labels()), f"labels step {t}"
+        if t % 10 == 0:
+            assert_same_obs(eng, ora, f"step {t}")
+    assert eng.volume() == ora.volume()
+
+
+def test_nonchessboard_input_config_keeps_eta():
+    L, N = 8, 512
+    rng = np.random.default_rng(9)
+    s = rng.integers(0, 6, 6 * N).astype(np.uint8)
+    e = rng.integers(0, 2, 6 * N).astype(np.uint8)
+    kw = dict(L=L, T=0.6, P=0.45, seed=3, n_cluster=1, flag_config=1)
+    eng = Simulation(Params(flag_chessboard=0, **kw))
+    ora = O.OracleSim(flag_chessboard=0, **kw)
+    assert eng.set_config(s, e) == 0 and ora.set_config(s, e) == 0
+    assert_same_state(eng, ora, "set")
+    for t in range(40):
+        eng.step(1)
+        ora.step(1)
+        assert_same_state(eng, ora, f"step {t}")
+        assert np.array_equal(eng.labels(), ora.labels()), f"labels step {t}"
+    assert np.array_equal(eng.get_config()[1], e)
+    assert_same_obs(eng, ora, "end")
+
+
+def test_batch_mixing_chessboard_modes_equals_single_runs():
+    pts = [Params(L=L, T=0.6 + 0.05 * i, P=0.45, seed=500 + i, n_cluster=1, flag_chessboard=i % 2,
+                  volume_mode=VOLUME_DEVICE) for i, L in enumerate([8, 16, 12, 8])]
+    batch = Simulation(pts)
+    singles = [Simulation(p) for p in pts]
+    batch.step(30)
+    for i, sim in enumerate(singles):
+        sim.step(30)
+        a, b = batch.get_config(i), sim.get_config()
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
+        assert batch.observe(i).nhb2 == sim.observe().nhb2, i
