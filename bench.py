@@ -137,6 +137,30 @@ def calibrate_copy(mib: float, reps: int = 10) -> int:
     return 0
 
 
+def tuning_signature(kname: str, t: dict) -> str:
+    if kname == "k_chunk":
+        return f"streams:{t['streams']},block:{t['block_chunk']},fused:1,slab:{t['slab_planes']}"
+    return f"streams:{t['streams']},block:{t['block_sigma']},fused:0"
+
+
+def measured_traffic(kname: str, tuning: dict):
+    """HBM bytes per dispatch of the roofline kernel under this launch
+    configuration, from the FETCH_SIZE / WRITE_SIZE summary kept in
+    profiles/ (lines "traffic kernel=... tuning=... bytes=..."), or None
+    when no counter run covers the configuration."""
+    path = ROOT / "profiles" / "batch128_traffic.txt"
+    if not path.exists():
+        return None
+    sig = tuning_signature(kname, tuning)
+    for line in path.read_text().splitlines():
+        if not line.startswith("traffic "):
+            continue
+        kv = dict(f.split("=", 1) for f in line.split()[1:])
+        if kv.get("kernel") == kname and kv.get("tuning") == sig:
+            return float(kv["bytes"])
+    return None
+
+
 def precision_form(steps: int = 5000, interval: int = 10, ev: HipEvents | None = None) -> dict:
     """Mixed-precision large lattice: L = 96, T = 0.65, P = 0.30, 1 Metropolis
     + 1 Swendsen-Wang + 1 volume move per step, run once with FP32-compensated
@@ -208,6 +232,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tune-steps", type=int, default=20,
                     help="MC steps per autotuner trial (0: default launch configuration)")
+    ap.add_argument("--tuning", type=json.loads, default=None, metavar="JSON",
+                    help="launch configuration of the batch leg, e.g. '{\"fused\": 0}' "
+                         "(skips the autotuner)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-steps", type=int, default=5000)
     ap.add_argument("--only", choices=["batch", "plain", "precision"], default=None,
@@ -232,7 +259,11 @@ def main() -> int:
         pts = batch_params(fp32=True)
         cells = sum(p.L ** 3 for p in pts)
         sim = Simulation(pts)
-        tuning = sim.autotune(args.tune_steps) if args.tune_steps else sim.tuning
+        if args.tuning is not None:
+            sim.tuning = args.tuning
+            tuning = sim.tuning
+        else:
+            tuning = sim.autotune(args.tune_steps) if args.tune_steps else sim.tuning
         ms = time_steps(sim, args.steps, args.warmup, ev)
         prof = sim.profile_steps(args.profile_steps)
         sim.close()
@@ -274,7 +305,8 @@ def main() -> int:
                           "tuning": out["tuning"]}
         line["roofline"] = {"bound": "hbm", "kernel": out["kname"], "achieved": out["achieved"],
                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": out["achieved"] / HBM_PEAK_GBS, "traffic": None,
+                            "frac": out["achieved"] / HBM_PEAK_GBS,
+                            "traffic": measured_traffic(out["kname"], out["tuning"]),
                             "bytes_per_launch": out["sig_bytes"], "launch_ms": out["sig_ms"]}
         line["kernel_ms"] = {k: v[0] for k, v in out["prof"].items() if v[1]}
     if "plain_rate" in out:

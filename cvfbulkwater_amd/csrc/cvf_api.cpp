@@ -387,7 +387,7 @@ uint32_t choose_slab(uint32_t L, int requested) {
   if (requested > 0 && L % (uint32_t)requested == 0 &&
       slab_lds_bytes(L, (uint32_t)requested) <= chunk_lds_limit())
     return (uint32_t)requested;
-  for (uint32_t T = 8; T >= 1; --T)
+  for (uint32_t T = kMaxSlab; T >= 1; --T)
     if (L % T == 0 && slab_lds_bytes(L, T) <= chunk_lds_limit()) return T;
   return 0;
 }
@@ -505,6 +505,11 @@ int build_groups(cvf_sim* s) {
       }
     }
     if (g->fused) {
+      // largest slabs first (the kernel takes entries in launch order, and
+      // each XCD's dispatcher hands out its share in that order)
+      std::stable_sort(ts.begin(), ts.end(), [](const BlockEnt& a, const BlockEnt& b) {
+        return (uint64_t)a.L * a.L * (a.T + 2) > (uint64_t)b.L * b.L * (b.T + 2);
+      });
       while (ts.size() % kNumXcd) {
         BlockEnt pad;
         memset(&pad, 0, sizeof pad);
@@ -623,7 +628,8 @@ int check_tuning(const cvf_tuning& t) {
     return fail(CVF_EINVAL, "block sizes must be 64, 128, 256, 512 or 1024");
   if (t.graph_steps < 1 || t.graph_steps > 64) return fail(CVF_EINVAL, "graph_steps must be in [1, 64]");
   if (!valid_block(t.block_chunk)) return fail(CVF_EINVAL, "block_chunk must be a power of 2 in [64, 1024]");
-  if (t.slab_planes < 0 || t.slab_planes > 64) return fail(CVF_EINVAL, "slab_planes must be in [0, 64]");
+  if (t.slab_planes < 0 || t.slab_planes > kMaxSlab)
+    return fail(CVF_EINVAL, "slab_planes must be in [0, %d]", kMaxSlab);
   if (t.fused != 0 && t.fused != 1) return fail(CVF_EINVAL, "fused must be 0 or 1");
   return CVF_OK;
 }

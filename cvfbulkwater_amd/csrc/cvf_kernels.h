@@ -50,7 +50,8 @@ struct BlockEnt {
 
 constexpr uint32_t kEntNoChess = 1;
 
-constexpr int kChunk = 6;  // half-sweeps per fused kernel
+constexpr int kChunk = 6;    // half-sweeps per fused kernel
+constexpr int kMaxSlab = 8;  // planes per slab of the fused kernel
 
 struct Launch {
   const BlockEnt* tab;

@@ -463,6 +463,26 @@ __device__ __forceinline__ uint32_t wrap_plane(int z, int L) {
   return (uint32_t)z;
 }
 
+// Work items it = tid, tid + nt, ... enumerated as it = pi * Q + u (plane
+// pi, in-plane item u) and advanced without a division per item.
+struct PlaneIter {
+  uint32_t pi, u;
+};
+__device__ __forceinline__ PlaneIter plane_first(uint32_t tid, uint32_t Q, FastDiv dq) {
+  const uint32_t pi = fdiv(tid, dq);
+  return PlaneIter{pi, tid - pi * Q};
+}
+__device__ __forceinline__ void plane_next(PlaneIter& it, uint32_t Q, uint32_t dp, uint32_t du) {
+  it.u += du;
+  it.pi += dp;
+  if (it.u >= Q) {
+    it.u -= Q;
+    ++it.pi;
+  }
+}
+
+constexpr int kMaxWin = kMaxSlab + kChunk + 2;  // window planes incl. halos
+
 __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
                         const PointDev* __restrict__ pts, const uint32_t* __restrict__ src,
                         uint32_t* __restrict__ dst, uint32_t k0, uint32_t nk, int with_eta) {
@@ -470,7 +490,10 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   __shared__ uint64_t thr_s[33];
   __shared__ uint64_t thr_e[9];
   __shared__ uint32_t sch[kChunk];
-  const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
+  __shared__ uint32_t gofs[kMaxWin];  // L^2 * global z of each window plane
+  // the table is ordered by decreasing slab work and used in launch order,
+  // so every XCD's dispatcher takes the large slabs first
+  const BlockEnt e = tab[blockIdx.x];
   if (e.point < 0) return;
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const PointDev& P = pts[e.point];
@@ -496,29 +519,29 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   }
   const int ext = with_eta ? 1 : 0;
   const int wlo = -h_lo - ext, whi = T + h_hi + ext, np = whi - wlo;
+  if (tid < (uint32_t)np) gofs[tid] = wrap_plane(z0 + wlo + (int)tid, L) * (uint32_t)LL;
+  __syncthreads();
   // stage the window
   const uint32_t* S = src + e.woff;
-  for (uint32_t i = tid; i < (uint32_t)(np * LL); i += nt) {
-    const uint32_t wz = fdiv(i, dLL);
-    const uint32_t rem = i - wz * LL;
-    win[i] = S[wrap_plane(z0 + wlo + (int)wz, L) * LL + rem];
+  {
+    const uint32_t Q = (uint32_t)LL, dp = fdiv(nt, dLL), du = nt - dp * Q;
+    for (PlaneIter it = plane_first(tid, Q, dLL); it.pi < (uint32_t)np; plane_next(it, Q, dp, du))
+      win[it.pi * Q + it.u] = S[gofs[it.pi] + it.u];
   }
   __syncthreads();
   if (with_eta) {
     // cubes with both planes inside the window, family by family
+    const uint32_t Q = (uint32_t)(qL * qL), dp = fdiv(nt, dC), du = nt - dp * Q;
     for (int fam = 0; fam < 2; ++fam) {
       // first window plane whose global z has the family's parity
       const int first = ((z0 + wlo) & 1) == fam ? 0 : 1;
       const int nplanes = first < np - 1 ? (np - 1 - first + 1) / 2 : 0;
-      const uint32_t work = (uint32_t)nplanes * (uint32_t)(qL * qL);
-      for (uint32_t it = tid; it < work; it += nt) {
-        const uint32_t pi = fdiv(it, dC);
-        const uint32_t u = it - pi * (uint32_t)(qL * qL);
-        const uint32_t jj = fdiv(u, dQ), ii = u - jj * qL;
-        const int wz = first + 2 * (int)pi;
+      for (PlaneIter it = plane_first(tid, Q, dC); it.pi < (uint32_t)nplanes; plane_next(it, Q, dp, du)) {
+        const uint32_t jj = fdiv(it.u, dQ), ii = it.u - jj * qL;
+        const int wz = first + 2 * (int)it.pi;
         const uint32_t ox = 2 * ii + fam, oy = 2 * jj + fam;
-        const uint32_t gz = wrap_plane(z0 + wlo + wz, L);
-        const uint32_t cube_ref = ox / 2 + hL * (oy / 2) + hL * hL * gz;  // gpu_water3D.cu:3405
+        // cube index x/2 + (L/2)(y/2) + (L^2/4) z (gpu_water3D.cu:3405)
+        const uint32_t cube_ref = ii + hL * jj + (gofs[wz] >> 2);
         const U4 r = draw(seed, kStEta, cube_ref, step, 0);
         uint32_t* Wp = win + wz * LL;  // cube origin plane; the cube spans wz, wz+1
         eta_cube_plane(Wp, thr_e, (uint32_t)L, (uint32_t)LL, ox, oy, r.x);
@@ -539,39 +562,33 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
     const int rlo = -rem_lo, rhi = T + rem_hi;  // planes updated by this half-sweep
     const uint32_t sub = k0 + k;
     if (ax < 2) {
-      const uint32_t work = (uint32_t)(rhi - rlo) * (uint32_t)(LL / 8);
-      for (uint32_t it = tid; it < work; it += nt) {
-        const uint32_t pi = fdiv(it, dP8);
-        const uint32_t u = it - pi * (uint32_t)(LL / 8);
+      const uint32_t Q = (uint32_t)(LL / 8), dp = fdiv(nt, dP8), du = nt - dp * Q;
+      const uint32_t P = (uint32_t)(rhi - rlo), wz0 = (uint32_t)(rlo - wlo);
+      for (PlaneIter it = plane_first(tid, Q, dP8); it.pi < P; plane_next(it, Q, dp, du)) {
         uint32_t x, y;
         if (ax == 0) {  // u = hx + qL y
-          y = fdiv(u, dQ);
-          x = 2 * (u - y * qL) + q;
+          y = fdiv(it.u, dQ);
+          x = 2 * (it.u - y * qL) + q;
         } else {  // u = x + hL hy
-          const uint32_t hy = fdiv(u, dH);
-          x = u - hy * hL;
+          const uint32_t hy = fdiv(it.u, dH);
+          x = it.u - hy * hL;
           y = 2 * hy + q;
         }
-        const int zr = rlo + (int)pi;
-        const uint32_t wz = (uint32_t)(zr - wlo);
-        const uint32_t gz = (uint32_t)(z0 + zr + 2 * L) % (uint32_t)L;
+        const uint32_t wz = wz0 + it.pi;
         sigma_quad_plane(win + wz * LL, thr_s, (uint32_t)L, x, y, arm, ax,
-                         draw(seed, kStSigma, x + L * y + LL * gz, step, sub));
+                         draw(seed, kStSigma, x + L * y + gofs[wz], step, sub));
       }
     } else {
       // planes whose global z has the kept parity
       const int first = rlo + ((((z0 + rlo) & 1) == (int)q) ? 0 : 1);
       const int nplanes = first < rhi ? (rhi - first + 1) / 2 : 0;
-      const uint32_t work = (uint32_t)nplanes * (uint32_t)(LL / 4);
-      for (uint32_t it = tid; it < work; it += nt) {
-        const uint32_t pi = fdiv(it, dP4);
-        const uint32_t u = it - pi * (uint32_t)(LL / 4);
-        const uint32_t y = fdiv(u, dH), x = u - y * hL;
-        const int zr = first + 2 * (int)pi;
-        const uint32_t wz = (uint32_t)(zr - wlo);
-        const uint32_t gz = (uint32_t)(z0 + zr + 2 * L) % (uint32_t)L;
+      const uint32_t Q = (uint32_t)(LL / 4), dp = fdiv(nt, dP4), du = nt - dp * Q;
+      const uint32_t wz0 = (uint32_t)(first - wlo);
+      for (PlaneIter it = plane_first(tid, Q, dP4); it.pi < (uint32_t)nplanes; plane_next(it, Q, dp, du)) {
+        const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
+        const uint32_t wz = wz0 + 2 * it.pi;
         sigma_quad_z(win + wz * LL, (int)LL, thr_s, (uint32_t)L, x, y, arm,
-                     draw(seed, kStSigma, x + L * y + LL * gz, step, sub));
+                     draw(seed, kStSigma, x + L * y + gofs[wz], step, sub));
       }
     }
     __syncthreads();
