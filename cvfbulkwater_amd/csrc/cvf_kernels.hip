@@ -247,7 +247,14 @@ __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint6
 // (indices = 0).  A cell's new sigma_a depends on its own word and on the
 // facing arm of the neighbour across arm a; that neighbour has the other
 // parity along the axis, so no word read in a launch is written in it.
-__device__ __forceinline__ void sigma_update(uint32_t* __restrict__ W, const uint64_t* thr, uint32_t c,
+// Sigma acceptance thresholds as 32-bit values min(thr, 2^32 - 1).  Exact:
+// the acceptance word of the q = 6 split is (6w) mod 2^32, always even, so
+// it is below 2^32 - 1 exactly when it is below 2^32.
+__device__ __forceinline__ uint32_t sigma_thr32(uint64_t t) {
+  return t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
+}
+
+__device__ __forceinline__ void sigma_update(uint32_t* __restrict__ W, const uint32_t* thr, uint32_t c,
                                              uint32_t nbc, int arm, uint32_t rw) {
   const uint32_t w = W[c];
   const uint32_t wn = W[nbc];
@@ -258,7 +265,7 @@ __device__ __forceinline__ void sigma_update(uint32_t* __restrict__ W, const uin
   const uint32_t snb = sigma_of(wn, arm ^ 1);
   const int h = eta_of(w, arm) ? ((int)(sp == snb) - (int)(sa == snb)) : 0;
   const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
-  if ((uint64_t)ra < thr[(h + 1) * 11 + d + 5]) W[c] = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
+  if (ra < thr[(h + 1) * 11 + d + 5]) W[c] = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
 }
 
 // Non-chessboard rule (gpu_water3D.cu:1966-1984, 2004-2007): with dN =
@@ -267,7 +274,7 @@ __device__ __forceinline__ void sigma_update(uint32_t* __restrict__ W, const uin
 // delta(old, nb), with the uint8 counters n in bits 24-31; an accepted move
 // adds dN to both counters.  The neighbour has the other parity along the
 // arm's axis, so no other thread of the launch reads or writes its word.
-__device__ __forceinline__ void sigma_update_nc(uint32_t* __restrict__ W, const uint64_t* thr, uint32_t c,
+__device__ __forceinline__ void sigma_update_nc(uint32_t* __restrict__ W, const uint32_t* thr, uint32_t c,
                                                 uint32_t nbc, int arm, uint32_t rw) {
   const uint32_t w = W[c];
   const uint32_t wn = W[nbc];
@@ -280,7 +287,7 @@ __device__ __forceinline__ void sigma_update_nc(uint32_t* __restrict__ W, const 
   const int ni = (int)(w >> kNhbShift), nj = (int)(wn >> kNhbShift);
   const int h = ((ni + dn <= 4) && (nj + dn <= 4) && sp == snb) - ((ni <= 4) && (nj <= 4) && sa == snb);
   const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
-  if ((uint64_t)ra < thr[(h + 1) * 11 + d + 5]) {
+  if (ra < thr[(h + 1) * 11 + d + 5]) {
     W[c] = (w & kLowBits & ~(7u << (3 * arm))) | (sp << (3 * arm)) |
            ((uint32_t)((ni + dn) & 0xFF) << kNhbShift);
     if (dn) W[nbc] = (wn & kLowBits) | ((uint32_t)((nj + dn) & 0xFF) << kNhbShift);
@@ -296,8 +303,8 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
                         const PointDev* __restrict__ pts, uint32_t* __restrict__ words, uint32_t sub) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
-  __shared__ uint64_t thr[33];
-  if (threadIdx.x < 33) thr[threadIdx.x] = pts[e.point].thr_sigma[threadIdx.x];
+  __shared__ uint32_t thr[33];
+  if (threadIdx.x < 33) thr[threadIdx.x] = sigma_thr32(pts[e.point].thr_sigma[threadIdx.x]);
   __syncthreads();
   const uint32_t L = e.L, hL = L / 2, qL = L / 4, N = e.N;
   const uint32_t t = e.start + threadIdx.x;
@@ -400,7 +407,7 @@ __device__ __forceinline__ void eta_cube_plane(uint32_t* Wp, const uint64_t* thr
 }
 
 // LDS form of sigma_update: cell word at `wc`, facing neighbour word `wn`.
-__device__ __forceinline__ void sigma_update_lds(uint32_t* wc, uint32_t wn, const uint64_t* thr, int arm,
+__device__ __forceinline__ void sigma_update_lds(uint32_t* wc, uint32_t wn, const uint32_t* thr, int arm,
                                                  uint32_t rw) {
   const uint32_t w = *wc;
   const uint32_t sa = sigma_of(w, arm);
@@ -410,29 +417,39 @@ __device__ __forceinline__ void sigma_update_lds(uint32_t* wc, uint32_t wn, cons
   const uint32_t snb = sigma_of(wn, arm ^ 1);
   const int h = eta_of(w, arm) ? ((int)(sp == snb) - (int)(sa == snb)) : 0;
   const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
-  if ((uint64_t)ra < thr[(h + 1) * 11 + d + 5]) *wc = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
+  if (ra < thr[(h + 1) * 11 + d + 5]) *wc = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
 }
 
 // in-plane (x or y arm) update of the quad with origin (x, y) on plane Wp
-__device__ __forceinline__ void sigma_quad_plane(uint32_t* Wp, const uint64_t* thr, uint32_t L, uint32_t x,
+__device__ __forceinline__ void sigma_quad_plane(uint32_t* Wp, const uint32_t* thr, uint32_t L, uint32_t x,
                                                  uint32_t y, int arm, int ax, U4 r) {
-  const uint32_t hL = L / 2;
-  const uint32_t xs[4] = {x, x, x + hL, x + hL}, ys[4] = {y, y + hL, y, y + hL};
+  // cells (x, y), (x, y + L/2), (x + L/2, y), (x + L/2, y + L/2) with x, y <
+  // L/2: along the arm's axis only the coordinate at 0 (-1 step) or L-1 (+1
+  // step) wraps, i.e. the low cells when x (y) = 0 and the high cells when
+  // x (y) = L/2 - 1
+  const int iL = (int)L, hL = iL / 2, cq = ax == 0 ? (int)x : (int)y, st = ax == 0 ? 1 : iL;
+  const int wrap = (iL - 1) * st;
+  int off_lo, off_hi;
+  if (arm & 1) {
+    off_lo = cq == 0 ? wrap : -st;
+    off_hi = -st;
+  } else {
+    off_lo = st;
+    off_hi = cq == hL - 1 ? -wrap : st;
+  }
+  const uint32_t b0 = x + L * y, dy = (uint32_t)hL * L, dx = (uint32_t)hL;
+  const uint32_t cell[4] = {b0, b0 + dy, b0 + dx, b0 + dx + dy};
+  // the cells on the high side of the arm's axis: 2, 3 for x, 1, 3 for y
+  const int off[4] = {off_lo, ax == 0 ? off_lo : off_hi, ax == 0 ? off_hi : off_lo, off_hi};
   const uint32_t rw[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    uint32_t nx = xs[m], ny = ys[m];
-    if (ax == 0)
-      nx = (arm & 1) ? (nx == 0 ? L - 1 : nx - 1) : (nx == L - 1 ? 0 : nx + 1);
-    else
-      ny = (arm & 1) ? (ny == 0 ? L - 1 : ny - 1) : (ny == L - 1 ? 0 : ny + 1);
-    sigma_update_lds(Wp + xs[m] + L * ys[m], Wp[nx + L * ny], thr, arm, rw[m]);
-  }
+  for (int m = 0; m < 4; ++m)
+    sigma_update_lds(Wp + cell[m], Wp[(int)cell[m] + off[m]], thr, arm, rw[m]);
 }
 
 // z-arm update of the quad with origin (x, y) on plane Wp: the neighbour is
 // on the plane above (arm 4) or below (arm 5)
-__device__ __forceinline__ void sigma_quad_z(uint32_t* Wp, int LL, const uint64_t* thr, uint32_t L, uint32_t x,
+__device__ __forceinline__ void sigma_quad_z(uint32_t* Wp, int LL, const uint32_t* thr, uint32_t L, uint32_t x,
                                              uint32_t y, int arm, U4 r) {
   const uint32_t hL = L / 2;
   const int dz = (arm & 1) ? -LL : LL;
@@ -487,7 +504,7 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
                         const PointDev* __restrict__ pts, const uint32_t* __restrict__ src,
                         uint32_t* __restrict__ dst, uint32_t k0, uint32_t nk, int with_eta) {
   extern __shared__ uint32_t win[];
-  __shared__ uint64_t thr_s[33];
+  __shared__ uint32_t thr_s[33];
   __shared__ uint64_t thr_e[9];
   __shared__ uint32_t sch[kChunk];
   __shared__ uint32_t gofs[kMaxWin];  // L^2 * global z of each window plane
@@ -497,7 +514,7 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   if (e.point < 0) return;
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const PointDev& P = pts[e.point];
-  if (tid < 33) thr_s[tid] = P.thr_sigma[tid];
+  if (tid < 33) thr_s[tid] = sigma_thr32(P.thr_sigma[tid]);
   if (tid < 9) thr_e[tid] = P.thr_eta[tid];
   const uint64_t step = *stepp;
   const uint64_t seed = seed_of(e);
