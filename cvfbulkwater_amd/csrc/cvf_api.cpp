@@ -450,6 +450,13 @@ uint32_t cells_all(const PointHost& P) { return P.N; }
 
 int valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512 || b == 1024; }
 
+// the groups' step counters take the handle's step count
+int upload_group_steps(cvf_sim* s) {
+  std::vector<uint64_t> v(s->groups.size(), (uint64_t)s->steps);
+  HIPC(hipMemcpy(s->cur_steps.p, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+  return CVF_OK;
+}
+
 // (Re)build the groups for the current tuning: points are dealt to groups
 // largest-first onto the group with the fewest cells.
 int build_groups(cvf_sim* s) {
@@ -521,7 +528,8 @@ int build_groups(cvf_sim* s) {
     }
   }
   HIPC(s->cur_steps.alloc(s->groups.size()));
-  HIPC(hipMemset(s->cur_steps.p, 0, s->groups.size() * 8));
+  int rc0 = upload_group_steps(s);
+  if (rc0) return rc0;
   std::vector<int> all(s->npts);
   for (int i = 0; i < s->npts; ++i) all[i] = i;
   const auto ta = make_table(s->ph, all, 256, cells_all);
@@ -532,45 +540,65 @@ int build_groups(cvf_sim* s) {
   return CVF_OK;
 }
 
-// One MC step of group g (MonteCarlo_Step, gpu_water3D.cu:1346-1733).
-void enqueue_step(cvf_sim* s, int gi, hipStream_t st) {
+struct Untimed {
+  template <class F>
+  void operator()(int, F&& f) const {
+    f();
+  }
+};
+
+// Lattice part of one MC step of group g (MonteCarlo_Step,
+// gpu_water3D.cu:1346-1733): the step is the group's counter + step_off.
+// `timed(class, launch)` wraps every launch (profiling) or just makes it.
+template <class Timed>
+void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Timed&& timed) {
   Group& g = *s->groups[gi];
   const uint64_t* stepp = s->cur_steps.p + gi;
-  const Launch lh{g.tab_half.p, g.nb_half, (uint32_t)s->tune.block_sigma, stepp};
-  const Launch lc{g.tab_cube.p, g.nb_cube, (uint32_t)s->tune.block_eta, stepp};
-  const Launch ll{g.tab_cell.p, g.nb_cell, (uint32_t)s->tune.block_cell, stepp};
-  launch_step_begin(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, s->cur_steps.p + gi,
-                    s->dist.p, s->freq.p, s->vol_device, s->precision);
+  const Launch lh{g.tab_half.p, g.nb_half, (uint32_t)s->tune.block_sigma, stepp, step_off};
+  const Launch lc{g.tab_cube.p, g.nb_cube, (uint32_t)s->tune.block_eta, stepp, step_off};
+  const Launch ll{g.tab_cell.p, g.nb_cell, (uint32_t)s->tune.block_cell, stepp, step_off};
   if (g.fused && s->nm > 0) {
     // 12*n_metropolis half-sweeps in chunks of kChunk, ping-pong between the
     // two word buffers (an even number of chunks ends in `words`)
-    const Launch lz{g.tab_slab.p, g.nb_slab, (uint32_t)s->tune.block_chunk, stepp};
+    const Launch lz{g.tab_slab.p, g.nb_slab, (uint32_t)s->tune.block_chunk, stepp, step_off};
     const int nchunks = 12 * s->nm / kChunk;
     for (int c = 0; c < nchunks; ++c) {
       const uint32_t* src = (c & 1) ? s->words2.p : s->words.p;
       uint32_t* dst = (c & 1) ? s->words.p : s->words2.p;
-      launch_chunk(st, lz, s->pts.p, src, dst, (uint32_t)(c * kChunk), kChunk, c == 0, g.chunk_lds);
+      timed(CVF_KCLASS_CHUNK, [&] {
+        launch_chunk(st, lz, s->pts.p, src, dst, (uint32_t)(c * kChunk), kChunk, c == 0, g.chunk_lds);
+      });
     }
   } else {
-    if (g.nb_cube) {
-      launch_eta(st, lc, s->pts.p, s->words.p, 0);
-      launch_eta(st, lc, s->pts.p, s->words.p, 1);
-    }
+    for (int f = 0; f < 2 && g.nb_cube; ++f)
+      timed(CVF_KCLASS_ETA, [&] { launch_eta(st, lc, s->pts.p, s->words.p, f); });
     for (int sub = 0; sub < 12 * s->nm; ++sub)
-      launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub);
+      timed(CVF_KCLASS_SIGMA, [&] { launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub); });
   }
   for (int jc = 0; jc < s->nc; ++jc) {
-    launch_sw_local(st, ll, s->words.p, s->parent.p, s->cov.p, (uint32_t)jc);
-    launch_sw_union(st, ll, s->parent.p, s->cov.p);
-    launch_sw_flip(st, ll, s->words.p, s->parent.p, (uint32_t)jc);
+    timed(CVF_KCLASS_SW_LOCAL, [&] { launch_sw_local(st, ll, s->words.p, s->parent.p, s->cov.p, (uint32_t)jc); });
+    timed(CVF_KCLASS_SW_UNION, [&] { launch_sw_union(st, ll, s->parent.p, s->cov.p); });
+    timed(CVF_KCLASS_SW_FLIP, [&] { launch_sw_flip(st, ll, s->words.p, s->parent.p, (uint32_t)jc); });
   }
+}
+
+// nsteps MC steps of group g: the lattice kernels of every step, then one
+// kernel with the device volume moves of all of them (the chain never reads
+// the lattice) that also moves the group's step counter on.
+int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps) {
+  Group& g = *s->groups[gi];
+  for (int i = 0; i < nsteps; ++i) enqueue_lattice(s, gi, st, (uint32_t)i, Untimed{});
+  launch_vol_chain(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, s->cur_steps.p + gi,
+                   (uint32_t)nsteps, s->dist.p, s->freq.p, s->precision, s->vol_device);
+  return CVF_OK;
 }
 
 int capture(cvf_sim* s, int gi, int nsteps, hipGraph_t* g, hipGraphExec_t* ge) {
   hipStream_t st = s->groups[gi]->stream;
   HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < nsteps; ++i) enqueue_step(s, gi, st);
+  const int rc = enqueue_steps(s, gi, st, nsteps);
   hipError_t e = hipStreamEndCapture(st, g);
+  if (rc) return rc;
   if (e != hipSuccess) return fail(CVF_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
   HIPC(hipGraphInstantiate(ge, *g, nullptr, nullptr, 0));
   return CVF_OK;
@@ -1106,38 +1134,11 @@ int cvf_profile_steps(cvf_sim* s, int64_t n_steps, double* kernel_ms, int64_t* l
     for (size_t gi = 0; gi < s->groups.size(); ++gi) {
       Group& g = *s->groups[gi];
       hipStream_t st = g.stream;
-      const uint64_t* stepp = s->cur_steps.p + gi;
-      const Launch lh{g.tab_half.p, g.nb_half, (uint32_t)s->tune.block_sigma, stepp};
-      const Launch lc{g.tab_cube.p, g.nb_cube, (uint32_t)s->tune.block_eta, stepp};
-      const Launch ll{g.tab_cell.p, g.nb_cell, (uint32_t)s->tune.block_cell, stepp};
+      enqueue_lattice(s, (int)gi, st, 0, timed);
       timed(CVF_KCLASS_BEGIN, [&] {
-        launch_step_begin(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p,
-                          s->cur_steps.p + gi, s->dist.p, s->freq.p, s->vol_device, s->precision);
+        launch_vol_chain(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, s->cur_steps.p + gi, 1,
+                         s->dist.p, s->freq.p, s->precision, s->vol_device);
       });
-      if (g.fused && s->nm > 0) {
-        const Launch lz{g.tab_slab.p, g.nb_slab, (uint32_t)s->tune.block_chunk, stepp};
-        const int nchunks = 12 * s->nm / kChunk;
-        for (int c = 0; c < nchunks; ++c) {
-          const uint32_t* src = (c & 1) ? s->words2.p : s->words.p;
-          uint32_t* dst = (c & 1) ? s->words.p : s->words2.p;
-          timed(CVF_KCLASS_CHUNK, [&] {
-            launch_chunk(st, lz, s->pts.p, src, dst, (uint32_t)(c * kChunk), kChunk, c == 0,
-                         g.chunk_lds);
-          });
-        }
-      } else {
-        for (int f = 0; f < 2 && g.nb_cube; ++f)
-          timed(CVF_KCLASS_ETA, [&] { launch_eta(st, lc, s->pts.p, s->words.p, f); });
-        for (int sub = 0; sub < 12 * s->nm; ++sub)
-          timed(CVF_KCLASS_SIGMA, [&] { launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub); });
-      }
-      for (int jc = 0; jc < s->nc; ++jc) {
-        timed(CVF_KCLASS_SW_LOCAL,
-              [&] { launch_sw_local(st, ll, s->words.p, s->parent.p, s->cov.p, (uint32_t)jc); });
-        timed(CVF_KCLASS_SW_UNION, [&] { launch_sw_union(st, ll, s->parent.p, s->cov.p); });
-        timed(CVF_KCLASS_SW_FLIP,
-              [&] { launch_sw_flip(st, ll, s->words.p, s->parent.p, (uint32_t)jc); });
-      }
     }
     ++s->steps;
   }
@@ -1180,6 +1181,8 @@ int cvf_set_tuning(cvf_sim* s, const cvf_tuning* t) {
   s->tune = *t;
   return build_groups(s);
 }
+
+}  // extern "C"
 
 namespace {
 
@@ -1233,6 +1236,8 @@ void cache_store(const std::string& key, const cvf_tuning& c) {
 }
 
 }  // namespace
+
+extern "C" {
 
 extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* chosen) {
   if (!s || steps_per_trial < 1) return fail(CVF_EINVAL, "bad arguments");
@@ -1309,6 +1314,7 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
   s->steps = steps0;
   s->obs_at = -1;
   if (rc) return rc;
+  if ((rc = upload_group_steps(s))) return rc;
   if ((rc = cvf_set_tuning(s, &best))) return rc;
   cache_store(key, best);
   if (chosen) *chosen = best;

@@ -57,7 +57,8 @@ struct Launch {
   const BlockEnt* tab;
   uint32_t nblocks;      // padded to a multiple of kNumXcd
   uint32_t block;        // threads per workgroup
-  const uint64_t* step;  // the step being executed (written by k_step_begin)
+  const uint64_t* step;  // the group's step counter (first step of the captured sequence)
+  uint32_t step_off;     // this launch's step within the sequence
 };
 
 // Per-kernel timing for profiling: while set, launches go through
@@ -65,9 +66,11 @@ struct Launch {
 void set_launch_timing(hipEvent_t start, hipEvent_t stop);
 
 // Launchers (all asynchronous on `st`).
-void launch_step_begin(hipStream_t st, const PointDev* pts, const int32_t* plist, int npts,
-                       PointState* ps, uint64_t* cur_step, const float* dist, const int32_t* freq,
-                       int vol_device, int precision);
+// after nsteps lattice steps of a group: *gstep += nsteps and, with `moves`,
+// the device volume moves of those steps for the listed points
+void launch_vol_chain(hipStream_t st, const PointDev* pts, const int32_t* plist, int npts, PointState* ps,
+                      uint64_t* gstep, uint32_t nsteps, const float* dist, const int32_t* freq,
+                      int precision, int moves);
 void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, int family);
 void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, uint32_t sub);
 // Fused z-slab kernel: optional eta sweep + half-sweeps k0 .. k0+nk-1 of the

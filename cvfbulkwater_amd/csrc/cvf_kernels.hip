@@ -134,37 +134,44 @@ __device__ double vdw_fp32_wave(double r, const float* dist, const int32_t* freq
 // in device-volume mode (volume_step, gpu_water3D.cu:1740-1781), advances the
 // point's step counter and publishes the step being executed for the
 // group's kernels.  All points of a handle advance in lockstep.
-__global__ __launch_bounds__(64) void k_step_begin(const PointDev* __restrict__ pts,
-                                                   const int32_t* __restrict__ plist,
-                                                   PointState* __restrict__ ps,
-                                                   uint64_t* __restrict__ cur_step,
-                                                   const float* __restrict__ dist,
-                                                   const int32_t* __restrict__ freq, int vol_device,
-                                                   int precision) {
-  const int p = plist[blockIdx.x];
+// End of a captured sequence of nsteps MC steps of a group: the device
+// volume moves of those steps (volume_step, gpu_water3D.cu:1740-1781), one
+// wavefront per point running them in order from the point's own counter,
+// and the group's step counter moved on.  The chain never reads the
+// lattice, so running its moves after the lattice kernels of the same steps
+// gives the same result as interleaving them.  No workgroup reads the group
+// counter here, so workgroup 0 may advance it while others still run.
+__global__ __launch_bounds__(64) void k_vol_chain(const PointDev* __restrict__ pts,
+                                                  const int32_t* __restrict__ plist,
+                                                  PointState* __restrict__ ps, uint64_t* __restrict__ gstep,
+                                                  uint32_t nsteps, const float* __restrict__ dist,
+                                                  const int32_t* __restrict__ freq, int precision,
+                                                  int moves) {
   const int lane = threadIdx.x;
+  if (blockIdx.x == 0 && lane == 0) *gstep += nsteps;
+  if (!moves) return;
+  const int p = plist[blockIdx.x];
   const PointDev& P = pts[p];
-  const uint64_t step = ps[p].step;
-  if (vol_device) {
-    VolState vs = ps[p].vol;
+  const uint64_t step0 = ps[p].step;
+  const float* d = dist + P.shell_off;
+  const int32_t* f = freq + P.shell_off;
+  VolState vs = ps[p].vol;
+  for (uint32_t i = 0; i < nsteps; ++i) {
     double u2 = 0;
-    const double trial = volume_trial(vs, P.vp, step, &u2);
-    const float* d = dist + P.shell_off;
-    const int32_t* f = freq + P.shell_off;
+    const double trial = volume_trial(vs, P.vp, step0 + i, &u2);
     double te;
     if (precision == 1)
       te = vdw_fp32_wave(trial, d, f, P.vp.nshell, P.vp.r_cutoff, P.vp.N);
     else
       te = vdw_energy_fp64(trial, P.vp.r_cutoff, d, f, P.vp.nshell);
-    if (lane == 0) {
-      volume_accept(vs, P.vp, trial, te, u2);
-      ps[p].vol = vs;
-    }
+    // lane 0's energy decides, as one value for the whole wavefront, so every
+    // lane keeps the same chain
+    te = __shfl(te, 0);
+    volume_accept(vs, P.vp, trial, te, u2);
   }
-  __syncthreads();
   if (lane == 0) {
-    ps[p].step = step + 1;
-    if (blockIdx.x == 0) *cur_step = step;
+    ps[p].vol = vs;
+    ps[p].step = step0 + nsteps;
   }
 }
 
@@ -216,6 +223,7 @@ __device__ __forceinline__ void eta_cube(uint32_t* __restrict__ W, const uint64_
 // One thread per quad of cubes of the family with origins
 // (ox + b L/2, oy + a L/2, oz), a,b in {0,1}: one Philox block, word a + 2b.
 __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
+                      uint32_t step_off,
                       const PointDev* __restrict__ pts, uint32_t* __restrict__ words, int family) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
@@ -225,7 +233,7 @@ __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint6
   const uint32_t L = e.L, hL = L / 2, qL = L / 4;
   const uint32_t t = e.start + threadIdx.x;
   if (t >= qL * qL * hL) return;
-  const uint64_t step = *stepp;
+  const uint64_t step = *stepp + step_off;
   const FastDiv dQ{qL, e.mQ};
   const uint32_t r1 = fdiv(t, dQ);  // t = i + qL (j + qL k), i,j < L/4, k < L/2
   const uint32_t i = t - r1 * qL;
@@ -300,6 +308,7 @@ __device__ __forceinline__ void sigma_update_nc(uint32_t* __restrict__ W, const 
 // The half-sweep's (arm, parity) is drawn in the kernel from uniform values
 // (gpu_water3D.cu:1393-1394 draws them on the host).
 __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
+                        uint32_t step_off,
                         const PointDev* __restrict__ pts, uint32_t* __restrict__ words, uint32_t sub) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
@@ -309,7 +318,7 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   const uint32_t L = e.L, hL = L / 2, qL = L / 4, N = e.N;
   const uint32_t t = e.start + threadIdx.x;
   if (t >= N / 8) return;
-  const uint64_t step = *stepp;
+  const uint64_t step = *stepp + step_off;
   const uint64_t seed = seed_of(e);
   const U4 sc = draw(seed, kStSched, 0, step, sub);
   const int arm = (int)uniform_int(sc.x, 6);
@@ -501,6 +510,7 @@ __device__ __forceinline__ void plane_next(PlaneIter& it, uint32_t Q, uint32_t d
 constexpr int kMaxWin = kMaxSlab + kChunk + 2;  // window planes incl. halos
 
 __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
+                        uint32_t step_off,
                         const PointDev* __restrict__ pts, const uint32_t* __restrict__ src,
                         uint32_t* __restrict__ dst, uint32_t k0, uint32_t nk, int with_eta) {
   extern __shared__ uint32_t win[];
@@ -516,7 +526,7 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   const PointDev& P = pts[e.point];
   if (tid < 33) thr_s[tid] = sigma_thr32(P.thr_sigma[tid]);
   if (tid < 9) thr_e[tid] = P.thr_eta[tid];
-  const uint64_t step = *stepp;
+  const uint64_t step = *stepp + step_off;
   const uint64_t seed = seed_of(e);
   if (tid < nk) {
     const U4 r = draw(seed, kStSched, 0, step, k0 + tid);
@@ -727,14 +737,15 @@ __device__ void uf_unite(uint32_t* par, uint32_t a, uint32_t b) {
 // neighbour) and 15 intra-molecule (arm pairs k<l in order).  Writes the
 // intra-cell forest into `parent` and the active covalent links into cov[c].
 __global__ void k_sw_local(const BlockEnt* __restrict__ tab, uint32_t nb,
-                           const uint64_t* __restrict__ stepp, const uint32_t* __restrict__ words,
+                           const uint64_t* __restrict__ stepp, uint32_t step_off,
+                           const uint32_t* __restrict__ words,
                            uint32_t* __restrict__ parent, uint8_t* __restrict__ cov, uint32_t jc) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
   const uint32_t t = e.start + threadIdx.x;
   if (t >= e.N) return;
   const uint32_t L = e.L, N = e.N;
-  const uint64_t step = *stepp;
+  const uint64_t step = *stepp + step_off;
   uint32_t x, y, z;
   cell_coords(e, t, x, y, z);
   const uint32_t* W = words + e.woff;
@@ -813,14 +824,15 @@ __global__ void k_sw_union(const BlockEnt* __restrict__ tab, uint32_t nb, uint32
 // Final labels (full path compression) and the cluster rotation of
 // gpu_update_cluster (gpu_water3D.cu:2531-2542): sigma += floor(6u(label)).
 __global__ void k_sw_flip(const BlockEnt* __restrict__ tab, uint32_t nb,
-                          const uint64_t* __restrict__ stepp, uint32_t* __restrict__ words,
+                          const uint64_t* __restrict__ stepp, uint32_t step_off,
+                          uint32_t* __restrict__ words,
                           uint32_t* __restrict__ parent, uint32_t jc) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
   const uint32_t t = e.start + threadIdx.x;
   if (t >= e.N) return;
   const uint32_t N = e.N;
-  const uint64_t step = *stepp;
+  const uint64_t step = *stepp + step_off;
   const uint64_t seed = seed_of(e);
   uint32_t* par = parent + e.loff;
   uint32_t* W = words + e.woff;
@@ -929,17 +941,19 @@ static void launch(K kernel, dim3 grid, dim3 block, hipStream_t st, A... args) {
     hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
 }
 
-void launch_step_begin(hipStream_t st, const PointDev* pts, const int32_t* plist, int npts,
-                       PointState* ps, uint64_t* cur_step, const float* dist, const int32_t* freq,
-                       int vol_device, int precision) {
-  launch(k_step_begin, dim3(npts), dim3(64), st, pts, plist, ps, cur_step, dist, freq, vol_device,
-         precision);
+void launch_vol_chain(hipStream_t st, const PointDev* pts, const int32_t* plist, int npts, PointState* ps,
+                      uint64_t* gstep, uint32_t nsteps, const float* dist, const int32_t* freq,
+                      int precision, int moves) {
+  launch(k_vol_chain, dim3(moves ? npts : 1), dim3(64), st, pts, plist, ps, gstep, nsteps, dist, freq,
+         precision, moves);
 }
 void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, int family) {
-  launch(k_eta, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, pts, words, family);
+  launch(k_eta, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, ln.step_off, pts, words,
+         family);
 }
 void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, uint32_t sub) {
-  launch(k_sigma, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, pts, words, sub);
+  launch(k_sigma, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, ln.step_off, pts, words,
+         sub);
 }
 uint32_t chunk_lds_limit() {
   // gfx950: 160 KB of LDS per CU; keep room for the static arrays
@@ -955,10 +969,10 @@ void launch_chunk(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t
   }
   if (g_t0)
     hipExtLaunchKernelGGL(k_chunk, dim3(ln.nblocks), dim3(ln.block), lds_bytes, st, g_t0, g_t1, 0,
-                          ln.tab, ln.nblocks, ln.step, pts, src, dst, k0, nk, with_eta);
+                          ln.tab, ln.nblocks, ln.step, ln.step_off, pts, src, dst, k0, nk, with_eta);
   else
     hipLaunchKernelGGL(k_chunk, dim3(ln.nblocks), dim3(ln.block), lds_bytes, st, ln.tab, ln.nblocks,
-                       ln.step, pts, src, dst, k0, nk, with_eta);
+                       ln.step, ln.step_off, pts, src, dst, k0, nk, with_eta);
 }
 void launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
   size_t blocks = (n + 255) / 256;
@@ -971,15 +985,15 @@ void launch_observe(hipStream_t st, Launch ln, const uint32_t* words, unsigned l
 }
 void launch_sw_local(hipStream_t st, Launch ln, const uint32_t* words, uint32_t* parent, uint8_t* cov,
                      uint32_t jc) {
-  launch(k_sw_local, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, words, parent,
-         cov, jc);
+  launch(k_sw_local, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, ln.step_off, words,
+         parent, cov, jc);
 }
 void launch_sw_union(hipStream_t st, Launch ln, uint32_t* parent, const uint8_t* cov) {
   launch(k_sw_union, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, parent, cov);
 }
 void launch_sw_flip(hipStream_t st, Launch ln, uint32_t* words, uint32_t* parent, uint32_t jc) {
-  launch(k_sw_flip, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, words, parent,
-         jc);
+  launch(k_sw_flip, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, ln.step_off, words,
+         parent, jc);
 }
 void launch_sigma_hist(hipStream_t st, const uint32_t* W, uint32_t N, unsigned long long* counts) {
   launch(k_sigma_hist, grid1(N, 256), dim3(256), st, W, N, counts);
