@@ -24,12 +24,14 @@ import numpy as np
 __all__ = [
     "CVFError", "Params", "Obs", "ClusterStats", "Simulation", "load_library",
     "FP64", "FP32_COMPENSATED", "VOLUME_HOST", "VOLUME_DEVICE", "vdw_energy",
-    "LIB_PATH", "EXE_PATH", "Correlator",
+    "LIB_PATH", "EXE_PATH", "THERMO_EXE_PATH", "Correlator", "KT_NOT_VALUE", "thermo_averages",
+    "thermo_derivative", "thermo_kt",
 ]
 
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libcvf.so"
 EXE_PATH = PKG / "cvf_water3d"
+THERMO_EXE_PATH = PKG / "cvf_thermo"
 
 FP64, FP32_COMPENSATED = 0, 1
 VOLUME_HOST, VOLUME_DEVICE = 0, 1
@@ -161,6 +163,9 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cvf_corr_add": (None, [C.c_void_p, C.c_double]),
         "cvf_corr_result": (C.c_int, [C.c_void_p, DP, DP, I32]),
         "cvf_corr_destroy": (None, [C.c_void_p]),
+        "cvf_thermo_averages": (C.c_int, [DP, C.c_int64, I32, DP]),
+        "cvf_thermo_derivative": (C.c_int, [DP, DP, DP, C.POINTER(C.c_int32), I32, I32, DP, DP, DP]),
+        "cvf_thermo_kt": (C.c_int, [DP, DP, DP, DP, I32, C.c_double, DP, DP]),
         "cvf_set_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
     }
     for name, (res, args) in sig.items():
@@ -371,3 +376,49 @@ class Correlator:
             self._lib.cvf_corr_destroy(self._h)
         except Exception:
             pass
+
+
+# ----------------------------------------------------- thermodynamic analysis
+KT_NOT_VALUE = -10.0
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def thermo_averages(rows) -> np.ndarray:
+    """Means and population variances of e, h, v, N_HB, m, r_cell over data.out
+    rows (columns 1..6 of each row after the step), as script_analisi.sh:
+    returns [mean_e, var_e, mean_h, var_h, ..., mean_r, var_r]."""
+    a = np.ascontiguousarray(np.asarray(rows, np.float64)[:, :6])
+    out = np.empty(12)
+    lib = load_library()
+    rc = lib.cvf_thermo_averages(_dp(a), a.shape[0], 6, _dp(out))
+    if rc:
+        raise CVFError(f"cvf_thermo_averages: {rc}")
+    return out
+
+
+def thermo_derivative(T, X, var, n, mode: int):
+    """d X/dT (mode 0, CPderivate) or (1/X) dX/dT (mode 1, AlphaPderivate)
+    with their error estimates, rows 2 .. len-3 of the T-sorted input."""
+    T, X, var = (np.ascontiguousarray(np.asarray(v, np.float64)) for v in (T, X, var))
+    n = np.ascontiguousarray(np.asarray(n, np.int32))
+    m = len(T)
+    to, d, e = np.empty(max(m, 1)), np.empty(max(m, 1)), np.empty(max(m, 1))
+    k = load_library().cvf_thermo_derivative(_dp(T), _dp(X), _dp(var), n.ctypes.data_as(C.POINTER(C.c_int32)),
+                                             m, mode, _dp(to), _dp(d), _dp(e))
+    if k < 0:
+        raise CVFError(f"cvf_thermo_derivative: {k}")
+    return to[:k].copy(), d[:k].copy(), e[:k].copy()
+
+
+def thermo_kt(T, P, rho, rho_err, gas_rho: float = 500.0):
+    """K_T = d ln(rho)/dP along isotherms with its error (KTderivate.c);
+    KT_NOT_VALUE where undefined."""
+    T, P, rho, rho_err = (np.ascontiguousarray(np.asarray(v, np.float64)) for v in (T, P, rho, rho_err))
+    kt, ekt = np.empty(max(len(T), 1)), np.empty(max(len(T), 1))
+    rc = load_library().cvf_thermo_kt(_dp(T), _dp(P), _dp(rho), _dp(rho_err), len(T), gas_rho, _dp(kt), _dp(ekt))
+    if rc:
+        raise CVFError(f"cvf_thermo_kt: {rc}")
+    return kt[:len(T)].copy(), ekt[:len(T)].copy()

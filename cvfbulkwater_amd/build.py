@@ -2,6 +2,7 @@
 
     libcvf.so      HIP kernels + C ABI (include/cvf_api.h), gfx950
     cvf_water3d    drop-in executable (input_data/input_config -> data.out ...)
+    cvf_thermo     drop-in for the thermodynamic analysis scripts (data.out -> isobars)
 
 Run ``python -m cvfbulkwater_amd.build`` or call :func:`build`.
 """
@@ -33,8 +34,9 @@ COMMON = [
     "-Wall", "-Wno-unused-function",
 ]
 
-LIB_SOURCES = ["cvf_kernels.hip", "cvf_api.cpp", "cvf_host.cpp", "cvf_corr.cpp"]
-EXE_SOURCES = ["cvf_water3d.cpp"]
+LIB_SOURCES = ["cvf_kernels.hip", "cvf_api.cpp", "cvf_host.cpp", "cvf_corr.cpp", "cvf_thermo.cpp"]
+# executable -> sources (each linked against libcvf.so)
+EXES = {"cvf_water3d": ["cvf_water3d.cpp"], "cvf_thermo": ["cvf_thermo_main.cpp"]}
 
 
 def _run(cmd: list[str]) -> None:
@@ -63,14 +65,15 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *objs])
     os.replace(tmp, LIB)
-    exe_objs = []
-    for name in EXE_SOURCES:
-        src = CSRC / name
-        obj = BUILD / (src.stem + ".o")
-        _compile(src, obj)
-        exe_objs.append(str(obj))
-    _run([HIPCC, f"--offload-arch={ARCH}", "-o", str(EXE), *exe_objs,
-          f"-L{PKG}", "-lcvf", f"-Wl,-rpath,$ORIGIN"])
+    for exe, sources in EXES.items():
+        exe_objs = []
+        for name in sources:
+            src = CSRC / name
+            obj = BUILD / (src.stem + ".o")
+            _compile(src, obj)
+            exe_objs.append(str(obj))
+        _run([HIPCC, f"--offload-arch={ARCH}", "-o", str(PKG / exe), *exe_objs,
+              f"-L{PKG}", "-lcvf", f"-Wl,-rpath,$ORIGIN"])
     return LIB
 
 

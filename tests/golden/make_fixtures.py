@@ -4,12 +4,14 @@
 Reads (as data, nothing is executed) from the reference checkout:
   SOURCE/distances, SOURCE/data.out, SOURCE/cluster.out, SOURCE/water3D.log,
   SOURCE/input_data, SOURCE/input_config, SOURCE/config.out,
-  PROCESSED_DATA/CVF_isobars.tar.gz (L32_P*_density.dat)
+  PROCESSED_DATA/CVF_isobars.tar.gz (L32_P*_density.dat, .dat.SI,
+  _AlphaPderivate.dat.SI, _KTderivate.dat.SI)
 and writes into tests/golden/:
   distances_L32, input_data, data.out.xz, input_config.xz, config.out.xz,
   reference_values.json (log summary lines, cluster.out first row, SHA-256 of
   the configs, N_HB / N_sigma of both configs computed here with numpy, and
-  the published L=32 density isobars).
+  the published L=32 density isobars), isobars_L32.tar.xz (the processed
+  isobar files above, byte for byte, for the thermodynamic post-processing).
 
 Usage: python tests/golden/make_fixtures.py [/path/to/reference]
 """
@@ -86,6 +88,18 @@ def main(ref: Path) -> None:
                 rows = np.loadtxt(io.TextIOWrapper(tf.extractfile(m)), ndmin=2)
                 iso[P] = rows.tolist()
     vals["published_density_isobars_L32"] = dict(sorted(iso.items(), key=lambda kv: float(kv[0])))
+    keep = ("_density.dat", "_density.dat.SI", "_AlphaPderivate.dat.SI", "_KTderivate.dat.SI")
+    buf = io.BytesIO()
+    with tarfile.open(ref / "PROCESSED_DATA" / "CVF_isobars.tar.gz") as tf, \
+            tarfile.open(fileobj=buf, mode="w") as out:
+        for m in sorted(tf.getmembers(), key=lambda m: m.name):
+            n = Path(m.name).name
+            if m.isfile() and n.startswith("L32_P") and n.endswith(keep):
+                data = tf.extractfile(m).read()
+                info = tarfile.TarInfo(n)
+                info.size = len(data)
+                out.addfile(info, io.BytesIO(data))
+    (HERE / "isobars_L32.tar.xz").write_bytes(lzma.compress(buf.getvalue(), preset=9 | lzma.PRESET_EXTREME))
     (HERE / "reference_values.json").write_text(json.dumps(vals, indent=1) + "\n")
     print("fixtures written to", HERE)
 

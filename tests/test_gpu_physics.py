@@ -241,3 +241,36 @@ def test_config5_precision_form_errors():
     for k in ("E", "H", "V", "density"):
         assert err[k]["max_rel"] <= 1e-6, (k, err[k])
         assert err[k]["mean_rel"] <= 1e-6, (k, err[k])
+
+
+def test_isobar_scan_feeds_the_analysis(tmp_path):
+    """A small isobar scan in one batch: the averages written from memory
+    equal those cvf_thermo computes from the data.out files of the same run
+    (to the 6 printed digits), and the analysis files exist."""
+    import subprocess
+
+    from cvfbulkwater_amd import THERMO_EXE_PATH
+    from cvfbulkwater_amd.scan import isobar_scan
+
+    a = tmp_path / "mem"
+    isobar_scan(a, 8, [0.6, 0.65, 0.7, 0.75, 0.8, 0.85], [0.45, 0.3], equil_steps=50, samples=40,
+                interval=5, data_out=True)
+    b = tmp_path / "files"
+    b.mkdir()
+    for d in a.glob("L8_T*_P*"):
+        (b / d.name).mkdir()
+        (b / d.name / "data.out").write_text((d / "data.out").read_text())
+    r = subprocess.run([str(THERMO_EXE_PATH), str(b)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    x = np.loadtxt(a / "L8_average_data.dat")
+    y = np.loadtxt(b / "L8_average_data.dat")
+    key = lambda m: m[np.lexsort((m[:, 2], m[:, 1]))]
+    x, y = key(x), key(y)
+    assert x.shape == (12, 16)
+    # means to the printed digits; variances lose more to data.out's 8 digits
+    assert np.array_equal(x[:, [0, 1, 2, 15]], y[:, [0, 1, 2, 15]])
+    assert np.allclose(x[:, 3:15:2], y[:, 3:15:2], rtol=2e-5)
+    assert np.allclose(x[:, 4:15:2], y[:, 4:15:2], rtol=5e-2, atol=1e-9)
+    for P in ("0.45", "0.3"):
+        for suffix in ("density.dat", "AlphaP.dat", "CP.dat", "density.dat.SI", "AlphaPderivate.dat.SI"):
+            assert (a / f"L8_P{P}_{suffix}").stat().st_size > 0, suffix

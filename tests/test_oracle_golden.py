@@ -538,9 +538,10 @@ def test_library_exports_every_declared_symbol():
 
     if not LIB_PATH.exists():
         pytest.skip("libcvf.so not built")
-    hdr = (Path(__file__).resolve().parent.parent / "include" / "cvf_api.h").read_text()
-    names = set(re.findall(r"\b(cvf_[a-z0-9_]+)\s*\(", hdr))
+    names = set()
+    for hdr in sorted((Path(__file__).resolve().parent.parent / "include").glob("*.h")):
+        names |= set(re.findall(r"\b(cvf_[a-z0-9_]+)\s*\(", hdr.read_text()))
     lib = C.CDLL(str(LIB_PATH))
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
-    assert len(names) >= 18
+    assert len(names) >= 21 and "cvf_thermo_kt" in names
