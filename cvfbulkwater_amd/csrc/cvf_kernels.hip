@@ -182,8 +182,52 @@ __global__ __launch_bounds__(64) void k_vol_chain(const PointDev* __restrict__ p
 // thread updates its 8 corner words without atomics.  Cube states of both
 // families are independent given sigma, so running the families one after
 // the other equals the reference's simultaneous update.
-__device__ __forceinline__ void eta_cube(uint32_t* __restrict__ W, const uint64_t* thr, uint32_t L,
-                                         uint32_t ox, uint32_t oy, uint32_t oz, uint32_t rw) {
+// Cube-move table: entry s holds the active-edge mask of solution s in bits
+// 0-11 and, in bits 12 + 6b .. 17 + 6b, the eta bits (arms 0..5) that the
+// mask gives corner b (b = bx + 2 by + 4 bz) on the arms it owns.
+__device__ __forceinline__ uint64_t eta_table_entry(uint32_t s) {
+  const uint32_t M = sol_mask(s);
+  uint64_t t = M;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
+    const uint32_t bits = (((M >> corner_edge(b, 0)) & 1u) << bx) | (((M >> corner_edge(b, 1)) & 1u) << (2 + by)) |
+                          (((M >> corner_edge(b, 2)) & 1u) << (4 + bz));
+    t |= (uint64_t)bits << (12 + 6 * b);
+  }
+  return t;
+}
+
+// One cube move on its 8 corner words w (corner b = bx + 2 by + 4 bz):
+// proposal floor(9u), Delta N_HB over the 12 edges, accept with
+// min(1, exp(beta Jeff DeltaN)); store(b, word) writes a changed corner.
+template <class Store>
+__device__ __forceinline__ void eta_cube_move(const uint32_t (&w)[8], const uint64_t* thr, const uint64_t* etab,
+                                              uint32_t rw, Store&& store) {
+  uint32_t D = 0, E = 0;
+#pragma unroll
+  for (int jj = 0; jj < kEdges; ++jj) {
+    const int A = edge_start(jj), ax = edge_axis(jj), B = A | (1 << ax);
+    D |= (uint32_t)(sigma_of(w[A], 2 * ax) == sigma_of(w[B], 2 * ax + 1)) << jj;
+    E |= eta_of(w[A], 2 * ax) << jj;
+  }
+  uint32_t prop, acc;
+  split_word(rw, 9, prop, acc);
+  const uint64_t ent = etab[prop];
+  const uint32_t M = (uint32_t)ent & 0xFFFu;
+  const int dn = popc32(M & D) - popc32(E & D);
+  if ((uint64_t)acc < thr[dn + 4] && M != E) {
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint32_t own = (1u << (b & 1)) | (1u << (2 + ((b >> 1) & 1))) | (1u << (4 + ((b >> 2) & 1)));
+      const uint32_t bits = (uint32_t)(ent >> (12 + 6 * b)) & 0x3Fu;
+      store(b, (w[b] & ~(own << kEtaShift)) | (bits << kEtaShift));
+    }
+  }
+}
+
+__device__ __forceinline__ void eta_cube(uint32_t* __restrict__ W, const uint64_t* thr, const uint64_t* etab,
+                                         uint32_t L, uint32_t ox, uint32_t oy, uint32_t oz, uint32_t rw) {
   const uint32_t x1 = ox + 1 == L ? 0 : ox + 1;
   const uint32_t y1 = oy + 1 == L ? 0 : oy + 1;
   const uint32_t z1 = oz + 1 == L ? 0 : oz + 1;
@@ -195,29 +239,7 @@ __device__ __forceinline__ void eta_cube(uint32_t* __restrict__ W, const uint64_
   }
 #pragma unroll
   for (int b = 0; b < 8; ++b) w[b] = W[idx[b]];
-  uint32_t D = 0, E = 0;
-#pragma unroll
-  for (int jj = 0; jj < kEdges; ++jj) {
-    const int A = edge_start(jj), ax = edge_axis(jj), B = A | (1 << ax);
-    D |= (uint32_t)(sigma_of(w[A], 2 * ax) == sigma_of(w[B], 2 * ax + 1)) << jj;
-    E |= eta_of(w[A], 2 * ax) << jj;
-  }
-  uint32_t prop, acc;
-  split_word(rw, 9, prop, acc);
-  const uint32_t M = sol_mask(prop);
-  const int dn = popc32(M & D) - popc32(E & D);
-  if ((uint64_t)acc < thr[dn + 4] && M != E) {
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
-      const uint32_t ax_arm = bx, ay_arm = 2 + by, az_arm = 4 + bz;
-      const uint32_t own = (1u << ax_arm) | (1u << ay_arm) | (1u << az_arm);
-      const uint32_t bits = (((M >> corner_edge(b, 0)) & 1u) << ax_arm) |
-                            (((M >> corner_edge(b, 1)) & 1u) << ay_arm) |
-                            (((M >> corner_edge(b, 2)) & 1u) << az_arm);
-      W[idx[b]] = (w[b] & ~(own << kEtaShift)) | (bits << kEtaShift);
-    }
-  }
+  eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t v) { W[idx[b]] = v; });
 }
 
 // One thread per quad of cubes of the family with origins
@@ -228,7 +250,11 @@ __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint6
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
   __shared__ uint64_t thr[9];
-  if (threadIdx.x < 9) thr[threadIdx.x] = pts[e.point].thr_eta[threadIdx.x];
+  __shared__ uint64_t etab[9];
+  if (threadIdx.x < 9) {
+    thr[threadIdx.x] = pts[e.point].thr_eta[threadIdx.x];
+    etab[threadIdx.x] = eta_table_entry(threadIdx.x);
+  }
   __syncthreads();
   const uint32_t L = e.L, hL = L / 2, qL = L / 4;
   const uint32_t t = e.start + threadIdx.x;
@@ -243,10 +269,10 @@ __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint6
   const uint32_t cube_ref = ox / 2 + hL * (oy / 2) + hL * hL * oz;  // gpu_water3D.cu:3405
   const U4 r = draw(seed_of(e), kStEta, cube_ref, step, 0);
   uint32_t* W = words + e.woff;
-  eta_cube(W, thr, L, ox, oy, oz, r.x);
-  eta_cube(W, thr, L, ox, oy + hL, oz, r.y);
-  eta_cube(W, thr, L, ox + hL, oy, oz, r.z);
-  eta_cube(W, thr, L, ox + hL, oy + hL, oz, r.w);
+  eta_cube(W, thr, etab, L, ox, oy, oz, r.x);
+  eta_cube(W, thr, etab, L, ox, oy + hL, oz, r.y);
+  eta_cube(W, thr, etab, L, ox + hL, oy, oz, r.z);
+  eta_cube(W, thr, etab, L, ox + hL, oy + hL, oz, r.w);
 }
 
 // --------------------------------------------------------- sigma half-sweep
@@ -378,8 +404,8 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
 
 // Cube update on LDS planes: Wp is the cube's origin plane (L x L words,
 // periodic in x and y), the top face is the next plane.
-__device__ __forceinline__ void eta_cube_plane(uint32_t* Wp, const uint64_t* thr, uint32_t L, uint32_t LL,
-                                               uint32_t ox, uint32_t oy, uint32_t rw) {
+__device__ __forceinline__ void eta_cube_plane(uint32_t* Wp, const uint64_t* thr, const uint64_t* etab,
+                                               uint32_t L, uint32_t LL, uint32_t ox, uint32_t oy, uint32_t rw) {
   const uint32_t x1 = ox + 1 == L ? 0 : ox + 1;
   const uint32_t y1 = oy + 1 == L ? 0 : oy + 1;
   uint32_t idx[8], w[8];
@@ -390,29 +416,7 @@ __device__ __forceinline__ void eta_cube_plane(uint32_t* Wp, const uint64_t* thr
   }
 #pragma unroll
   for (int b = 0; b < 8; ++b) w[b] = Wp[idx[b]];
-  uint32_t D = 0, E = 0;
-#pragma unroll
-  for (int jj = 0; jj < kEdges; ++jj) {
-    const int A = edge_start(jj), ax = edge_axis(jj), B = A | (1 << ax);
-    D |= (uint32_t)(sigma_of(w[A], 2 * ax) == sigma_of(w[B], 2 * ax + 1)) << jj;
-    E |= eta_of(w[A], 2 * ax) << jj;
-  }
-  uint32_t prop, acc;
-  split_word(rw, 9, prop, acc);
-  const uint32_t M = sol_mask(prop);
-  const int dn = popc32(M & D) - popc32(E & D);
-  if ((uint64_t)acc < thr[dn + 4] && M != E) {
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
-      const uint32_t ax_arm = bx, ay_arm = 2 + by, az_arm = 4 + bz;
-      const uint32_t own = (1u << ax_arm) | (1u << ay_arm) | (1u << az_arm);
-      const uint32_t bits = (((M >> corner_edge(b, 0)) & 1u) << ax_arm) |
-                            (((M >> corner_edge(b, 1)) & 1u) << ay_arm) |
-                            (((M >> corner_edge(b, 2)) & 1u) << az_arm);
-      Wp[idx[b]] = (w[b] & ~(own << kEtaShift)) | (bits << kEtaShift);
-    }
-  }
+  eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t v) { Wp[idx[b]] = v; });
 }
 
 // LDS form of sigma_update: cell word at `wc`, facing neighbour word `wn`.
@@ -516,6 +520,7 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   extern __shared__ uint32_t win[];
   __shared__ uint32_t thr_s[33];
   __shared__ uint64_t thr_e[9];
+  __shared__ uint64_t etab[9];
   __shared__ uint32_t sch[kChunk];
   __shared__ uint32_t gofs[kMaxWin];  // L^2 * global z of each window plane
   // the table is ordered by decreasing slab work and used in launch order,
@@ -525,7 +530,10 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const PointDev& P = pts[e.point];
   if (tid < 33) thr_s[tid] = sigma_thr32(P.thr_sigma[tid]);
-  if (tid < 9) thr_e[tid] = P.thr_eta[tid];
+  if (tid < 9) {
+    thr_e[tid] = P.thr_eta[tid];
+    etab[tid] = eta_table_entry(tid);
+  }
   const uint64_t step = *stepp + step_off;
   const uint64_t seed = seed_of(e);
   if (tid < nk) {
@@ -571,10 +579,10 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
         const uint32_t cube_ref = ii + hL * jj + (gofs[wz] >> 2);
         const U4 r = draw(seed, kStEta, cube_ref, step, 0);
         uint32_t* Wp = win + wz * LL;  // cube origin plane; the cube spans wz, wz+1
-        eta_cube_plane(Wp, thr_e, (uint32_t)L, (uint32_t)LL, ox, oy, r.x);
-        eta_cube_plane(Wp, thr_e, (uint32_t)L, (uint32_t)LL, ox, oy + hL, r.y);
-        eta_cube_plane(Wp, thr_e, (uint32_t)L, (uint32_t)LL, ox + hL, oy, r.z);
-        eta_cube_plane(Wp, thr_e, (uint32_t)L, (uint32_t)LL, ox + hL, oy + hL, r.w);
+        eta_cube_plane(Wp, thr_e, etab, (uint32_t)L, (uint32_t)LL, ox, oy, r.x);
+        eta_cube_plane(Wp, thr_e, etab, (uint32_t)L, (uint32_t)LL, ox, oy + hL, r.y);
+        eta_cube_plane(Wp, thr_e, etab, (uint32_t)L, (uint32_t)LL, ox + hL, oy, r.z);
+        eta_cube_plane(Wp, thr_e, etab, (uint32_t)L, (uint32_t)LL, ox + hL, oy + hL, r.w);
       }
       __syncthreads();
     }
