@@ -288,18 +288,35 @@ __device__ __forceinline__ uint32_t sigma_thr32(uint64_t t) {
   return t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
 }
 
-__device__ __forceinline__ void sigma_update(uint32_t* __restrict__ W, const uint32_t* thr, uint32_t c,
-                                             uint32_t nbc, int arm, uint32_t rw) {
-  const uint32_t w = W[c];
-  const uint32_t wn = W[nbc];
-  const uint32_t sa = sigma_of(w, arm);
-  uint32_t sp, ra;
-  split_word(rw, 6, sp, ra);
-  if (sp == sa) return;
-  const uint32_t snb = sigma_of(wn, arm ^ 1);
-  const int h = eta_of(w, arm) ? ((int)(sp == snb) - (int)(sa == snb)) : 0;
-  const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
-  if (ra < thr[(h + 1) * 11 + d + 5]) W[c] = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
+// The sigma rule for the four cells of a quad: words at W[cell[m]], facing
+// neighbours at W[nb[m]] (LDS or global memory).  The four cells and four
+// neighbours are eight distinct words and only the cells are written, so
+// all loads are issued first and the four updates are computed without
+// waiting on one another.
+template <class Ptr>
+__device__ __forceinline__ void sigma_quad(Ptr W, const uint32_t (&cell)[4], const uint32_t (&nb)[4],
+                                           const uint32_t* thr, int arm, U4 r) {
+  const uint32_t rw[4] = {r.x, r.y, r.z, r.w};
+  uint32_t w[4], wn[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    w[m] = W[cell[m]];
+    wn[m] = W[nb[m]];
+  }
+  uint32_t sp[4], ra[4], t[4];
+  bool act[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    split_word(rw[m], 6, sp[m], ra[m]);
+    const uint32_t sa = sigma_of(w[m], arm), snb = sigma_of(wn[m], arm ^ 1);
+    act[m] = sp[m] != sa;
+    const int h = eta_of(w[m], arm) ? ((int)(sp[m] == snb) - (int)(sa == snb)) : 0;
+    const int d = count_eq(w[m], sp[m]) - count_eq(w[m], sa) + 1;
+    t[m] = thr[(h + 1) * 11 + d + 5];
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+    if (act[m] && ra[m] < t[m]) W[cell[m]] = (w[m] & ~(7u << (3 * arm))) | (sp[m] << (3 * arm));
 }
 
 // Non-chessboard rule (gpu_water3D.cu:1966-1984, 2004-2007): with dN =
@@ -396,10 +413,7 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
     sigma_update_nc(W, thr, c[3], nbc[3], arm, r.w);
     return;
   }
-  sigma_update(W, thr, c[0], nbc[0], arm, r.x);
-  sigma_update(W, thr, c[1], nbc[1], arm, r.y);
-  sigma_update(W, thr, c[2], nbc[2], arm, r.z);
-  sigma_update(W, thr, c[3], nbc[3], arm, r.w);
+  sigma_quad(W, c, nbc, thr, arm, r);
 }
 
 // Cube update on LDS planes: Wp is the cube's origin plane (L x L words,
@@ -417,20 +431,6 @@ __device__ __forceinline__ void eta_cube_plane(uint32_t* Wp, const uint64_t* thr
 #pragma unroll
   for (int b = 0; b < 8; ++b) w[b] = Wp[idx[b]];
   eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t v) { Wp[idx[b]] = v; });
-}
-
-// LDS form of sigma_update: cell word at `wc`, facing neighbour word `wn`.
-__device__ __forceinline__ void sigma_update_lds(uint32_t* wc, uint32_t wn, const uint32_t* thr, int arm,
-                                                 uint32_t rw) {
-  const uint32_t w = *wc;
-  const uint32_t sa = sigma_of(w, arm);
-  uint32_t sp, ra;
-  split_word(rw, 6, sp, ra);
-  if (sp == sa) return;
-  const uint32_t snb = sigma_of(wn, arm ^ 1);
-  const int h = eta_of(w, arm) ? ((int)(sp == snb) - (int)(sa == snb)) : 0;
-  const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
-  if (ra < thr[(h + 1) * 11 + d + 5]) *wc = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
 }
 
 // in-plane (x or y arm) update of the quad with origin (x, y) on plane Wp
@@ -454,10 +454,8 @@ __device__ __forceinline__ void sigma_quad_plane(uint32_t* Wp, const uint32_t* t
   const uint32_t cell[4] = {b0, b0 + dy, b0 + dx, b0 + dx + dy};
   // the cells on the high side of the arm's axis: 2, 3 for x, 1, 3 for y
   const int off[4] = {off_lo, ax == 0 ? off_lo : off_hi, ax == 0 ? off_hi : off_lo, off_hi};
-  const uint32_t rw[4] = {r.x, r.y, r.z, r.w};
-#pragma unroll
-  for (int m = 0; m < 4; ++m)
-    sigma_update_lds(Wp + cell[m], Wp[(int)cell[m] + off[m]], thr, arm, rw[m]);
+  const uint32_t nb[4] = {cell[0] + off[0], cell[1] + off[1], cell[2] + off[2], cell[3] + off[3]};
+  sigma_quad(Wp, cell, nb, thr, arm, r);
 }
 
 // z-arm update of the quad with origin (x, y) on plane Wp: the neighbour is
@@ -466,13 +464,10 @@ __device__ __forceinline__ void sigma_quad_z(uint32_t* Wp, int LL, const uint32_
                                              uint32_t y, int arm, U4 r) {
   const uint32_t hL = L / 2;
   const int dz = (arm & 1) ? -LL : LL;
-  const uint32_t xs[4] = {x, x, x + hL, x + hL}, ys[4] = {y, y + hL, y, y + hL};
-  const uint32_t rw[4] = {r.x, r.y, r.z, r.w};
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    uint32_t* wc = Wp + xs[m] + L * ys[m];
-    sigma_update_lds(wc, wc[dz], thr, arm, rw[m]);
-  }
+  const uint32_t b0 = x + L * y, dy = hL * L;
+  const uint32_t cell[4] = {b0, b0 + dy, b0 + hL, b0 + hL + dy};
+  const uint32_t nb[4] = {cell[0] + dz, cell[1] + dz, cell[2] + dz, cell[3] + dz};
+  sigma_quad(Wp, cell, nb, thr, arm, r);
 }
 
 // ------------------------------------------------------ fused slab kernel
