@@ -288,34 +288,36 @@ __device__ __forceinline__ uint32_t sigma_thr32(uint64_t t) {
   return t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
 }
 
-// The sigma rule for the four cells of a quad: words at W[cell[m]], facing
-// neighbours at W[nb[m]] (LDS or global memory).  The four cells and four
-// neighbours are eight distinct words and only the cells are written, so
-// all loads are issued first and the four updates are computed without
-// waiting on one another.
-template <class Ptr>
-__device__ __forceinline__ void sigma_quad(Ptr W, const uint32_t (&cell)[4], const uint32_t (&nb)[4],
-                                           const uint32_t* thr, int arm, U4 r) {
-  const uint32_t rw[4] = {r.x, r.y, r.z, r.w};
-  uint32_t w[4], wn[4];
+// The sigma rule for the cells of K quads: words at W[cell[m]], facing
+// neighbours at W[nb[m]] (LDS or global memory); quads with ok[k] false are
+// skipped (their indices must still be in range).  The cells and their
+// neighbours are distinct words (active cells against the other parity)
+// and only the cells are written, so all loads are issued first and the
+// updates are computed without waiting on one another.
+template <int K, class Ptr>
+__device__ __forceinline__ void sigma_quads(Ptr W, const uint32_t (&cell)[4 * K], const uint32_t (&nb)[4 * K],
+                                            const uint32_t* thr, int arm, const U4 (&r)[K], const bool (&ok)[K]) {
+  uint32_t w[4 * K], wn[4 * K];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
+  for (int m = 0; m < 4 * K; ++m) {
     w[m] = W[cell[m]];
     wn[m] = W[nb[m]];
   }
-  uint32_t sp[4], ra[4], t[4];
-  bool act[4];
+  uint32_t sp[4 * K], ra[4 * K], t[4 * K];
+  bool act[4 * K];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    split_word(rw[m], 6, sp[m], ra[m]);
+  for (int m = 0; m < 4 * K; ++m) {
+    const U4& q = r[m / 4];
+    const uint32_t rw = (m & 3) == 0 ? q.x : (m & 3) == 1 ? q.y : (m & 3) == 2 ? q.z : q.w;
+    split_word(rw, 6, sp[m], ra[m]);
     const uint32_t sa = sigma_of(w[m], arm), snb = sigma_of(wn[m], arm ^ 1);
-    act[m] = sp[m] != sa;
+    act[m] = ok[m / 4] && sp[m] != sa;
     const int h = eta_of(w[m], arm) ? ((int)(sp[m] == snb) - (int)(sa == snb)) : 0;
     const int d = count_eq(w[m], sp[m]) - count_eq(w[m], sa) + 1;
     t[m] = thr[(h + 1) * 11 + d + 5];
   }
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
+  for (int m = 0; m < 4 * K; ++m)
     if (act[m] && ra[m] < t[m]) W[cell[m]] = (w[m] & ~(7u << (3 * arm))) | (sp[m] << (3 * arm));
 }
 
@@ -413,7 +415,9 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
     sigma_update_nc(W, thr, c[3], nbc[3], arm, r.w);
     return;
   }
-  sigma_quad(W, c, nbc, thr, arm, r);
+  const U4 rq[1] = {r};
+  const bool ok[1] = {true};
+  sigma_quads<1>(W, c, nbc, thr, arm, rq, ok);
 }
 
 // Cube update on LDS planes: Wp is the cube's origin plane (L x L words,
@@ -433,13 +437,13 @@ __device__ __forceinline__ void eta_cube_plane(uint32_t* Wp, const uint64_t* thr
   eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t v) { Wp[idx[b]] = v; });
 }
 
-// in-plane (x or y arm) update of the quad with origin (x, y) on plane Wp
-__device__ __forceinline__ void sigma_quad_plane(uint32_t* Wp, const uint32_t* thr, uint32_t L, uint32_t x,
-                                                 uint32_t y, int arm, int ax, U4 r) {
-  // cells (x, y), (x, y + L/2), (x + L/2, y), (x + L/2, y + L/2) with x, y <
-  // L/2: along the arm's axis only the coordinate at 0 (-1 step) or L-1 (+1
-  // step) wraps, i.e. the low cells when x (y) = 0 and the high cells when
-  // x (y) = L/2 - 1
+// Cells and facing neighbours (window indices) of the in-plane (x or y arm)
+// quad with origin (x, y) on the plane starting at word `base`: cells (x, y),
+// (x, y + L/2), (x + L/2, y), (x + L/2, y + L/2) with x, y < L/2.  Along the
+// arm's axis only the coordinate at 0 (-1 step) or L-1 (+1 step) wraps, i.e.
+// the low cells when x (y) = 0 and the high cells when x (y) = L/2 - 1.
+__device__ __forceinline__ void quad_plane_cells(uint32_t base, uint32_t L, uint32_t x, uint32_t y, int arm,
+                                                 int ax, uint32_t* cell, uint32_t* nb) {
   const int iL = (int)L, hL = iL / 2, cq = ax == 0 ? (int)x : (int)y, st = ax == 0 ? 1 : iL;
   const int wrap = (iL - 1) * st;
   int off_lo, off_hi;
@@ -450,24 +454,30 @@ __device__ __forceinline__ void sigma_quad_plane(uint32_t* Wp, const uint32_t* t
     off_lo = st;
     off_hi = cq == hL - 1 ? -wrap : st;
   }
-  const uint32_t b0 = x + L * y, dy = (uint32_t)hL * L, dx = (uint32_t)hL;
-  const uint32_t cell[4] = {b0, b0 + dy, b0 + dx, b0 + dx + dy};
+  const uint32_t b0 = base + x + L * y, dy = (uint32_t)hL * L, dx = (uint32_t)hL;
+  cell[0] = b0;
+  cell[1] = b0 + dy;
+  cell[2] = b0 + dx;
+  cell[3] = b0 + dx + dy;
   // the cells on the high side of the arm's axis: 2, 3 for x, 1, 3 for y
-  const int off[4] = {off_lo, ax == 0 ? off_lo : off_hi, ax == 0 ? off_hi : off_lo, off_hi};
-  const uint32_t nb[4] = {cell[0] + off[0], cell[1] + off[1], cell[2] + off[2], cell[3] + off[3]};
-  sigma_quad(Wp, cell, nb, thr, arm, r);
+  nb[0] = cell[0] + off_lo;
+  nb[1] = cell[1] + (ax == 0 ? off_lo : off_hi);
+  nb[2] = cell[2] + (ax == 0 ? off_hi : off_lo);
+  nb[3] = cell[3] + off_hi;
 }
 
-// z-arm update of the quad with origin (x, y) on plane Wp: the neighbour is
-// on the plane above (arm 4) or below (arm 5)
-__device__ __forceinline__ void sigma_quad_z(uint32_t* Wp, int LL, const uint32_t* thr, uint32_t L, uint32_t x,
-                                             uint32_t y, int arm, U4 r) {
+// z-arm quad with origin (x, y) on the plane starting at `base`: the
+// neighbour is on the plane above (arm 4) or below (arm 5)
+__device__ __forceinline__ void quad_z_cells(uint32_t base, uint32_t L, int LL, uint32_t x, uint32_t y, int arm,
+                                             uint32_t* cell, uint32_t* nb) {
   const uint32_t hL = L / 2;
   const int dz = (arm & 1) ? -LL : LL;
-  const uint32_t b0 = x + L * y, dy = hL * L;
-  const uint32_t cell[4] = {b0, b0 + dy, b0 + hL, b0 + hL + dy};
-  const uint32_t nb[4] = {cell[0] + dz, cell[1] + dz, cell[2] + dz, cell[3] + dz};
-  sigma_quad(Wp, cell, nb, thr, arm, r);
+  const uint32_t b0 = base + x + L * y, dy = hL * L;
+  cell[0] = b0;
+  cell[1] = b0 + dy;
+  cell[2] = b0 + hL;
+  cell[3] = b0 + hL + dy;
+  for (int m = 0; m < 4; ++m) nb[m] = cell[m] + dz;
 }
 
 // ------------------------------------------------------ fused slab kernel
@@ -605,8 +615,11 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
           y = 2 * hy + q;
         }
         const uint32_t wz = wz0 + it.pi;
-        sigma_quad_plane(win + wz * LL, thr_s, (uint32_t)L, x, y, arm, ax,
-                         draw(seed, kStSigma, x + L * y + gofs[wz], step, sub));
+        uint32_t cell[4], nb[4];
+        quad_plane_cells(wz * LL, (uint32_t)L, x, y, arm, ax, cell, nb);
+        const U4 r[1] = {draw(seed, kStSigma, x + L * y + gofs[wz], step, sub)};
+        const bool ok[1] = {true};
+        sigma_quads<1>(win, cell, nb, thr_s, arm, r, ok);
       }
     } else {
       // planes whose global z has the kept parity
@@ -617,8 +630,11 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
       for (PlaneIter it = plane_first(tid, Q, dP4); it.pi < (uint32_t)nplanes; plane_next(it, Q, dp, du)) {
         const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
         const uint32_t wz = wz0 + 2 * it.pi;
-        sigma_quad_z(win + wz * LL, (int)LL, thr_s, (uint32_t)L, x, y, arm,
-                     draw(seed, kStSigma, x + L * y + gofs[wz], step, sub));
+        uint32_t cell[4], nb[4];
+        quad_z_cells(wz * LL, (uint32_t)L, (int)LL, x, y, arm, cell, nb);
+        const U4 r[1] = {draw(seed, kStSigma, x + L * y + gofs[wz], step, sub)};
+        const bool ok[1] = {true};
+        sigma_quads<1>(win, cell, nb, thr_s, arm, r, ok);
       }
     }
     __syncthreads();
