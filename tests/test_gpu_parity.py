@@ -181,3 +181,50 @@ def test_batch_mixing_chessboard_modes_equals_single_runs():
         a, b = batch.get_config(i), sim.get_config()
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
         assert batch.observe(i).nhb2 == sim.observe().nhb2, i
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("L,nm,nc,steps", [(4, 1, 1, 40), (8, 2, 0, 30), (12, 3, 1, 20), (64, 1, 0, 3)])
+def test_fused_and_unfused_paths_edge_cases(fused, L, nm, nc, steps):
+    """Both Metropolis paths against the oracle at the edges of the fused
+    kernel: L=4 (window planes repeat around the lattice), several sweeps per
+    step (2 nm fused launches), and L=64 (one-plane slabs, the largest L
+    whose window fits in LDS)."""
+    kw = dict(L=L, T=0.65, P=0.3, seed=4242, n_metropolis=nm, n_cluster=nc)
+    eng = Simulation(Params(volume_mode=VOLUME_DEVICE, **kw))
+    eng.tuning = {"fused": fused}
+    ora = O.OracleSim(**kw)
+    for t in range(steps):
+        eng.step(1)
+        ora.step(1)
+        assert_same_state(eng, ora, f"step {t}")
+        if nc:
+            assert np.array_equal(eng.labels(), ora.labels()), f"labels step {t}"
+    assert_same_obs(eng, ora, "end")
+
+
+def test_lattice_above_the_fused_limit_runs_unfused():
+    """L=68: one plane plus the worst-case halo exceeds the LDS, so the
+    handle runs the unfused path even when the fused one is requested."""
+    kw = dict(L=68, T=0.7, P=0.45, seed=99)
+    eng = Simulation(Params(volume_mode=VOLUME_DEVICE, **kw))
+    eng.tuning = {"fused": 1}
+    ora = O.OracleSim(**kw)
+    for t in range(2):
+        eng.step(1)
+        ora.step(1)
+        assert_same_state(eng, ora, f"step {t}")
+    assert eng.volume() == ora.volume()
+
+
+def test_many_point_batch_equals_single_runs():
+    pts = [Params(L=(8, 12)[i % 2], T=0.5 + 0.01 * i, P=0.1 * (i % 5), seed=77 + i, volume_mode=VOLUME_DEVICE)
+           for i in range(40)]
+    batch = Simulation(pts)
+    batch.step(25)
+    for i in (0, 1, 17, 38, 39):
+        single = Simulation(pts[i])
+        single.step(25)
+        a, b = batch.get_config(i), single.get_config()
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
+        assert batch.volume(i) == single.volume()
