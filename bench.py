@@ -205,6 +205,21 @@ def precision_form(steps: int = 5000, interval: int = 10, ev: HipEvents | None =
     return out
 
 
+def cluster_form(steps: int, warmup: int, ev: HipEvents) -> dict:
+    """Cluster form: one state point, L = 32, T = 0.70, P = 0.45, 1 Metropolis
+    sweep + 1 Swendsen-Wang step + 1 volume move per step (FP64, host volume
+    move), labels kept canonical every step."""
+    from cvfbulkwater_amd import FP64, VOLUME_HOST, Params, Simulation
+
+    sim = Simulation(Params(L=32, T=0.70, P=0.45, seed=7, n_cluster=1, precision=FP64,
+                            volume_mode=VOLUME_HOST))
+    ms = time_steps(sim, steps, warmup, ev)
+    sim.close()
+    return {"workload": "cluster form: L=32, T=0.70, P=0.45, 1 Metropolis + 1 Swendsen-Wang + 1 volume "
+                        "move per step, FP64, host volume move", "ms_per_step": ms,
+            "value": 32 ** 3 / (ms * 1e-3)}
+
+
 def plain_per_L(steps: int, warmup: int, ev: HipEvents) -> dict:
     """The batch run as 128 sequential plain drop-in runs (one state point
     per handle, FP64, host volume move): each L's plain step time times its
@@ -237,7 +252,7 @@ def main() -> int:
                          "(skips the autotuner)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-steps", type=int, default=5000)
-    ap.add_argument("--only", choices=["batch", "plain", "precision"], default=None,
+    ap.add_argument("--only", choices=["batch", "plain", "precision", "cluster"], default=None,
                     help="run one leg only (for profiling)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (calibrates the "
@@ -287,7 +302,10 @@ def main() -> int:
         pms = time_steps(sim, args.plain_steps, args.warmup, ev)
         sim.close()
         out.update(plain_ms=pms, plain_rate=32 ** 3 / (pms * 1e-3), plain_tuning=ptuning,
-                   seq=plain_per_L(max(args.plain_steps // 4, 100), args.warmup, ev))
+                   seq=plain_per_L(max(args.plain_steps // 4, 100), args.warmup, ev),
+                   cluster=cluster_form(max(args.plain_steps // 4, 100), args.warmup, ev))
+    if args.only == "cluster":
+        out["cluster_only"] = cluster_form(args.plain_steps, args.warmup, ev)
     if args.only in (None, "precision"):
         out["precision"] = precision_form(args.precision_steps, 10, ev)
 
@@ -314,6 +332,7 @@ def main() -> int:
                                      "volume move", "value": out["plain_rate"],
                          "ms_per_step": out["plain_ms"], "steps": args.plain_steps,
                          "tuning": out["plain_tuning"]}
+        line["cluster_form"] = out["cluster"]
         line["plain_sequential_128"] = {
             "workload": "the 128 batch state points as sequential plain runs (FP64, host volume "
                         "move), from the per-L plain step times", **out["seq"]}
@@ -322,6 +341,9 @@ def main() -> int:
             line["batched_over_sequential_plain"] = out["batch_rate"] / out["seq"]["value"]
         else:
             line.update(value=out["plain_rate"], ms_per_step=out["plain_ms"])
+    if "cluster_only" in out:
+        line.update(value=out["cluster_only"]["value"], ms_per_step=out["cluster_only"]["ms_per_step"],
+                    cluster_form=out["cluster_only"])
     if "precision" in out:
         line["precision_form"] = out["precision"]
         if "value" not in line:

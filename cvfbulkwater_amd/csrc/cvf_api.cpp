@@ -383,13 +383,28 @@ uint32_t slab_lds_bytes(uint32_t L, uint32_t T) { return (T + kChunk + 2) * L * 
 
 // Planes per slab: the requested count if it divides L and fits, else the
 // largest divisor of L up to 8 that fits in LDS; 0 if none does.
-uint32_t choose_slab(uint32_t L, int requested) {
+uint32_t choose_slab(uint32_t L, int requested, uint32_t cap = kMaxSlab) {
   if (requested > 0 && L % (uint32_t)requested == 0 &&
       slab_lds_bytes(L, (uint32_t)requested) <= chunk_lds_limit())
     return (uint32_t)requested;
-  for (uint32_t T = kMaxSlab; T >= 1; --T)
+  for (uint32_t T = cap; T >= 1; --T)
     if (L % T == 0 && slab_lds_bytes(L, T) <= chunk_lds_limit()) return T;
   return 0;
+}
+
+// Automatic slab thickness cap: thick slabs (less halo work) while the
+// handle still has at least one slab per CU (256 on MI355X); small handles
+// (one L=32 point) get one-plane slabs so that more CUs work.
+uint32_t auto_slab_cap(const cvf_sim* s) {
+  for (uint32_t cap = kMaxSlab; cap > 1; cap /= 2) {
+    uint64_t slabs = 0;
+    for (const auto& P : s->ph) {
+      const uint32_t T = choose_slab(P.L, 0, cap);
+      if (T) slabs += P.L / T;
+    }
+    if (slabs >= 256) return cap;
+  }
+  return 1;
 }
 
 // Block table over `count(P)` work items per point of `points`, `block`
@@ -461,6 +476,7 @@ int upload_group_steps(cvf_sim* s) {
 // largest-first onto the group with the fewest cells.
 int build_groups(cvf_sim* s) {
   HIPC(hipDeviceSynchronize());
+  const uint32_t slab_cap = auto_slab_cap(s);
   s->groups.clear();
   const int ng = std::max(1, std::min(s->tune.streams, s->npts));
   for (int g = 0; g < ng; ++g) s->groups.emplace_back(new Group);
@@ -498,7 +514,7 @@ int build_groups(cvf_sim* s) {
     g->sigma_cells = 0;
     for (int p : g->points) {
       const PointHost& P = s->ph[p];
-      const uint32_t T = choose_slab(P.L, s->tune.slab_planes);
+      const uint32_t T = choose_slab(P.L, s->tune.slab_planes, slab_cap);
       if (!T || !P.prm.flag_chessboard) {
         g->fused = false;
         break;
@@ -532,7 +548,7 @@ int build_groups(cvf_sim* s) {
   if (rc0) return rc0;
   std::vector<int> all(s->npts);
   for (int i = 0; i < s->npts; ++i) all[i] = i;
-  const auto ta = make_table(s->ph, all, 256, cells_all);
+  const auto ta = make_table(s->ph, all, kObserveSpan, cells_all);
   int rc = upload_vec(s->tab_cell_all, ta);
   if (rc) return rc;
   s->nb_cell_all = (uint32_t)ta.size();

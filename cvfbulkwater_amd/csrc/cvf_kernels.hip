@@ -655,18 +655,23 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
     p[i] = 0;
 }
 
-__global__ void k_observe(const BlockEnt* __restrict__ tab, uint32_t nb,
-                          const uint32_t* __restrict__ words,
-                          unsigned long long* __restrict__ counters) {
+// One workgroup of 256 threads per kObserveSpan cells of a point: each
+// thread sums kObserveSpan / 256 cells, the workgroup reduces through LDS,
+// and one pair of 64-bit atomics per workgroup adds to the point's totals
+// (one pair per wavefront serialised on the same address at large L).
+__global__ __launch_bounds__(256) void k_observe(const BlockEnt* __restrict__ tab, uint32_t nb,
+                                                const uint32_t* __restrict__ words,
+                                                unsigned long long* __restrict__ counters) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
   const uint32_t L = e.L;
-  const uint32_t t = e.start + threadIdx.x;
+  const uint32_t* W = words + e.woff;
   uint32_t nhb2 = 0, nsig = 0;
-  if (t < e.N) {
+  for (uint32_t k = 0; k < kObserveSpan / 256; ++k) {
+    const uint32_t t = e.start + k * 256 + threadIdx.x;
+    if (t >= e.N) break;
     uint32_t x, y, z;
     cell_coords(e, t, x, y, z);
-    const uint32_t* W = words + e.woff;
     const uint32_t w = W[t];
     const uint32_t nbs[3] = {neighbor_up(t, x, 1, L), neighbor_up(t, y, L, L),
                              neighbor_up(t, z, L * L, L)};
@@ -679,16 +684,30 @@ __global__ void k_observe(const BlockEnt* __restrict__ tab, uint32_t nb,
     uint32_t pairs = 0;
 #pragma unroll
     for (int a = 0; a < 6; ++a) pairs += (uint32_t)count_eq(w, sigma_of(w, a)) - 1;
-    nsig = pairs / 2;
+    nsig += pairs / 2;
   }
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
     nhb2 += __shfl_xor(nhb2, m, 64);
     nsig += __shfl_xor(nsig, m, 64);
   }
-  if ((threadIdx.x & 63) == 0 && (nhb2 | nsig)) {
-    atomicAdd(&counters[2 * e.point], (unsigned long long)nhb2);
-    atomicAdd(&counters[2 * e.point + 1], (unsigned long long)nsig);
+  __shared__ uint32_t part[2][4];
+  const uint32_t wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    part[0][wv] = nhb2;
+    part[1][wv] = nsig;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0, b = 0;
+    for (int i = 0; i < 4; ++i) {
+      a += part[0][i];
+      b += part[1][i];
+    }
+    if (a | b) {
+      atomicAdd(&counters[2 * e.point], a);
+      atomicAdd(&counters[2 * e.point + 1], b);
+    }
   }
 }
 

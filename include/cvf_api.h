@@ -142,7 +142,7 @@ typedef struct cvf_tuning {
   int32_t graph_steps;  /* MC steps per replayed hipGraph (1..64) */
   int32_t fused;        /* 1: eta sweep + 6 half-sweeps per kernel on LDS-staged z-slabs
                            where the slab fits in LDS; 0: one kernel per half-sweep */
-  int32_t slab_planes;  /* planes per slab of the fused kernel (<= 8); 0 = largest that fits */
+  int32_t slab_planes;  /* planes per slab of the fused kernel (<= 8); 0 = automatic */
   int32_t block_chunk;  /* workgroup size of the fused kernel */
 } cvf_tuning;
 
