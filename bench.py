@@ -161,6 +161,24 @@ def measured_traffic(kname: str, tuning: dict):
     return None
 
 
+def measured_valu(kname: str, tuning: dict):
+    """VALU wave instructions (all, and 64-bit integer ones) per dispatch of
+    the roofline kernel under this launch configuration, from the SQ counter
+    summary kept in profiles/ (lines "valu kernel=... tuning=... insts=...
+    int64=..."), or None."""
+    path = ROOT / "profiles" / "k_chunk_batch128_sq_counters.txt"
+    if not path.exists():
+        return None
+    sig = tuning_signature(kname, tuning)
+    for line in path.read_text().splitlines():
+        if not line.startswith("valu "):
+            continue
+        kv = dict(f.split("=", 1) for f in line.split()[1:])
+        if kv.get("kernel") == kname and kv.get("tuning") == sig:
+            return float(kv["insts"]), float(kv["int64"])
+    return None
+
+
 def precision_form(steps: int = 5000, interval: int = 10, ev: HipEvents | None = None) -> dict:
     """Mixed-precision large lattice: L = 96, T = 0.65, P = 0.30, 1 Metropolis
     + 1 Swendsen-Wang + 1 volume move per step, run once with FP32-compensated
@@ -327,6 +345,19 @@ def main() -> int:
                             "traffic": measured_traffic(out["kname"], out["tuning"]),
                             "bytes_per_launch": out["sig_bytes"], "launch_ms": out["sig_ms"]}
         line["kernel_ms"] = {k: v[0] for k, v in out["prof"].items() if v[1]}
+        valu = measured_valu(out["kname"], out["tuning"])
+        if valu is not None:
+            # the kernel is bound by integer VALU issue, not HBM: its wave64
+            # instructions per MC step against the chip's issue peak
+            # (256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction)
+            per_step = out["prof"][{"k_chunk": "chunk", "k_sigma": "sigma"}[out["kname"]]][1] / args.profile_steps
+            peak = 256 * 4 * 2.4e9 / 4
+            rate = valu[0] * per_step / (out["batch_ms"] * 1e-3)
+            line["valu_issue"] = {"kernel": out["kname"], "insts_per_step": valu[0] * per_step,
+                                  "achieved": rate / 1e9, "peak": peak / 1e9, "unit": "G wave-instr/s",
+                                  "frac": rate / peak,
+                                  "frac_int64_quarter_rate": (valu[0] + 3 * valu[1]) * per_step /
+                                                             (out["batch_ms"] * 1e-3) / peak}
     if "plain_rate" in out:
         line["plain"] = {"workload": "plain drop-in form: L=32, T=0.815, P=0.45, FP64 vdW, host "
                                      "volume move", "value": out["plain_rate"],
