@@ -125,6 +125,7 @@ struct cvf_sim {
   DevBuf<unsigned long long> counters;
   DevBuf<uint64_t> cur_steps;          // one slot per group
   DevBuf<BlockEnt> tab_cell_all;       // every point's cells, for observables
+  DevBuf<unsigned long long> hist;     // sigma histogram of one point (6 counters)
   uint32_t nb_cell_all = 0;
   std::vector<std::unique_ptr<Group>> groups;
   uint32_t max_slots = 0;
@@ -994,12 +995,11 @@ int cvf_sigma_histogram(cvf_sim* s, int32_t pt, int64_t counts6[6]) {
   if (!counts6) return fail(CVF_EINVAL, "null output");
   HIPC(hipSetDevice(s->device));
   const PointHost& P = s->ph[pt];
-  DevBuf<unsigned long long> h;
-  HIPC(h.alloc(6));
-  launch_zero_u32(s->stream, (uint32_t*)h.p, 12);
-  launch_sigma_hist(s->stream, s->words.p + P.woff, P.N, h.p);
+  if (!s->hist.p) HIPC(s->hist.alloc(6));  // kept for the handle's life: called every step
+  launch_zero_u32(s->stream, (uint32_t*)s->hist.p, 12);
+  launch_sigma_hist(s->stream, s->words.p + P.woff, P.N, s->hist.p);
   unsigned long long out[6];
-  HIPC(hipMemcpyAsync(out, h.p, sizeof out, hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipMemcpyAsync(out, s->hist.p, sizeof out, hipMemcpyDeviceToHost, s->stream));
   HIPC(hipStreamSynchronize(s->stream));
   for (int i = 0; i < 6; ++i) counts6[i] = (int64_t)out[i];
   return CVF_OK;

@@ -814,29 +814,30 @@ __global__ void k_sw_local(const BlockEnt* __restrict__ tab, uint32_t nb,
       act = s0 != s1 && (uint64_t)rw[ax] < thb;
     cmask |= (uint32_t)act << ax;
   }
-  // intra-cell union-find over 6 arms, minimum-arm roots
-  uint32_t lab[6] = {0, 1, 2, 3, 4, 5};
+  // intra-cell components of the 6 arms as bitmasks (Warshall closure in
+  // registers); each arm's root is the lowest arm of its component
+  uint32_t reach[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) reach[a] = 1u << a;
   int link = 3;
 #pragma unroll
   for (int ii = 0; ii < 5; ++ii)
 #pragma unroll
     for (int jj = ii + 1; jj < 6; ++jj) {
       if (sigma_of(w, ii) == sigma_of(w, jj) && (uint64_t)rw[link] < ts) {
-        uint32_t ra = ii, rb = jj;
-        while (lab[ra] != ra) ra = lab[ra];
-        while (lab[rb] != rb) rb = lab[rb];
-        if (ra < rb) lab[rb] = ra;
-        else if (rb < ra) lab[ra] = rb;
+        reach[ii] |= 1u << jj;
+        reach[jj] |= 1u << ii;
       }
       ++link;
     }
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+      if ((reach[a] >> k) & 1u) reach[a] |= reach[k];
   uint32_t* par = parent + e.loff;
 #pragma unroll
-  for (int a = 0; a < 6; ++a) {
-    uint32_t r = a;
-    while (lab[r] != r) r = lab[r];
-    par[a * N + t] = r * N + t;
-  }
+  for (int a = 0; a < 6; ++a) par[a * N + t] = (uint32_t)__builtin_ctz(reach[a]) * N + t;
   cov[e.woff + t] = (uint8_t)cmask;
 }
 
