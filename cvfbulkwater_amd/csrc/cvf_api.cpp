@@ -85,16 +85,20 @@ struct Group {
   bool fused = false;        // eta + sigma through the fused slab kernel
   uint32_t chunk_lds = 0;    // dynamic LDS bytes of a fused launch
   size_t sigma_cells = 0;    // cells swept per fused launch
-  hipGraph_t graph = nullptr, graphK = nullptr;
-  hipGraphExec_t gexec = nullptr, gexecK = nullptr;
+  // captured step sequences, longest first: graph_steps, then the powers of
+  // two below it, so any step count is a few graph launches
+  std::vector<int> gsteps;
+  std::vector<hipGraph_t> graphs;
+  std::vector<hipGraphExec_t> gexecs;
   size_t cells = 0;
   void drop_graphs() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (gexecK) (void)hipGraphExecDestroy(gexecK);
-    if (graphK) (void)hipGraphDestroy(graphK);
-    gexec = gexecK = nullptr;
-    graph = graphK = nullptr;
+    for (auto e : gexecs)
+      if (e) (void)hipGraphExecDestroy(e);
+    for (auto g : graphs)
+      if (g) (void)hipGraphDestroy(g);
+    gsteps.clear();
+    graphs.clear();
+    gexecs.clear();
   }
   ~Group() {
     drop_graphs();
@@ -626,11 +630,20 @@ int capture(cvf_sim* s, int gi, int nsteps, hipGraph_t* g, hipGraphExec_t* ge) {
 int capture_step_graphs(cvf_sim* s) {
   for (size_t gi = 0; gi < s->groups.size(); ++gi) {
     Group& g = *s->groups[gi];
-    if (g.gexec && g.gexecK) continue;
+    if (!g.gexecs.empty()) continue;
     g.drop_graphs();
-    int rc = capture(s, (int)gi, 1, &g.graph, &g.gexec);
-    if (!rc && s->tune.graph_steps > 1) rc = capture(s, (int)gi, s->tune.graph_steps, &g.graphK, &g.gexecK);
-    if (rc) return rc;
+    std::vector<int> sizes = {s->tune.graph_steps};
+    for (int p = 1; p < s->tune.graph_steps; p *= 2) sizes.push_back(p);
+    std::sort(sizes.rbegin(), sizes.rend());
+    for (int k : sizes) {
+      hipGraph_t gr = nullptr;
+      hipGraphExec_t ge = nullptr;
+      const int rc = capture(s, (int)gi, k, &gr, &ge);
+      g.gsteps.push_back(k);
+      g.graphs.push_back(gr);
+      g.gexecs.push_back(ge);
+      if (rc) return rc;
+    }
   }
   return CVF_OK;
 }
@@ -904,11 +917,12 @@ int cvf_step(cvf_sim* s, int64_t n_steps) {
   int rc = capture_step_graphs(s);
   if (rc) return rc;
   if ((rc = fork_groups(s))) return rc;
-  const int64_t K = s->tune.graph_steps;
+  const std::vector<int>& sizes = s->groups[0]->gsteps;  // the same for every group
   for (int64_t i = 0; i < n_steps;) {
-    const bool bulk = K > 1 && n_steps - i >= K;
-    for (auto& g : s->groups) HIPC(hipGraphLaunch(bulk ? g->gexecK : g->gexec, g->stream));
-    const int64_t k = bulk ? K : 1;
+    size_t c = 0;
+    while (sizes[c] > n_steps - i) ++c;  // the last size is 1
+    for (auto& g : s->groups) HIPC(hipGraphLaunch(g->gexecs[c], g->stream));
+    const int64_t k = sizes[c];
     // the volume chain does not read the lattice: in host-volume mode the
     // host advances it while the device runs the lattice steps
     for (int64_t j = 0; j < k; ++j) {
