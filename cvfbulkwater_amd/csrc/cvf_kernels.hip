@@ -14,6 +14,8 @@
 // contiguous range of the table (each XCD has its own L2).
 #include "cvf_kernels.h"
 
+#include <atomic>
+
 #include <hip/hip_ext.h>
 
 namespace cvf {
@@ -1000,11 +1002,16 @@ uint32_t chunk_lds_limit() {
 }
 void launch_chunk(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* src, uint32_t* dst,
                   uint32_t k0, uint32_t nk, int with_eta, uint32_t lds_bytes) {
-  static bool attr = false;
-  if (!attr) {
+  // the dynamic-LDS limit is a per-device function attribute: set it once
+  // on each device the library launches on
+  static std::atomic<uint64_t> attr_set{0};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const uint64_t bit = 1ull << (dev & 63);
+  if (!(attr_set.load(std::memory_order_acquire) & bit)) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chunk),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)chunk_lds_limit());
-    attr = true;
+    attr_set.fetch_or(bit, std::memory_order_acq_rel);
   }
   if (g_t0)
     hipExtLaunchKernelGGL(k_chunk, dim3(ln.nblocks), dim3(ln.block), lds_bytes, st, g_t0, g_t1, 0,
