@@ -25,13 +25,14 @@ __all__ = [
     "CVFError", "Params", "Obs", "ClusterStats", "Simulation", "load_library",
     "FP64", "FP32_COMPENSATED", "VOLUME_HOST", "VOLUME_DEVICE", "vdw_energy",
     "LIB_PATH", "EXE_PATH", "THERMO_EXE_PATH", "Correlator", "KT_NOT_VALUE", "thermo_averages",
-    "thermo_derivative", "thermo_kt",
+    "thermo_derivative", "thermo_kt", "REWEIGHT_EXE_PATH",
 ]
 
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libcvf.so"
 EXE_PATH = PKG / "cvf_water3d"
 THERMO_EXE_PATH = PKG / "cvf_thermo"
+REWEIGHT_EXE_PATH = PKG / "cvf_reweight"
 
 FP64, FP32_COMPENSATED = 0, 1
 VOLUME_HOST, VOLUME_DEVICE = 0, 1

@@ -3,6 +3,7 @@
     libcvf.so      HIP kernels + C ABI (include/cvf_api.h), gfx950
     cvf_water3d    drop-in executable (input_data/input_config -> data.out ...)
     cvf_thermo     drop-in for the thermodynamic analysis scripts (data.out -> isobars)
+    cvf_reweight   drop-in for the histogram reweighting program (data_L* -> Binder, Confront, ...)
 
 Run ``python -m cvfbulkwater_amd.build`` or call :func:`build`.
 """
@@ -34,9 +35,11 @@ COMMON = [
     "-Wall", "-Wno-unused-function",
 ]
 
-LIB_SOURCES = ["cvf_kernels.hip", "cvf_api.cpp", "cvf_host.cpp", "cvf_corr.cpp", "cvf_thermo.cpp"]
+LIB_SOURCES = ["cvf_kernels.hip", "cvf_rw_kernels.hip", "cvf_api.cpp", "cvf_host.cpp", "cvf_corr.cpp",
+               "cvf_thermo.cpp", "cvf_reweight.cpp"]
 # executable -> sources (each linked against libcvf.so)
-EXES = {"cvf_water3d": ["cvf_water3d.cpp"], "cvf_thermo": ["cvf_thermo_main.cpp"]}
+EXES = {"cvf_water3d": ["cvf_water3d.cpp"], "cvf_thermo": ["cvf_thermo_main.cpp"],
+        "cvf_reweight": ["cvf_reweight_main.cpp"]}
 
 
 def _run(cmd: list[str]) -> None:

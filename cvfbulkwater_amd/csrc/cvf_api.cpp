@@ -109,6 +109,18 @@ struct Group {
 
 }  // namespace
 
+namespace cvf {
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+}  // namespace cvf
+
 struct cvf_sim {
   int device = 0;
   hipStream_t stream = nullptr;  // main stream: uploads, observables, statistics

@@ -28,4 +28,7 @@ struct Thresholds {
 // with Jeff = J_hb - P*v_hb evaluated in float (391) and beta = 1/(double)T.
 Thresholds make_thresholds(float P, float T, float J_hb, float J_sig, float v_hb);
 
+// Sets the message cvf_last_error() returns (printf format) and returns code.
+int set_error(int code, const char* fmt, ...);
+
 }  // namespace cvf

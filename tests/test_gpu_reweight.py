@@ -1,0 +1,222 @@
+"""Parity of the engine's histogram reweighting (HIP kernels + host code in
+libcvf.so, the cvf_reweight executable) with the oracle restatement of
+HistogramReweighting/histogram_reweighting_mixed_paramSI_norm_x.f90.
+
+FP64 throughout; the device sums run in a different order than the
+Fortran loops (tree reductions over cells), so values agree to rounding:
+the tolerances below are 1e-9 relative (constants: 1e-9 absolute).
+Geometry (bin counts, edges, widths, occupied cells) is bit-exact.
+"""
+from __future__ import annotations
+
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from cvfbulkwater_amd import REWEIGHT_EXE_PATH
+from cvfbulkwater_amd.reweight import Reweighting, from_series
+from oracle import rw
+import rw_synth
+
+pytestmark = pytest.mark.gpu
+
+SL = lambda x: float(np.float32(x))  # noqa: E731
+TS = [0.1058, 0.1060, 0.1062, 0.1064, 0.1066, 0.1068]
+RTOL = 1e-9
+
+
+def si_T(t):
+    return t * SL(140.5678) + SL(185.4676)
+
+
+def si_P(p):
+    return p * SL(4694.59) + SL(-2111.5552)
+
+
+@pytest.fixture(scope="module")
+def pair():
+    data = rw_synth.samples(TS, 3000)
+    print("reweight case: L=8, 6 simulations x 3000 samples, bin factor 1.0", flush=True)
+    ora = rw.OraGrid.from_samples(8, TS, [0.5] * 6, [len(d) for d in data], np.concatenate(data), 1.0)
+    eng = Reweighting(8, TS, [0.5] * 6, data, 1.0)
+    c_ora, prev_ora, rc = ora.constants(300)
+    assert rc == 0
+    yield ora, eng, c_ora, prev_ora, data
+    eng.close()
+    ora.close()
+
+
+def test_geometry_matches_oracle(pair):
+    ora, eng, *_ = pair
+    i = eng.info()
+    g = ora.g
+    assert (i["n_bin"], i["n_bin2d"], i["n_bin_mixed"], i["N"]) == (g.n_bin, g.n_bin2d, g.n_bin_mixed, g.N)
+    assert (i["e_min"], i["rho_min"], i["de"], i["drho"]) == (g.Emin, g.Dmin, g.dE, g.dD)
+    assert i["n_occupied"] == int(np.count_nonzero(ora.datos().sum(axis=0)))
+    T, P = eng.sim_points()
+    assert np.array_equal(T, ora.T()) and np.array_equal(P, ora.P())
+
+
+def test_constants_match_oracle(pair):
+    ora, eng, c_ora, prev_ora, _ = pair
+    c, prev, ok = eng.solve_constants(300)
+    assert ok
+    np.testing.assert_allclose(c, c_ora, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(prev, prev_ora, rtol=0, atol=1e-9)
+    _, _, ok2 = eng.solve_constants(2)
+    assert not ok2
+    assert ora.constants(2)[2] == 1
+    eng.set_constants(c_ora)
+
+
+def targets():
+    T = [si_T(0.1058), si_T(0.1063), si_T(0.10645), si_T(0.1068), si_T(0.1061)]
+    P = [si_P(0.5), si_P(0.5), si_P(0.502), si_P(0.49), si_P(0.5)]
+    return T, P
+
+
+def test_batched_densities_match_oracle(pair):
+    ora, eng, c_ora, *_ = pair
+    eng.set_constants(c_ora)
+    T, P = targets()
+    lz = eng.reweight(T, P)
+    for t in range(len(T)):
+        _, _, lz_ora = ora.calcula(c_ora, T[t], P[t])
+        assert lz[t] == pytest.approx(lz_ora, rel=0, abs=1e-9)
+        d_ora = ora.distribution(c_ora, T[t], P[t])
+        d = eng.density(t)
+        assert np.array_equal(d > 0, d_ora > 0)
+        np.testing.assert_allclose(d, d_ora, rtol=RTOL, atol=1e-300)
+        assert eng.binder_ed(t) == pytest.approx(ora.binder_ed(d_ora), rel=RTOL)
+
+
+@pytest.mark.parametrize("alpha", [0.4, 1.55, 1.5707, 1.6])
+def test_order_parameter_and_scores_match_oracle(pair, alpha):
+    ora, eng, c_ora, *_ = pair
+    eng.set_constants(c_ora)
+    T, P = targets()
+    eng.reweight(T, P)
+    p2, p3 = rw.peaks()
+    for t in range(len(T)):
+        x, p, (media, a1, binder) = eng.order_parameter(t, alpha)
+        xo, po, mo, ao, rc = ora.normx(ora.distribution(c_ora, T[t], P[t]), alpha)
+        assert rc == 0
+        np.testing.assert_allclose(x, xo, rtol=RTOL, atol=1e-12)
+        np.testing.assert_allclose(p, po, rtol=RTOL, atol=1e-12)
+        assert (media, a1) == pytest.approx((mo, ao), rel=RTOL)
+        assert binder == pytest.approx(rw.binder(xo, po, mo, ao), rel=1e-8)
+        from cvfbulkwater_amd.reweight import ising_scores
+        np.testing.assert_allclose(ising_scores(x, p), rw.scores(xo, po, p2, p3), rtol=1e-8, atol=1e-12)
+
+
+def test_error_bars_match_oracle(pair):
+    ora, eng, c_ora, _, data = pair
+    eng.set_constants(c_ora)
+    T, P = eng.sim_points()
+    for k in (0, 3):
+        eng.reweight([T[k]], [P[k]])
+        eh, ex = eng.errors(0, k, 1.57)
+        e, r = [], []
+        for h, rho in data[k]:
+            rho_si = SL(1527.3) * rho + SL(-23.1017)
+            e.append(SL(11.272132) * h + SL(9.028914) * TS[k] - SL(1.8) * P[k] / rho_si)
+            r.append(rho_si)
+        eh_o, ex_o, rc = ora.errors(ora.distribution(c_ora, T[k], P[k]), 1.57, np.stack([e, r], 1).ravel())
+        assert rc == 0
+        np.testing.assert_allclose(eh, eh_o, rtol=1e-8, atol=1e-12)
+        np.testing.assert_allclose(ex, ex_o, rtol=1e-8, atol=1e-12)
+
+
+def test_fixed_point_at_full_size():
+    # the reference's production shape: 11 simulations x 100000 samples,
+    # BIN_FCT 2 (3560^2 grid); after convergence one more sweep is the
+    # identity, so reweighting to the simulations returns the constants
+    Ts = [0.1058 + 0.0001 * k for k in range(11)]
+    data = rw_synth.samples(Ts, 100000, seed=9)
+    print("reweight full size: 11 x 100000 samples, bin factor 2", flush=True)
+    with Reweighting(8, Ts, [0.5] * 11, data, 2.0) as eng:
+        info = eng.info()
+        assert info["n_bin2d"] == int((int(np.sqrt(np.float32(100000)) / 2) + 20) * 10 * 2.0)
+        c, prev, ok = eng.solve_constants(300)
+        assert ok and np.max(np.abs(c - prev)) <= 1e-7
+        T, P = eng.sim_points()
+        lz = eng.reweight(T, P)
+        np.testing.assert_allclose(lz - lz[0], c, rtol=0, atol=2e-7)
+        # every density is normalized: sum * de * drho = 1
+        for t in range(0, 11, 5):
+            assert eng.density(t).sum() * info["de"] * info["drho"] == pytest.approx(1.0, rel=1e-12)
+
+
+def run_exe(d: Path):
+    return subprocess.run([str(REWEIGHT_EXE_PATH), str(d)], capture_output=True, text=True, timeout=300)
+
+
+def numbers(text: str) -> list[float]:
+    out = []
+    for tok in text.replace("=", " ").split():
+        try:
+            out.append(float(tok.rstrip("K")))
+        except ValueError:
+            pass
+    return out
+
+
+def compare_dirs(a: Path, b: Path, skip=()):
+    na = sorted(p.name for p in a.iterdir() if p.name not in skip)
+    nb = sorted(p.name for p in b.iterdir() if p.name not in skip)
+    assert na == nb
+    for n in na:
+        if n.startswith("data_") or n == "input":
+            continue
+        ta, tb = (a / n).read_text(), (b / n).read_text()
+        assert ta.count("\n") == tb.count("\n"), n
+        np.testing.assert_allclose(numbers(ta), numbers(tb), rtol=1e-8, atol=1e-12, err_msg=n)
+
+
+@pytest.mark.parametrize("P_label,n_sim", [("0.5", 6), ("-0.2", 4), ("0.5", 1)])
+def test_executable_matches_oracle_program(tmp_path, P_label, n_sim):
+    labels = ["0.1058", "0.106", "0.1062", "0.1064", "0.1066", "0.1068"][:n_sim]
+    a, b = tmp_path / "oracle", tmp_path / "engine"
+    P_range = ("0.500", "0.501", "0.1") if P_label == "0.5" else ("-0.200", "-0.199", "0.1")
+    rw_synth.write_case(a, 8, labels, P_label, 1500, P_range=P_range)
+    shutil.copytree(a, b)
+    print(f"reweight program: P={P_label}, {n_sim} simulations", flush=True)
+    assert rw.run_program(a, a / "summary.txt") == 0
+    r = run_exe(b)
+    assert r.returncode == 0, r.stderr
+    (b / "summary.txt").write_text(r.stderr)
+    compare_dirs(a, b)
+    # the constants file is read back on a second run
+    r2 = run_exe(b)
+    assert r2.returncode == 0 and r2.stderr == r.stderr
+
+
+def test_executable_exit_statuses(tmp_path):
+    d = tmp_path / "short"
+    rw_synth.write_case(d, 8, ["0.1058", "0.106", "0.1062"], "0.5", 800, steps=2)
+    r = run_exe(d)
+    assert r.returncode == 1 and "converged" in r.stderr
+    d2 = tmp_path / "bad"
+    rw_synth.write_case(d2, 8, ["0.1058", "0.106"], "0.5", 500, S_range=("1.60", "1.55", "0.02"))
+    r = run_exe(d2)
+    assert r.returncode == 1 and "Smax <= Smin" in r.stderr
+
+
+def test_from_series_takes_h_over_n_and_n_over_v():
+    L, N = 8, 512.0
+    data = rw_synth.samples(TS[:3], 1000, seed=2)
+    series = np.zeros((3, 1000, 6))
+    for k, d in enumerate(data):
+        series[k, :, 1] = d[:, 0] * N  # H
+        series[k, :, 2] = N / d[:, 1]  # V
+    with from_series(L, TS[:3], [0.5] * 3, series) as a, Reweighting(L, TS[:3], [0.5] * 3, data) as b:
+        ia, ib = a.info(), b.info()
+        # N / (N / rho) may differ from rho in the last bit
+        for k in ("n_sim", "N", "n_bin", "n_bin2d", "n_bin_mixed"):
+            assert ia[k] == ib[k]
+        for k in ("e_min", "rho_min", "de", "drho"):
+            assert ia[k] == pytest.approx(ib[k], rel=1e-12)
+        assert abs(ia["n_occupied"] - ib["n_occupied"]) <= 3
