@@ -281,6 +281,70 @@ def plain_per_L(steps: int, warmup: int, ev: HipEvents) -> dict:
     return {"ms_per_step_by_L": per, "value": cells / (total_ms * 1e-3)}
 
 
+def reweight_samples(n_sim: int = 11, n: int = 100000, seed: int = 9):
+    """(h, rho) samples in internal units for the histogram reweighting leg:
+    an isobar of n_sim temperatures 0.0001 apart near T^ = 0.1063, each a
+    T-dependent mixture of a low- and a high-density blob, n samples each."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    T_hat = [0.1058 + 0.0001 * k for k in range(n_sim)]
+    out = []
+    for T in T_hat:
+        hi = rng.random(n) < 1.0 / (1.0 + np.exp((T - 0.1063) / 0.0004))
+        rho = np.where(hi, 0.690, 0.640) + 0.006 * rng.standard_normal(n)
+        h = np.where(hi, -2.80, -2.72) + 0.012 * rng.standard_normal(n) + 0.5 * (rho - 0.665) + 0.8 * (T - 0.1063)
+        out.append(np.round(np.stack([h, rho], axis=1), 5))
+    return T_hat, out
+
+
+def reweight_form(sweeps: int = 300, cpu_sweeps: int = 2, with_cpu: bool = True) -> dict:
+    """Histogram reweighting at the reference's production shape: 11
+    simulations x 100000 samples, BIN_FCT 2 (3560 x 3560 grid), L = 8.
+    Times `sweeps` Jacobi sweeps of the constants (the program's STEP) and
+    one 64-point (T, P) scan reweighting, wall clock around the library
+    calls; the CPU oracle (the Fortran loops restated in C, serial) times a
+    few sweeps of the same input."""
+    from cvfbulkwater_amd.reweight import Reweighting
+
+    T_hat, data = reweight_samples()
+    eng = Reweighting(8, T_hat, [0.5] * len(T_hat), data, 2.0)
+    info = eng.info()
+    eng.solve_constants(3)  # warm-up
+    t0 = time.perf_counter()
+    c, prev, ok = eng.solve_constants(sweeps)
+    sweep_ms = (time.perf_counter() - t0) * 1e3 / sweeps
+    TK = [(0.1058 + 0.00002 * k) * 140.5678 + 185.4676 for k in range(64)]
+    PB = [0.5 * 4694.59 - 2111.5552] * 64
+    eng.reweight(TK[:4], PB[:4])
+    t0 = time.perf_counter()
+    eng.reweight(TK, PB)
+    scan_ms = (time.perf_counter() - t0) * 1e3
+    eng.close()
+    terms = info["n_occupied"] * len(T_hat) ** 2
+    res = {"workload": "histogram reweighting: 11 simulations x 100000 (h, rho) samples, BIN_FCT 2, "
+                       f"{info['n_bin2d']}^2 grid, {info['n_occupied']} occupied cells, {sweeps} constant "
+                       "sweeps (wall clock)", "ms_per_sweep": sweep_ms, "converged": ok,
+           "terms_per_s": terms / (sweep_ms * 1e-3), "scan_64_points_ms": scan_ms,
+           "scan_densities_copied_mb": 64 * info["n_occupied"] * 8 / 2**20}
+    if with_cpu:
+        import numpy as np
+        from oracle import rw as ORW
+
+        g = ORW.OraGrid.from_samples(8, T_hat, [0.5] * len(T_hat), [len(d) for d in data],
+                                     np.concatenate(data), 2.0)
+        t0 = time.perf_counter()
+        g.constants(cpu_sweeps)
+        cpu_ms = (time.perf_counter() - t0) * 1e3 / cpu_sweeps
+        g.close()
+        res["cpu_baseline"] = {"value": terms / (cpu_ms * 1e-3), "unit": "cell-simulation terms/s",
+                               "ms_per_sweep": cpu_ms, "cores": 1, "kind": "port",
+                               "sample": f"CPU oracle, serial C over the full grid as the Fortran loops run, "
+                                         f"{cpu_sweeps} sweeps of the same input"}
+        res["speedup_per_sweep"] = cpu_ms / sweep_ms
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,7 +360,7 @@ def main() -> int:
                          "(skips the autotuner)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-steps", type=int, default=5000)
-    ap.add_argument("--only", choices=["batch", "plain", "precision", "cluster"], default=None,
+    ap.add_argument("--only", choices=["batch", "plain", "precision", "cluster", "reweight"], default=None,
                     help="run one leg only (for profiling)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (calibrates the "
@@ -353,6 +417,8 @@ def main() -> int:
         out["cluster_only"] = cluster_form(args.plain_steps, args.warmup, ev)
     if args.only in (None, "precision"):
         out["precision"] = precision_form(args.precision_steps, 10, ev)
+    if args.only in (None, "reweight"):
+        out["reweight"] = reweight_form(with_cpu=not args.no_cpu_baseline)
 
     line = {"metric": "site-updates/s per GPU, batched mixed-prec vs plain; achieved HBM GB/s",
             "unit": "site-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -407,6 +473,10 @@ def main() -> int:
         line["precision_form"] = out["precision"]
         if "value" not in line:
             line.update(value=None, ms_per_step=out["precision"].get("ms_per_step", {}).get("fp32c"))
+    if "reweight" in out:
+        line["reweight_form"] = out["reweight"]
+        if "value" not in line:
+            line.update(value=out["reweight"]["terms_per_s"], ms_per_step=out["reweight"]["ms_per_sweep"])
     if not args.no_cpu_baseline and args.only is None:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
         line["cpu_baseline"]["cores"] = 1

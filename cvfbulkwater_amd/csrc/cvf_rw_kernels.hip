@@ -3,19 +3,29 @@
 // paramSI_norm_x.f90:1003-1124) for many targets in one launch, and the
 // per-target max / log-sum-exp / normalization reductions.
 //
-// Every cell and target is independent, so one thread takes one (cell,
-// target) pair; the simulations' parameters sit in LDS and every lane of a
-// wave reads the same entry (a broadcast).  The arithmetic is FP64 in the
-// reference's order, including its two cut-offs: a cell's sum over
-// simulations switches to the max-shifted form with terms below max - 10
-// dropped once any |arg| exceeds 10, and log_Z drops cells more than 10
-// below the maximum.
+// The reference's argument for cell c, target t and simulation j is
+//   arg_j = beta_t (E + p_t V) - beta_j (E + p_j V) - c_j + lm_j = t1 + u_j,
+// so the target only shifts a cell's sum: log sum_j exp(arg_j) =
+// t1 + log sum_j exp(u_j).  k_rw_logp evaluates the n_sim terms u_j once
+// per cell and then serves every target of its block with a few flops.
+// The reference's two cut-offs are kept exactly:
+//  - a cell's sum switches to the max-shifted form that drops terms more
+//    than 10 below the largest once any |arg_j| > 10 (f90:1048-1078); the
+//    switch depends on the target through t1 (t1 + max u > 10 or
+//    t1 + min u < -10), the dropped set does not, so both sums (all terms,
+//    and the terms within 10 of the largest) are formed once per cell;
+//  - log_Z drops cells more than 10 below the largest log_P (1106-1118).
+// Factoring t1 out rounds differently from the reference's left-to-right
+// sum: log_P moves by about |t1| x 1e-16 (1e-10 for L = 8).
 #include <math.h>
 
 #include "cvf_rw_kernels.h"
 
 namespace cvf {
 namespace {
+
+constexpr int kTargetsPerBlock = 16;  // targets served per k_rw_logp block
+constexpr int kSumBlocks = 128;       // k_rw_sum blocks per target (grid-stride)
 
 __device__ inline double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
@@ -27,7 +37,7 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// block-wide reduction of kRwBlock threads; result valid in thread 0
+// block-wide reduction of kRwBlock threads, result in every thread
 template <bool kMax>
 __device__ inline double block_reduce(double v, double* red) {
   v = kMax ? wave_max(v) : wave_sum(v);
@@ -40,17 +50,15 @@ __device__ inline double block_reduce(double v, double* red) {
   return r;
 }
 
-// arg_j of f90:1048: beta1 (E + p1 V) - beta_j (E + p_j V) - c_j + lm_j
-__device__ inline double rw_arg(double t1, double E, double V, const double* s) {
-  return t1 - s[0] * (E + s[1] * V) - s[2] + s[3];
-}
+// u_j = -beta_j (E + p_j V) - c_j + lm_j; s = {beta, p, c, lm}
+__device__ inline double rw_u(double E, double V, const double* s) { return -(s[0] * (E + s[1] * V)) - s[2] + s[3]; }
 
 __global__ __launch_bounds__(kRwBlock) void k_rw_logp(RwCells cells, const RwSim* __restrict__ sims,
                                                       const double* __restrict__ ctes, int32_t n_sim,
                                                       RwTargets tg, double* __restrict__ logp,
                                                       double* __restrict__ pmax, int32_t nblk) {
   extern __shared__ double sl[];  // 4 per simulation: beta, p, c, lm
-  __shared__ double red[kRwBlock / 64];
+  __shared__ double red[kTargetsPerBlock][kRwBlock / 64];
   for (int j = threadIdx.x; j < n_sim; j += kRwBlock) {
     sl[4 * j] = sims[j].beta;
     sl[4 * j + 1] = sims[j].pres;
@@ -58,86 +66,98 @@ __global__ __launch_bounds__(kRwBlock) void k_rw_logp(RwCells cells, const RwSim
     sl[4 * j + 3] = sims[j].lm;
   }
   __syncthreads();
-  const int t = blockIdx.y;
   const int64_t b = (int64_t)blockIdx.x * kRwBlock + threadIdx.x;
-  double lp = -INFINITY;
-  if (b < cells.n) {
-    const double E = cells.en[b], V = cells.vn[b];
-    const double2 bp = tg.bp[t];
-    const double t1 = bp.x * (E + bp.y * V);
-    bool flag = false;
-    double mx = rw_arg(t1, E, V, sl);
+  const bool live = b < cells.n;
+  double E = 0, V = 0, lh = 0, umax = -INFINITY, umin = INFINITY, all = 0, near = 0;
+  if (live) {
+    E = cells.en[b];
+    V = cells.vn[b];
+    lh = cells.lh[b];
     for (int j = 0; j < n_sim; ++j) {
-      const double a = rw_arg(t1, E, V, sl + 4 * j);
-      flag |= fabs(a) > 10.0;
-      if (mx < a) mx = a;
+      const double u = rw_u(E, V, sl + 4 * j);
+      umax = fmax(umax, u);
+      umin = fmin(umin, u);
     }
-    double xx = 0;
-    if (!flag) {
-      for (int j = 0; j < n_sim; ++j) xx = xx + exp(rw_arg(t1, E, V, sl + 4 * j));
-      xx = log(xx);
-    } else {
-      for (int j = 0; j < n_sim; ++j) {
-        const double a = rw_arg(t1, E, V, sl + 4 * j);
-        if (mx - a < 10.0) xx = xx + exp(a - mx);
-      }
-      if (xx != 0) xx = log(xx);
-      xx = xx + mx;
+    double sa = 0, st = 0;
+    for (int j = 0; j < n_sim; ++j) {
+      const double d = rw_u(E, V, sl + 4 * j) - umax;
+      const double e = exp(d);
+      sa = sa + e;
+      if (-d < 10.0) st = st + e;
     }
-    lp = cells.lh[b] - xx;
-    logp[(int64_t)t * cells.n + b] = lp;
+    all = log(sa) + umax;   // log sum_j exp(u_j)
+    near = log(st) + umax;  // the same over the terms within 10 of the largest
   }
-  const double m = block_reduce<true>(lp, red);
-  if (threadIdx.x == 0) pmax[(int64_t)t * nblk + blockIdx.x] = m;
+  // every target's wave maxima into LDS, then one barrier for all of them
+  const int t0 = blockIdx.y * kTargetsPerBlock, nt = min(tg.n - t0, kTargetsPerBlock);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int k = 0; k < nt; ++k) {
+    const int t = t0 + k;
+    double lp = -INFINITY;
+    if (live) {
+      const double2 bp = tg.bp[t];
+      const double t1 = bp.x * (E + bp.y * V);
+      const bool cut = t1 + umax > 10.0 || t1 + umin < -10.0;
+      lp = lh - (t1 + (cut ? near : all));
+      logp[(int64_t)t * cells.n + b] = lp;
+    }
+    lp = wave_max(lp);
+    if (lane == 0) red[k][wave] = lp;
+  }
+  __syncthreads();
+  if (threadIdx.x < nt) {
+    double m = red[threadIdx.x][0];
+    for (int w = 1; w < kRwBlock / 64; ++w) m = fmax(m, red[threadIdx.x][w]);
+    pmax[(int64_t)(t0 + threadIdx.x) * nblk + blockIdx.x] = m;
+  }
 }
 
-__global__ __launch_bounds__(kRwBlock) void k_rw_max(const double* __restrict__ pmax, int32_t nblk,
-                                                     double* __restrict__ mx) {
-  __shared__ double red[kRwBlock / 64];
-  const double* p = pmax + (int64_t)blockIdx.x * nblk;
-  double m = -INFINITY;
-  for (int i = threadIdx.x; i < nblk; i += kRwBlock) m = fmax(m, p[i]);
-  m = block_reduce<true>(m, red);
-  if (threadIdx.x == 0) mx[blockIdx.x] = m;
-}
-
-// partial sums of exp(log_P - max): over cells within 10 of the max (for
-// log_Z) and over all cells (for the normalization)
+// per target: the max of log_P from the k_rw_logp partials, then partial
+// sums of exp(log_P - max) over the cells within 10 of the max (for log_Z)
+// and over all cells (for the normalization), grid-stride over the cells
 __global__ __launch_bounds__(kRwBlock) void k_rw_sum(const double* __restrict__ logp, int64_t n,
-                                                     const double* __restrict__ mx, int32_t nblk,
-                                                     double* __restrict__ ps) {
+                                                     const double* __restrict__ pmax, int32_t nblk,
+                                                     double* __restrict__ mx, double* __restrict__ ps) {
   __shared__ double red[kRwBlock / 64];
   const int t = blockIdx.y;
-  const int64_t b = (int64_t)blockIdx.x * kRwBlock + threadIdx.x;
+  const double* pm = pmax + (int64_t)t * nblk;
+  double m = -INFINITY;
+  for (int i = threadIdx.x; i < nblk; i += kRwBlock) m = fmax(m, pm[i]);
+  m = block_reduce<true>(m, red);
+  const double* lp = logp + (int64_t)t * n;
   double st = 0, sa = 0;
-  if (b < n) {
-    const double m = mx[t], v = logp[(int64_t)t * n + b];
+  for (int64_t b = (int64_t)blockIdx.x * kRwBlock + threadIdx.x; b < n; b += (int64_t)gridDim.x * kRwBlock) {
+    const double v = lp[b];
     const double e = exp(v - m);
-    sa = e;
-    if (m - v < 10.0) st = e;
+    sa = sa + e;
+    if (m - v < 10.0) st = st + e;
   }
   st = block_reduce<false>(st, red);
   sa = block_reduce<false>(sa, red);
   if (threadIdx.x == 0) {
-    ps[2 * ((int64_t)t * nblk + blockIdx.x)] = st;
-    ps[2 * ((int64_t)t * nblk + blockIdx.x) + 1] = sa;
+    ps[2 * ((int64_t)t * gridDim.x + blockIdx.x)] = st;
+    ps[2 * ((int64_t)t * gridDim.x + blockIdx.x) + 1] = sa;
+    if (blockIdx.x == 0) mx[t] = m;
   }
 }
 
-__global__ __launch_bounds__(kRwBlock) void k_rw_finish(const double* __restrict__ ps, int32_t nblk, int32_t n_t,
+// one wave per target: log_Z = log(truncated sum) + max, the
+// normalization sum exp(log_P - log_Z) de drho, and with ctes_out the
+// constants' update log_Z(t) - log_Z(0)
+__global__ __launch_bounds__(kRwBlock) void k_rw_finish(const double* __restrict__ ps, int32_t sblk, int32_t n_t,
                                                         const double* __restrict__ mx, double de, double drho,
                                                         double* __restrict__ lz, double* __restrict__ norm,
                                                         double* __restrict__ ctes_out) {
-  __shared__ double red[kRwBlock / 64];
-  for (int t = 0; t < n_t; ++t) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int t = wave; t < n_t; t += kRwBlock / 64) {
     double st = 0, sa = 0;
-    for (int i = threadIdx.x; i < nblk; i += kRwBlock) {
-      st += ps[2 * ((int64_t)t * nblk + i)];
-      sa += ps[2 * ((int64_t)t * nblk + i) + 1];
+    for (int i = lane; i < sblk; i += 64) {
+      st += ps[2 * ((int64_t)t * sblk + i)];
+      sa += ps[2 * ((int64_t)t * sblk + i) + 1];
     }
-    st = block_reduce<false>(st, red);
-    sa = block_reduce<false>(sa, red);
-    if (threadIdx.x == 0) {
+    st = wave_sum(st);
+    sa = wave_sum(sa);
+    if (lane == 0) {
       const double z = (st != 0 ? log(st) : 0.0) + mx[t];
       lz[t] = z;
       norm[t] = sa * exp(mx[t] - z) * de * drho;
@@ -159,20 +179,25 @@ __global__ __launch_bounds__(kRwBlock) void k_rw_dist(double* __restrict__ logp,
   }
 }
 
+int32_t rw_sum_blocks(int32_t nblk) { return nblk < kSumBlocks ? nblk : kSumBlocks; }
+
 }  // namespace
 
 hipError_t launch_rw_pass(const RwCells& cells, const RwSim* sims, const double* ctes, int32_t n_sim,
                           const RwTargets& tg, const RwWork& w, double de, double drho, double* ctes_out,
                           bool dist, hipStream_t stream) {
-  const dim3 grid((unsigned)w.nblk, (unsigned)tg.n);
+  const unsigned rows = (unsigned)((tg.n + kTargetsPerBlock - 1) / kTargetsPerBlock);
   const size_t lds = sizeof(double) * 4 * (size_t)n_sim;
-  hipLaunchKernelGGL(k_rw_logp, grid, dim3(kRwBlock), lds, stream, cells, sims, ctes, n_sim, tg, w.logp, w.pmax,
-                     w.nblk);
-  hipLaunchKernelGGL(k_rw_max, dim3((unsigned)tg.n), dim3(kRwBlock), 0, stream, w.pmax, w.nblk, w.mx);
-  hipLaunchKernelGGL(k_rw_sum, grid, dim3(kRwBlock), 0, stream, w.logp, cells.n, w.mx, w.nblk, w.ps);
-  hipLaunchKernelGGL(k_rw_finish, dim3(1), dim3(kRwBlock), 0, stream, w.ps, w.nblk, tg.n, w.mx, de, drho, w.lz,
+  hipLaunchKernelGGL(k_rw_logp, dim3((unsigned)w.nblk, rows), dim3(kRwBlock), lds, stream, cells, sims, ctes,
+                     n_sim, tg, w.logp, w.pmax, w.nblk);
+  const int32_t sblk = rw_sum_blocks(w.nblk);
+  hipLaunchKernelGGL(k_rw_sum, dim3((unsigned)sblk, (unsigned)tg.n), dim3(kRwBlock), 0, stream, w.logp, cells.n,
+                     w.pmax, w.nblk, w.mx, w.ps);
+  hipLaunchKernelGGL(k_rw_finish, dim3(1), dim3(kRwBlock), 0, stream, w.ps, sblk, tg.n, w.mx, de, drho, w.lz,
                      w.norm, ctes_out);
-  if (dist) hipLaunchKernelGGL(k_rw_dist, grid, dim3(kRwBlock), 0, stream, w.logp, cells.n, w.lz, w.norm);
+  if (dist)
+    hipLaunchKernelGGL(k_rw_dist, dim3((unsigned)w.nblk, (unsigned)tg.n), dim3(kRwBlock), 0, stream, w.logp,
+                       cells.n, w.lz, w.norm);
   return hipGetLastError();
 }
 
