@@ -118,6 +118,55 @@ def cpu_baseline(budget_s: float) -> dict:
                       + f" MC steps), {dt:.1f} s on {platform.processor() or platform.machine()}"}
 
 
+def _cpu_point(args) -> tuple[int, int, int]:
+    """One batch state point on the CPU oracle for `budget` seconds (a
+    worker of cpu_baseline_all_cores): (cells x steps, L, steps)."""
+    i, budget = args
+    from oracle import py as O
+
+    L = (16, 24, 32, 48)[i % 4]
+    k, j = (i % 128) % 8, (i % 128) // 8
+    sim = O.OracleSim(L=L, T=0.55 + 0.45 * k / 7, P=-0.10 + 0.06 * j, seed=44648604 + 1000003 * i)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget:
+        sim.step(1)
+        n += 1
+    sim.close()
+    return n * L ** 3, L, n
+
+
+def host_info() -> dict:
+    model = platform.processor() or platform.machine()
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline_all_cores(budget_s: float, max_workers: int = 16) -> dict:
+    """The CPU oracle on every usable host core (at most 16, the GPU box's
+    share): one independent batch state point per worker process, all
+    running for the same budget; value = summed site-updates / wall time."""
+    import multiprocessing as mp
+
+    from oracle import py as O
+
+    O.build()
+    n = max(1, min(max_workers, len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(n) as pool:
+        res = pool.map(_cpu_point, [(i, budget_s) for i in range(n)])
+    dt = time.perf_counter() - t0
+    total = sum(r[0] for r in res)
+    return {"value": total / dt, "unit": "site-updates/s", "cores": n, "kind": "port",
+            "sample": f"CPU oracle, serial C, {n} worker processes each stepping one batch state point "
+                      f"(L = 16/24/32/48 by index) for {budget_s:.0f} s; wall {dt:.1f} s incl. start-up"}
+
+
 def calibrate_copy(mib: float, reps: int = 10) -> int:
     """hipMemcpyDtoD of a known size with the runtime's copy kernel: under
     rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch give the scale
@@ -376,8 +425,12 @@ def main() -> int:
 
     from cvfbulkwater_amd import FP64, VOLUME_HOST, Params, Simulation
 
-    ev = HipEvents()
     out = {}
+    if not args.no_cpu_baseline and args.only is None:
+        # the CPU legs run before this process touches the GPU
+        out["cpu"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_all"] = cpu_baseline_all_cores(args.cpu_seconds / 2)
+    ev = HipEvents()
     if args.only in (None, "batch"):
         pts = batch_params(fp32=True)
         cells = sum(p.L ** 3 for p in pts)
@@ -477,9 +530,13 @@ def main() -> int:
         line["reweight_form"] = out["reweight"]
         if "value" not in line:
             line.update(value=out["reweight"]["terms_per_s"], ms_per_step=out["reweight"]["ms_per_sweep"])
-    if not args.no_cpu_baseline and args.only is None:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
-        line["cpu_baseline"]["cores"] = 1
+    if "batch_rate" in out:
+        # 6 arms x NMETROPOLIS (= 1) sigma proposals per molecule-step
+        line["sigma_attempts_per_s"] = 6 * out["batch_rate"]
+    if "cpu" in out:
+        line["cpu_baseline"] = out["cpu"]
+        line["cpu_baseline_all_cores"] = out["cpu_all"]
+        line["host"] = host_info()
     print(json.dumps(line), flush=True)
     return 0
 
