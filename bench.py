@@ -14,12 +14,20 @@ T = 0.815, P = 0.45, FP64 vdW, host volume move (the reference's
 production shape).  value = site-updates/s of the batched form
 (site-update = one molecule advanced one full MC step).
 
-The roofline is for the dominant kernel (k_sigma, the half-lattice sigma
-sweep): algorithmic bytes per launch = (cells/2) x 12 B (4 B own word read,
-4 B neighbour word read, 4 B write), divided by the mean launch duration
-recorded by the dispatch itself (hipExtLaunchKernelGGL start/stop events).
+The roofline is for the dominant kernel of the chosen launch configuration:
+k_chunk (fused, six half-sweeps per launch) at 8 B per cell of the launch
+(every word read and written once), or k_sigma (one half-lattice sigma
+sweep) at 12 B per active cell; divided by the mean launch duration recorded
+by the dispatch itself (hipExtLaunchKernelGGL start/stop events).
 
-Inputs are synthetic (random initial lattices); nothing is read from disk.
+Further legs: the plain form run as the drop-in executable samples it, the
+cluster form, the 128 points as sequential plain runs, the non-chessboard
+and correlation modes, the L = 96 precision form, and the histogram
+reweighting at its production shape.  The CPU legs (the oracle on one core
+and on all usable cores) run before the GPU is touched.
+
+Inputs are synthetic (random initial lattices, synthetic (h, rho) samples);
+nothing is read from disk.
 """
 from __future__ import annotations
 
@@ -313,6 +321,54 @@ def cluster_form(steps: int, warmup: int, ev: HipEvents) -> dict:
             "value": 32 ** 3 / (ms * 1e-3)}
 
 
+def nonchess_form(steps: int, warmup: int, ev: HipEvents, cpu_seconds: float = 0.0) -> dict:
+    """Flag_chessboard = 0 at the plain shape: L = 32, T = 0.815, P = 0.45,
+    eta fixed, hydrogen bonds gated by the per-molecule counters, FP64,
+    host volume move; with cpu_seconds, the CPU oracle runs the same mode."""
+    from cvfbulkwater_amd import FP64, VOLUME_HOST, Params, Simulation
+
+    sim = Simulation(Params(L=32, T=0.815, P=0.45, seed=44648604, flag_chessboard=0, precision=FP64,
+                            volume_mode=VOLUME_HOST))
+    ms = time_steps(sim, steps, warmup, ev)
+    sim.close()
+    res = {"workload": "non-chessboard mode (Flag_chessboard=0): L=32, T=0.815, P=0.45, 1 Metropolis + "
+                       "1 volume move per step, FP64, host volume move", "ms_per_step": ms,
+           "value": 32 ** 3 / (ms * 1e-3)}
+    if cpu_seconds > 0:
+        from oracle import py as O
+
+        o = O.OracleSim(L=32, T=0.815, P=0.45, seed=44648604, flag_chessboard=0)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            o.step(1)
+            n += 1
+        dt = time.perf_counter() - t0
+        o.close()
+        res["cpu_baseline"] = {"value": n * 32 ** 3 / dt, "unit": "site-updates/s", "cores": 1, "kind": "port",
+                               "sample": f"CPU oracle, serial C, the same state point, {n} MC steps"}
+    return res
+
+
+def correlation_form(steps: int, warmup: int) -> dict:
+    """Flag_correlation = 1 as the drop-in executable runs it: after every
+    step the sigma histogram (one device reduction, six counters back to the
+    host) feeds the multi-tau correlator; L = 32 plain form, wall clock."""
+    from cvfbulkwater_amd import FP64, VOLUME_HOST, Correlator, Params, Simulation
+
+    sim = Simulation(Params(L=32, T=0.815, P=0.45, seed=44648604, precision=FP64, volume_mode=VOLUME_HOST))
+    corr = Correlator()
+    sim.step(warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step(1)
+        corr.add(sim.sigma_histogram(0).max() / (6.0 * 32 ** 3))
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    sim.close()
+    return {"workload": "Flag_correlation=1 on the plain form: one sigma histogram per step into the "
+                        "multi-tau correlator, wall clock", "ms_per_step": ms, "value": 32 ** 3 / (ms * 1e-3)}
+
+
 def plain_per_L(steps: int, warmup: int, ev: HipEvents) -> dict:
     """The batch run as 128 sequential plain drop-in runs (one state point
     per handle, FP64, host volume move): each L's plain step time times its
@@ -465,7 +521,10 @@ def main() -> int:
         out.update(plain_ms=pms, plain_rate=32 ** 3 / (pms * 1e-3), plain_tuning=ptuning,
                    seq=plain_per_L(max(args.plain_steps // 4, 100), args.warmup, ev),
                    cluster=cluster_form(max(args.plain_steps // 4, 100), args.warmup, ev),
-                   sampled=plain_sampled(args.plain_steps, args.warmup, ev))
+                   sampled=plain_sampled(args.plain_steps, args.warmup, ev),
+                   nonchess=nonchess_form(max(args.plain_steps // 4, 100), args.warmup, ev,
+                                          0.0 if args.no_cpu_baseline else 3.0),
+                   correlation=correlation_form(max(args.plain_steps // 4, 100), args.warmup))
     if args.only == "cluster":
         out["cluster_only"] = cluster_form(args.plain_steps, args.warmup, ev)
     if args.only in (None, "precision"):
@@ -511,6 +570,8 @@ def main() -> int:
                          "tuning": out["plain_tuning"]}
         line["cluster_form"] = out["cluster"]
         line["plain_sampled"] = out["sampled"]
+        line["nonchess_form"] = out["nonchess"]
+        line["correlation_form"] = out["correlation"]
         line["plain_sequential_128"] = {
             "workload": "the 128 batch state points as sequential plain runs (FP64, host volume "
                         "move), from the per-L plain step times", **out["seq"]}
