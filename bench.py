@@ -360,13 +360,16 @@ def correlation_form(steps: int, warmup: int) -> dict:
     sim.step(warmup)
     sim.sync()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        sim.step(1)
-        corr.add(sim.sigma_histogram(0).max() / (6.0 * 32 ** 3))
-    ms = (time.perf_counter() - t0) * 1e3 / steps
+    # as the drop-in runs it: the histograms of the steps between two samples
+    # (interval 10) collect on the device and come back together
+    for _ in range(steps // 10):
+        for h in sim.step_histograms(10):
+            corr.add(h.max() / (6.0 * 32 ** 3))
+    ms = (time.perf_counter() - t0) * 1e3 / (steps // 10 * 10)
     sim.close()
-    return {"workload": "Flag_correlation=1 on the plain form: one sigma histogram per step into the "
-                        "multi-tau correlator, wall clock", "ms_per_step": ms, "value": 32 ** 3 / (ms * 1e-3)}
+    return {"workload": "Flag_correlation=1 on the plain form: the sigma histogram after every step into "
+                        "the multi-tau correlator, histograms copied back every 10 steps, wall clock",
+            "ms_per_step": ms, "value": 32 ** 3 / (ms * 1e-3)}
 
 
 def plain_per_L(steps: int, warmup: int, ev: HipEvents) -> dict:

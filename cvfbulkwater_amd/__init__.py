@@ -160,6 +160,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cvf_get_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
         "cvf_autotune": (C.c_int, [P, I32, C.POINTER(_Tuning)]),
         "cvf_sigma_histogram": (C.c_int, [P, I32, C.POINTER(C.c_int64)]),
+        "cvf_step_histograms": (C.c_int, [P, I32, I64, C.POINTER(C.c_int64)]),
         "cvf_corr_create": (C.c_void_p, []),
         "cvf_corr_add": (None, [C.c_void_p, C.c_double]),
         "cvf_corr_result": (C.c_int, [C.c_void_p, DP, DP, I32]),
@@ -325,6 +326,14 @@ class Simulation:
         out = (C.c_int64 * 6)()
         _check(self._lib, self._lib.cvf_sigma_histogram(self._h, pt, out), "cvf_sigma_histogram")
         return np.array(list(out), np.int64)
+
+    def step_histograms(self, n: int, pt: int = 0) -> np.ndarray:
+        """Advance n steps and return point pt's sigma histogram after each:
+        shape (n, 6)."""
+        out = np.zeros((max(n, 0), 6), np.int64)
+        _check(self._lib, self._lib.cvf_step_histograms(self._h, pt, n, out.ctypes.data_as(C.POINTER(C.c_int64))),
+               "cvf_step_histograms")
+        return out
 
     # -- measurement
     def observe(self, pt: int = 0) -> Obs:

@@ -90,8 +90,21 @@ struct Group {
   std::vector<int> gsteps;
   std::vector<hipGraph_t> graphs;
   std::vector<hipGraphExec_t> gexecs;
+  // the same sequences with a sigma histogram of one point after every
+  // step (cvf_step_histograms), captured on demand
+  std::vector<hipGraph_t> rgraphs;
+  std::vector<hipGraphExec_t> rgexecs;
   size_t cells = 0;
+  void drop_rec_graphs() {
+    for (auto e : rgexecs)
+      if (e) (void)hipGraphExecDestroy(e);
+    for (auto g : rgraphs)
+      if (g) (void)hipGraphDestroy(g);
+    rgraphs.clear();
+    rgexecs.clear();
+  }
   void drop_graphs() {
+    drop_rec_graphs();
     for (auto e : gexecs)
       if (e) (void)hipGraphExecDestroy(e);
     for (auto g : graphs)
@@ -142,6 +155,10 @@ struct cvf_sim {
   DevBuf<uint64_t> cur_steps;          // one slot per group
   DevBuf<BlockEnt> tab_cell_all;       // every point's cells, for observables
   DevBuf<unsigned long long> hist;     // sigma histogram of one point (6 counters)
+  DevBuf<unsigned long long> hist_series;  // per-step histograms (6 counters per step)
+  DevBuf<uint64_t> hist_base;              // step count at the first recorded step
+  uint64_t hist_base_host = 0;
+  int rec_pt = -1;                         // point recorded by the groups' rec graphs
   uint32_t nb_cell_all = 0;
   std::vector<std::unique_ptr<Group>> groups;
   uint32_t max_slots = 0;
@@ -615,21 +632,31 @@ void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Time
   }
 }
 
+constexpr uint32_t kHistSpan = 4096;  // per-step histograms per copy back
+
 // nsteps MC steps of group g: the lattice kernels of every step, then one
 // kernel with the device volume moves of all of them (the chain never reads
-// the lattice) that also moves the group's step counter on.
-int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps) {
+// the lattice) that also moves the group's step counter on.  With rec, the
+// sigma histogram of point s->rec_pt follows every step.
+int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps, bool rec = false) {
   Group& g = *s->groups[gi];
-  for (int i = 0; i < nsteps; ++i) enqueue_lattice(s, gi, st, (uint32_t)i, Untimed{});
+  for (int i = 0; i < nsteps; ++i) {
+    enqueue_lattice(s, gi, st, (uint32_t)i, Untimed{});
+    if (rec) {
+      const PointHost& P = s->ph[s->rec_pt];
+      launch_sigma_hist_slot(st, s->words.p + P.woff, P.N, s->hist_series.p, s->cur_steps.p + gi, (uint32_t)i,
+                             s->hist_base.p, kHistSpan);
+    }
+  }
   launch_vol_chain(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, s->cur_steps.p + gi,
                    (uint32_t)nsteps, s->dist.p, s->freq.p, s->precision, s->vol_device);
   return CVF_OK;
 }
 
-int capture(cvf_sim* s, int gi, int nsteps, hipGraph_t* g, hipGraphExec_t* ge) {
+int capture(cvf_sim* s, int gi, int nsteps, hipGraph_t* g, hipGraphExec_t* ge, bool rec = false) {
   hipStream_t st = s->groups[gi]->stream;
   HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  const int rc = enqueue_steps(s, gi, st, nsteps);
+  const int rc = enqueue_steps(s, gi, st, nsteps, rec);
   hipError_t e = hipStreamEndCapture(st, g);
   if (rc) return rc;
   if (e != hipSuccess) return fail(CVF_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
@@ -656,6 +683,21 @@ int capture_step_graphs(cvf_sim* s) {
       g.gexecs.push_back(ge);
       if (rc) return rc;
     }
+  }
+  return CVF_OK;
+}
+
+// The recording sequences of group gi (same sizes as its plain ones).
+int capture_rec_graphs(cvf_sim* s, int gi) {
+  Group& g = *s->groups[gi];
+  if (!g.rgexecs.empty()) return CVF_OK;
+  for (int k : g.gsteps) {
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ge = nullptr;
+    const int rc = capture(s, gi, k, &gr, &ge, true);
+    g.rgraphs.push_back(gr);
+    g.rgexecs.push_back(ge);
+    if (rc) return rc;
   }
   return CVF_OK;
 }
@@ -921,19 +963,21 @@ int cvf_get_config(cvf_sim* s, int32_t pt, uint8_t* sigma6N, uint8_t* eta6N) {
   return CVF_OK;
 }
 
-int cvf_step(cvf_sim* s, int64_t n_steps) {
-  if (!s) return fail(CVF_EINVAL, "null handle");
-  if (n_steps < 0) return fail(CVF_EINVAL, "negative step count");
-  if (n_steps == 0) return CVF_OK;
-  HIPC(hipSetDevice(s->device));
+// n_steps MC steps as graph launches; group rec_gi (if >= 0) replays its
+// recording sequences.
+int run_steps(cvf_sim* s, int64_t n_steps, int rec_gi) {
   int rc = capture_step_graphs(s);
   if (rc) return rc;
+  if (rec_gi >= 0 && (rc = capture_rec_graphs(s, rec_gi))) return rc;
   if ((rc = fork_groups(s))) return rc;
   const std::vector<int>& sizes = s->groups[0]->gsteps;  // the same for every group
   for (int64_t i = 0; i < n_steps;) {
     size_t c = 0;
     while (sizes[c] > n_steps - i) ++c;  // the last size is 1
-    for (auto& g : s->groups) HIPC(hipGraphLaunch(g->gexecs[c], g->stream));
+    for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+      Group& g = *s->groups[gi];
+      HIPC(hipGraphLaunch((int)gi == rec_gi ? g.rgexecs[c] : g.gexecs[c], g.stream));
+    }
     const int64_t k = sizes[c];
     // the volume chain does not read the lattice: in host-volume mode the
     // host advances it while the device runs the lattice steps
@@ -944,6 +988,14 @@ int cvf_step(cvf_sim* s, int64_t n_steps) {
     i += k;
   }
   return join_groups(s);
+}
+
+int cvf_step(cvf_sim* s, int64_t n_steps) {
+  if (!s) return fail(CVF_EINVAL, "null handle");
+  if (n_steps < 0) return fail(CVF_EINVAL, "negative step count");
+  if (n_steps == 0) return CVF_OK;
+  HIPC(hipSetDevice(s->device));
+  return run_steps(s, n_steps, -1);
 }
 
 int cvf_sync(cvf_sim* s) {
@@ -1021,13 +1073,56 @@ int cvf_sigma_histogram(cvf_sim* s, int32_t pt, int64_t counts6[6]) {
   if (!counts6) return fail(CVF_EINVAL, "null output");
   HIPC(hipSetDevice(s->device));
   const PointHost& P = s->ph[pt];
-  if (!s->hist.p) HIPC(s->hist.alloc(6));  // kept for the handle's life: called every step
-  launch_zero_u32(s->stream, (uint32_t*)s->hist.p, 12);
+  if (!s->hist.p) HIPC(s->hist.alloc(kHistWords));  // kept for the handle's life: called every step
+  launch_zero_u32(s->stream, (uint32_t*)s->hist.p, 2 * kHistWords);
   launch_sigma_hist(s->stream, s->words.p + P.woff, P.N, s->hist.p);
-  unsigned long long out[6];
+  unsigned long long out[kHistWords];
   HIPC(hipMemcpyAsync(out, s->hist.p, sizeof out, hipMemcpyDeviceToHost, s->stream));
   HIPC(hipStreamSynchronize(s->stream));
-  for (int i = 0; i < 6; ++i) counts6[i] = (int64_t)out[i];
+  for (int i = 0; i < 6; ++i) {
+    counts6[i] = 0;
+    for (int r = 0; r < kHistReplicas; ++r) counts6[i] += (int64_t)out[6 * r + i];
+  }
+  return CVF_OK;
+}
+
+int cvf_step_histograms(cvf_sim* s, int32_t pt, int64_t n_steps, int64_t* counts) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  if (n_steps < 0) return fail(CVF_EINVAL, "negative step count");
+  if (n_steps == 0) return CVF_OK;
+  if (!counts) return fail(CVF_EINVAL, "null output");
+  HIPC(hipSetDevice(s->device));
+  if (!s->hist_series.p) HIPC(s->hist_series.alloc(kHistWords * (size_t)kHistSpan));
+  if (!s->hist_base.p) HIPC(s->hist_base.alloc(1));
+  if (s->rec_pt != pt) {
+    for (auto& g : s->groups) g->drop_rec_graphs();
+    s->rec_pt = pt;
+  }
+  int rec_gi = -1;
+  for (size_t gi = 0; gi < s->groups.size() && rec_gi < 0; ++gi)
+    for (int q : s->groups[gi]->points)
+      if (q == pt) rec_gi = (int)gi;
+  if (rec_gi < 0) return fail(CVF_EINVAL, "point %d is in no stream group", pt);
+  std::vector<unsigned long long> buf;
+  for (int64_t i0 = 0; i0 < n_steps; i0 += kHistSpan) {
+    const int64_t n = std::min<int64_t>(kHistSpan, n_steps - i0);
+    // slots are counted from the step count before this span
+    s->hist_base_host = (uint64_t)s->steps;
+    HIPC(hipMemcpyAsync(s->hist_base.p, &s->hist_base_host, sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
+    launch_zero_u32(s->stream, (uint32_t*)s->hist_series.p, (size_t)2 * kHistWords * n);
+    if ((rc = run_steps(s, n, rec_gi))) return rc;
+    buf.resize(kHistWords * (size_t)n);
+    HIPC(hipMemcpyAsync(buf.data(), s->hist_series.p, buf.size() * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+    for (int64_t i = 0; i < n; ++i)
+      for (int v = 0; v < 6; ++v) {
+        int64_t c = 0;
+        for (int r = 0; r < kHistReplicas; ++r) c += (int64_t)buf[kHistWords * i + 6 * r + v];
+        counts[6 * (i0 + i) + v] = c;
+      }
+  }
   return CVF_OK;
 }
 

@@ -86,7 +86,15 @@ void launch_sw_local(hipStream_t st, Launch ln, const uint32_t* words, uint32_t*
                      uint32_t jc);
 void launch_sw_union(hipStream_t st, Launch ln, uint32_t* parent, const uint8_t* cov);
 void launch_sw_flip(hipStream_t st, Launch ln, uint32_t* words, uint32_t* parent, uint32_t jc);
+// sigma histogram as kHistReplicas partial histograms of 6 counters
+// (kHistWords counters, zeroed by the caller) that the caller sums
+constexpr int kHistReplicas = 8;
+constexpr int kHistWords = 6 * kHistReplicas;
 void launch_sigma_hist(hipStream_t st, const uint32_t* W, uint32_t N, unsigned long long* counts);
+// the same into slot (*gstep + step_off - *base) of a per-step series of
+// `span` histograms (graph-replayable: the slot is read on the device)
+void launch_sigma_hist_slot(hipStream_t st, const uint32_t* W, uint32_t N, unsigned long long* series,
+                            const uint64_t* gstep, uint32_t step_off, const uint64_t* base, uint32_t span);
 // cluster statistics of one point (labels at `label`, 6N nodes)
 void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, int32_t* size);
 void launch_perc_slots(hipStream_t st, const uint32_t* label, uint32_t nn, const int32_t* size,

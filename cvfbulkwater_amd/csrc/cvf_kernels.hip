@@ -713,12 +713,27 @@ __global__ __launch_bounds__(256) void k_observe(const BlockEnt* __restrict__ ta
   }
 }
 
-// Counts of the six sigma values over the 6N arms of one point.
-__global__ void k_sigma_hist(const uint32_t* __restrict__ W, uint32_t N,
-                             unsigned long long* __restrict__ counts) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+// Counts of the six sigma values over the 6N arms of one point, as
+// kHistReplicas partial histograms (counts[r * 6 + s], r = workgroup mod
+// kHistReplicas) that the host sums: spreading the atomics keeps them from
+// queueing on six addresses.  With a slot table (series, gstep, base, span)
+// the counts go to the slot of the step just run, series + kHistWords
+// (gstep + step_off - base), read on the device so that a captured step
+// graph records every step it replays; slots outside [0, span) are not
+// written.
+__global__ __launch_bounds__(256) void k_sigma_hist(const uint32_t* __restrict__ W, uint32_t N,
+                                                    unsigned long long* __restrict__ counts,
+                                                    const uint64_t* __restrict__ gstep, uint32_t step_off,
+                                                    const uint64_t* __restrict__ base, uint32_t span) {
+  if (gstep) {
+    const uint64_t slot = *gstep + step_off - *base;
+    if (slot >= span) return;  // uniform over the grid
+    counts += kHistWords * slot;
+  }
+  counts += 6 * (blockIdx.x % kHistReplicas);
+  __shared__ uint32_t part[4][6];
   uint32_t c[6] = {0, 0, 0, 0, 0, 0};
-  if (t < N) {
+  for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < N; t += gridDim.x * 256) {
     const uint32_t w = W[t];
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
@@ -733,8 +748,12 @@ __global__ void k_sigma_hist(const uint32_t* __restrict__ W, uint32_t N,
     for (int m = 32; m >= 1; m >>= 1) c[s] += __shfl_xor(c[s], m, 64);
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
-    for (int s = 0; s < 6; ++s)
-      if (c[s]) atomicAdd(&counts[s], (unsigned long long)c[s]);
+    for (int s = 0; s < 6; ++s) part[threadIdx.x >> 6][s] = c[s];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const uint32_t v = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+    if (v) atomicAdd(&counts[threadIdx.x], (unsigned long long)v);
+  }
 }
 
 // --------------------------------------------------------- Swendsen-Wang
@@ -1041,8 +1060,15 @@ void launch_sw_flip(hipStream_t st, Launch ln, uint32_t* words, uint32_t* parent
   launch(k_sw_flip, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, ln.step_off, words,
          parent, jc);
 }
+// workgroups of the sigma histogram: about 4 cells per thread, at most 256
+static dim3 hist_grid(uint32_t N) { return dim3(std::max(1u, std::min(256u, (N + 1023) / 1024))); }
 void launch_sigma_hist(hipStream_t st, const uint32_t* W, uint32_t N, unsigned long long* counts) {
-  launch(k_sigma_hist, grid1(N, 256), dim3(256), st, W, N, counts);
+  launch(k_sigma_hist, hist_grid(N), dim3(256), st, W, N, counts, (const uint64_t*)nullptr, 0u,
+         (const uint64_t*)nullptr, 0u);
+}
+void launch_sigma_hist_slot(hipStream_t st, const uint32_t* W, uint32_t N, unsigned long long* series,
+                            const uint64_t* gstep, uint32_t step_off, const uint64_t* base, uint32_t span) {
+  launch(k_sigma_hist, hist_grid(N), dim3(256), st, W, N, series, gstep, step_off, base, span);
 }
 void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, int32_t* size) {
   launch(k_cluster_sizes, grid1(nn, 256), dim3(256), st, label, nn, size);

@@ -257,17 +257,20 @@ int main(int argc, char** argv) {
   int64_t pending = 0;
   const int interval = in.n_measure > 0 ? in.n_measure : 1;
   cvf_correlator* corr = in.flag_correlation == 1 ? cvf_corr_create() : nullptr;
+  std::vector<int64_t> hist;
   for (int i = 0; i < in.n_loops; ++i) {
     ++pending;
-    if (corr) {
-      check(cvf_step(sim, pending), "cvf_step");
+    if (corr && (i % interval == 0 || i == in.n_loops - 1)) {
+      // the steps since the last sample, each followed by its histogram
+      hist.resize(6 * (size_t)pending);
+      check(cvf_step_histograms(sim, 0, pending, hist.data()), "sigma histograms");
+      for (int64_t j = 0; j < pending; ++j) {
+        double mx = -2e10;
+        for (int k = 0; k < 6; ++k)
+          if (hist[6 * j + k] > mx) mx = (double)hist[6 * j + k];
+        cvf_corr_add(corr, mx / (double)(6 * (int64_t)N));
+      }
       pending = 0;
-      int64_t h[6];
-      check(cvf_sigma_histogram(sim, 0, h), "sigma histogram");
-      double mx = -2e10;
-      for (int k = 0; k < 6; ++k)
-        if (h[k] > mx) mx = (double)h[k];
-      cvf_corr_add(corr, mx / (double)(6 * (int64_t)N));
     }
     if (i % interval == 0) {
       check(cvf_step(sim, pending), "cvf_step");

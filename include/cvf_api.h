@@ -184,6 +184,13 @@ int cvf_profile_steps(cvf_sim *s, int64_t n_steps, double *kernel_ms, int64_t *l
  * correlator observable M = max_s n_s / 6N, gpu_water3D.cu:1254-1266). */
 int cvf_sigma_histogram(cvf_sim *s, int32_t pt, int64_t counts6[6]);
 
+/* Flag_correlation's per-step record (gpu_water3D.cu:1246-1274): advances
+ * n_steps MC steps (as cvf_step) and writes point pt's sigma histogram after
+ * each of them, counts[6 * i .. 6 * i + 5] after step i.  The histograms
+ * collect in device memory and come back in one copy per 4096 steps, so
+ * the steps run back to back instead of waiting on a host round trip each. */
+int cvf_step_histograms(cvf_sim *s, int32_t pt, int64_t n_steps, int64_t *counts);
+
 /* Multi-tau correlator of the Flag_correlation = 1 mode (gpu_water3D.cu:
  * 161-191, 2983-3128; Ramirez, Sinha, Bhatt, Harmandaris, Schmid, J. Chem.
  * Phys. 133, 154103 (2010)): 32 levels of 16 points, blocking by 2.

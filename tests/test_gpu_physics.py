@@ -227,6 +227,28 @@ def test_sigma_histogram_matches_config():
         assert np.array_equal(sim.sigma_histogram(i), np.bincount(s, minlength=6))
 
 
+@pytest.mark.parametrize("n_cluster", [0, 1])
+def test_step_histograms_equal_per_step_histograms(n_cluster):
+    # the device-recorded per-step histograms equal stepping one at a time and
+    # reading the histogram after each step, for every point of a batch and
+    # across the copy-back span (4096 steps)
+    pts = batch_points([8, 12], nc=n_cluster)
+    a, b = Simulation(pts), Simulation(pts)
+    n = 4100 if n_cluster == 0 else 40
+    rec = a.step_histograms(n, pt=1)
+    ref = []
+    for _ in range(n):
+        b.step(1)
+        ref.append(b.sigma_histogram(1))
+    assert np.array_equal(rec, np.array(ref))
+    assert np.all(rec.sum(axis=1) == 6 * 12 ** 3)
+    assert a.steps == b.steps == n
+    for i in range(2):
+        ca, cb = a.get_config(i), b.get_config(i)
+        assert np.array_equal(ca[0], cb[0]) and np.array_equal(ca[1], cb[1])
+    assert a.step_histograms(0).shape == (0, 6)
+
+
 def test_config5_precision_form_errors():
     """L=96, T=0.65, P=0.30 with SW, FP32-compensated vdW against FP64
     everywhere from the same seed: the lattice trajectory is identical (N_HB
