@@ -416,7 +416,7 @@ BlockEnt make_entry(const PointHost& P, int point, uint32_t start) {
 uint32_t slab_lds_bytes(uint32_t L, uint32_t T) { return (T + kChunk + 2) * L * L * 4; }
 
 // Planes per slab: the requested count if it divides L and fits, else the
-// largest divisor of L up to 8 that fits in LDS; 0 if none does.
+// largest divisor of L up to the cap that fits in LDS; 0 if none does.
 uint32_t choose_slab(uint32_t L, int requested, uint32_t cap = kMaxSlab) {
   if (requested > 0 && L % (uint32_t)requested == 0 &&
       slab_lds_bytes(L, (uint32_t)requested) <= chunk_lds_limit())
@@ -1405,7 +1405,7 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
   const cvf_tuning base = default_tuning(s);
   for (int streams : {1, 2, 4}) {
     if (streams > s->npts) continue;
-    for (int slab : {0, 4, 2, 1})
+    for (int slab : {0, 8, 4, 2, 1})
       for (int bc : {256, 512, 1024}) {
         cvf_tuning t = base;
         t.streams = streams;

@@ -52,7 +52,7 @@ constexpr uint32_t kEntNoChess = 1;
 
 constexpr int kChunk = 6;    // half-sweeps per fused kernel
 constexpr uint32_t kObserveSpan = 4096;  // cells per k_observe workgroup (256 threads)
-constexpr int kMaxSlab = 8;  // planes per slab of the fused kernel
+constexpr int kMaxSlab = 16;  // planes per slab of the fused kernel
 
 struct Launch {
   const BlockEnt* tab;
