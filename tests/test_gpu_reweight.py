@@ -220,3 +220,20 @@ def test_from_series_takes_h_over_n_and_n_over_v():
         for k in ("e_min", "rho_min", "de", "drho"):
             assert ia[k] == pytest.approx(ib[k], rel=1e-12)
         assert abs(ia["n_occupied"] - ib["n_occupied"]) <= 3
+
+
+def test_executable_recomputes_on_a_stale_constants_file(tmp_path):
+    # a Constants_L file for other simulations is reported, not used and not
+    # overwritten (f90:386-495); the results equal a run without the file
+    a, b = tmp_path / "fresh", tmp_path / "stale"
+    rw_synth.write_case(a, 8, ["0.1058", "0.106", "0.1062"], "0.5", 1000)
+    shutil.copytree(a, b)
+    stale = " 0.1 0.5 0\n 0.2 0.5 1.5\n 0.3 0.5 2.5\n"
+    (b / "Constants_L8").write_text(stale)
+    ra, rb = run_exe(a), run_exe(b)
+    assert ra.returncode == 0 and rb.returncode == 0, rb.stderr
+    assert "WARNING" in rb.stderr
+    assert (b / "Constants_L8").read_text() == stale
+    for n in sorted(p.name for p in a.iterdir()):
+        if n.startswith(("Binder", "Confront", "Histogram")):
+            assert (a / n).read_text() == (b / n).read_text(), n
