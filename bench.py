@@ -409,9 +409,10 @@ def reweight_samples(n_sim: int = 11, n: int = 100000, seed: int = 9):
 def reweight_form(sweeps: int = 300, cpu_sweeps: int = 2, with_cpu: bool = True) -> dict:
     """Histogram reweighting at the reference's production shape: 11
     simulations x 100000 samples, BIN_FCT 2 (3560 x 3560 grid), L = 8.
-    Times `sweeps` Jacobi sweeps of the constants (the program's STEP) and
-    one 64-point (T, P) scan reweighting, wall clock around the library
-    calls; the CPU oracle (the Fortran loops restated in C, serial) times a
+    Times `sweeps` Jacobi sweeps of the constants (the program's STEP), a
+    64-point x 3-angle scan with the projections reduced on the device, and
+    the same 64 densities reweighted and copied to the host, wall clock
+    around the library calls; the CPU oracle (the Fortran loops restated in C, serial) times a
     few sweeps of the same input."""
     from cvfbulkwater_amd.reweight import Reweighting
 
@@ -427,14 +428,21 @@ def reweight_form(sweeps: int = 300, cpu_sweeps: int = 2, with_cpu: bool = True)
     eng.reweight(TK[:4], PB[:4])
     t0 = time.perf_counter()
     eng.reweight(TK, PB)
+    copy_ms = (time.perf_counter() - t0) * 1e3
+    angles = [1.55, 1.57, 1.59]
+    eng.scan(TK[:4], PB[:4], angles)
+    t0 = time.perf_counter()
+    eng.scan(TK, PB, angles)
     scan_ms = (time.perf_counter() - t0) * 1e3
     eng.close()
     terms = info["n_occupied"] * len(T_hat) ** 2
     res = {"workload": "histogram reweighting: 11 simulations x 100000 (h, rho) samples, BIN_FCT 2, "
                        f"{info['n_bin2d']}^2 grid, {info['n_occupied']} occupied cells, {sweeps} constant "
                        "sweeps (wall clock)", "ms_per_sweep": sweep_ms, "converged": ok,
-           "terms_per_s": terms / (sweep_ms * 1e-3), "scan_64_points_ms": scan_ms,
-           "scan_densities_copied_mb": 64 * info["n_occupied"] * 8 / 2**20}
+           "terms_per_s": terms / (sweep_ms * 1e-3),
+           "scan_64_points_3_angles_ms": scan_ms,
+           "reweight_64_points_to_host_ms": copy_ms,
+           "densities_copied_mb": 64 * info["n_occupied"] * 8 / 2**20}
     if with_cpu:
         import numpy as np
         from oracle import rw as ORW

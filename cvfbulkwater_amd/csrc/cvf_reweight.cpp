@@ -1,8 +1,8 @@
 // Host side of the histogram reweighting (include/cvf_reweight.h):
 // sample conversion and binning, the occupied-cell list in HBM, the
-// constants' Jacobi sweeps and batched reweighting on the device, and the
-// order-parameter projections, cumulants and Ising scores over the
-// occupied cells.
+// constants' Jacobi sweeps and batched reweighting on the device, the
+// device-reduced scan, and the per-target projections, cumulants and Ising
+// scores over the occupied cells on the host.
 //
 // Reference: HistogramReweighting/histogram_reweighting_mixed_paramSI_norm_x.f90.
 // That program stores real*8 constants written as default-real literals
@@ -135,6 +135,59 @@ double ising_peak(double (*f)(double), bool by_x) {
   }
 }
 
+// The order-parameter histogram p of nm bins of width w from xmin, as
+// Calcula_Histograma_Rotado_normX has it after binning (f90:1466-1617):
+// bin centres into x, norm / mean / variance, the sine stretch, the same
+// with the stretched widths; stats3 = {media, a1, binder} (2580-2608).
+int finish_order_parameter(double xmin, double w, int nm, double* x, double* p, double* stats3) {
+  for (int i = 0; i < nm; ++i) x[i] = xmin + ((i + 1) - 0.5) * w;
+  const double media = standardize(x, p, nm, &w);
+  if (!standardized(x, p, nullptr, w, nm)) return set_error(CVF_ERANGE, "order-parameter moments off");
+  // sine stretch of the axis (f90:1530-1534), then norm / mean / variance
+  // again with the stretched bin widths (1538-1617)
+  const double b = sl(1.53f), pi = sl(3.14159f);
+  for (int i = 0; i < nm; ++i) x[i] = sl(1.0f) * b * b * sin(pi * x[i] / b) / (pi * pi) + sl(1.0f) * x[i];
+  std::vector<double> rx(nm);
+  rx[0] = x[1] - x[0];
+  for (int i = 1; i < nm - 1; ++i) rx[i] = (x[i + 1] - x[i - 1]) / 2.0;
+  rx[nm - 1] = x[nm - 1] - x[nm - 2];
+  double norma = 0;
+  for (int i = 0; i < nm; ++i) norma = norma + p[i] * rx[i];
+  for (int i = 0; i < nm; ++i) p[i] = p[i] / norma;
+  double mrx = 0;
+  for (int i = 0; i < nm; ++i) mrx = mrx + x[i] * p[i] * rx[i];
+  for (int i = 0; i < nm; ++i) x[i] = x[i] - mrx;
+  norma = 0;
+  double var = 0;
+  for (int i = 0; i < nm; ++i)
+    if (p[i] > 0) {
+      norma = norma + p[i] * rx[i];
+      var = var + (x[i] * x[i] * p[i] * rx[i]);
+    }
+  const double a1 = sqrt(norma / var), a2 = 1 / (norma * a1);
+  for (int i = 0; i < nm; ++i) {
+    x[i] = x[i] * a1;
+    rx[i] = rx[i] * a1;
+    p[i] = p[i] * a2;
+  }
+  if (!standardized(x, p, rx.data(), 0, nm)) return set_error(CVF_ERANGE, "stretched order-parameter moments off");
+  if (stats3) {
+    // binder (f90:2580-2608): cumulant of M = x / a1 + media
+    double s = 0;
+    for (int i = 0; i < nm; ++i) s = s + p[i];
+    double u2 = 0, u4 = 0;
+    for (int i = 0; i < nm; ++i) {
+      const double m = (x[i] / a1) + media, m2 = m * m, m4 = m2 * m2;
+      u2 = u2 + m2 * p[i] / s;
+      u4 = u4 + m4 * p[i] / s;
+    }
+    stats3[0] = media;
+    stats3[1] = a1;
+    stats3[2] = 1.0 - u4 / (3.0 * u2 * u2);
+  }
+  return CVF_OK;
+}
+
 }  // namespace
 
 struct cvf_reweight {
@@ -150,6 +203,7 @@ struct cvf_reweight {
 
   hipStream_t stream = nullptr;
   DMem<double> en, vn, lh, c0, c1, logp, pmax, mx, ps, lz, norm;
+  DMem<uint32_t> dcell;  // the occupied cells' grid indices, for the projections
   DMem<RwSim> sims;
   DMem<double2> tgt;
   int32_t cap_t = 0, n_t = 0;
@@ -158,6 +212,7 @@ struct cvf_reweight {
   int32_t nblk() const { return (int32_t)((cell.size() + kRwBlock - 1) / kRwBlock); }
   RwCells cells() const { return RwCells{en.p, vn.p, lh.p, (int64_t)cell.size()}; }
   RwWork work() const { return RwWork{logp.p, pmax.p, mx.p, ps.p, lz.p, norm.p, nblk()}; }
+  RwGeom geom() const { return RwGeom{dcell.p, (int64_t)cell.size(), nb, nm, e_min, de, rho_min, drho}; }
 
   int reserve_targets(int32_t n) {
     if (n <= cap_t) return CVF_OK;
@@ -307,6 +362,11 @@ int cvf_rw_create(int32_t L, int32_t n_sim, const double* T_hat, const double* P
   HIPR(rw->c0.alloc(n_sim));
   HIPR(rw->c1.alloc(n_sim));
   HIPR(rw->sims.alloc(n_sim));
+  HIPR(rw->dcell.alloc(nc));
+  {
+    std::vector<uint32_t> c32(rw->cell.begin(), rw->cell.end());  // nb^2 < 2^31
+    HIPR(hipMemcpy(rw->dcell.p, c32.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
   std::vector<RwSim> hs(n_sim);
   for (int j = 0; j < n_sim; ++j) hs[j] = RwSim{1000. / (sl(0.2293f) * rw->T[j]), rw->P[j] * 100.0, rw->log_n[j], 0};
   HIPR(hipMemcpy(rw->en.p, hen.data(), nc * sizeof(double), hipMemcpyHostToDevice));
@@ -469,50 +529,92 @@ int cvf_rw_order_parameter(const cvf_reweight* rw, int32_t t, double alpha, doub
     else
       return set_error(CVF_ERANGE, "order parameter %g outside the histogram", xc[k]);
   }
-  for (int i = 0; i < nm; ++i) x[i] = xmin + ((i + 1) - 0.5) * w;
-  const double media = standardize(x, p, nm, &w);
-  if (!standardized(x, p, nullptr, w, nm)) return set_error(CVF_ERANGE, "order-parameter moments off");
-  // sine stretch of the axis (f90:1530-1534), then norm / mean / variance
-  // again with the stretched bin widths (1538-1617)
-  const double b = sl(1.53f), pi = sl(3.14159f);
-  for (int i = 0; i < nm; ++i) x[i] = sl(1.0f) * b * b * sin(pi * x[i] / b) / (pi * pi) + sl(1.0f) * x[i];
-  std::vector<double> rx(nm);
-  rx[0] = x[1] - x[0];
-  for (int i = 1; i < nm - 1; ++i) rx[i] = (x[i + 1] - x[i - 1]) / 2.0;
-  rx[nm - 1] = x[nm - 1] - x[nm - 2];
-  double norma = 0;
-  for (int i = 0; i < nm; ++i) norma = norma + p[i] * rx[i];
-  for (int i = 0; i < nm; ++i) p[i] = p[i] / norma;
-  double mrx = 0;
-  for (int i = 0; i < nm; ++i) mrx = mrx + x[i] * p[i] * rx[i];
-  for (int i = 0; i < nm; ++i) x[i] = x[i] - mrx;
-  norma = 0;
-  double var = 0;
-  for (int i = 0; i < nm; ++i)
-    if (p[i] > 0) {
-      norma = norma + p[i] * rx[i];
-      var = var + (x[i] * x[i] * p[i] * rx[i]);
-    }
-  const double a1 = sqrt(norma / var), a2 = 1 / (norma * a1);
-  for (int i = 0; i < nm; ++i) {
-    x[i] = x[i] * a1;
-    rx[i] = rx[i] * a1;
-    p[i] = p[i] * a2;
+  return finish_order_parameter(xmin, w, nm, x, p, stats3);
+}
+
+int cvf_rw_scan(cvf_reweight* rw, int32_t n_targets, const double* T_K, const double* P_bar, int32_t n_angles,
+                const double* alpha, double* binder_ed, double* x, double* p, double* stats3, double* scores6) {
+  if (!rw || !T_K || !P_bar || !alpha || !binder_ed || !x || !p || !stats3 || !scores6)
+    return set_error(CVF_EINVAL, "null argument");
+  if (n_targets < 1 || n_angles < 1 || (int64_t)n_targets * n_angles > 4096)
+    return set_error(CVF_EINVAL, "n_targets x n_angles = %d x %d (1 .. 4096 pairs)", n_targets, n_angles);
+  if (!rw->have_ctes) return set_error(CVF_EINVAL, "the reweighting constants are not set");
+  const int nm = rw->nm, nta = n_targets * n_angles;
+  if (nm < 3) return set_error(CVF_EINVAL, "n_bin_mixed = %d is too small", nm);
+  for (int t = 0; t < n_targets; ++t)
+    if (!(T_K[t] > 0)) return set_error(CVF_EINVAL, "target %d: T = %g K", t, T_K[t]);
+  HIPR(hipSetDevice(rw->device));
+  if (int rc = rw->reserve_targets(n_targets)) return rc;
+  // densities of every target, left on the device
+  std::vector<double2> bp(n_targets);
+  for (int t = 0; t < n_targets; ++t) bp[t] = double2{1000. / (sl(0.2293f) * T_K[t]), P_bar[t] * 100.0};
+  HIPR(hipMemcpyAsync(rw->tgt.p, bp.data(), bp.size() * sizeof(double2), hipMemcpyHostToDevice, rw->stream));
+  HIPR(hipMemcpyAsync(rw->c0.p, rw->ctes.data(), rw->n_sim * sizeof(double), hipMemcpyHostToDevice, rw->stream));
+  HIPR(launch_rw_pass(rw->cells(), rw->sims.p, rw->c0.p, rw->n_sim, RwTargets{rw->tgt.p, n_targets}, rw->work(),
+                      rw->de, rw->drho, nullptr, true, rw->stream));
+  rw->n_t = 0;  // the host copy of the densities no longer matches the device
+  const RwGeom g = rw->geom();
+  std::vector<double2> sc(n_angles);
+  for (int a = 0; a < n_angles; ++a) sc[a] = double2{sin(alpha[a]), cos(alpha[a])};  // as Rotate's sin / cos
+  DMem<double2> dsc, dbin;
+  DMem<double> pm, px, ph;
+  DMem<uint32_t> derr;
+  const int B = kRwProjBlocks;
+  HIPR(dsc.alloc(n_angles));
+  HIPR(dbin.alloc(nta));
+  HIPR(pm.alloc((size_t)n_targets * B * 5));
+  HIPR(px.alloc((size_t)nta * B * 2));
+  HIPR(ph.alloc((size_t)nta * B * nm));
+  HIPR(derr.alloc(1));
+  HIPR(hipMemcpyAsync(dsc.p, sc.data(), sc.size() * sizeof(double2), hipMemcpyHostToDevice, rw->stream));
+  HIPR(launch_rw_moments(g, rw->logp.p, n_targets, pm.p, rw->stream));
+  HIPR(launch_rw_xrange(g, rw->logp.p, n_targets, n_angles, dsc.p, px.p, rw->stream));
+  std::vector<double> hm((size_t)n_targets * B * 5), hx((size_t)nta * B * 2);
+  HIPR(hipMemcpyAsync(hm.data(), pm.p, hm.size() * sizeof(double), hipMemcpyDeviceToHost, rw->stream));
+  HIPR(hipMemcpyAsync(hx.data(), px.p, hx.size() * sizeof(double), hipMemcpyDeviceToHost, rw->stream));
+  HIPR(hipStreamSynchronize(rw->stream));
+  // Binder cumulants of the marginals: <x^2> = sum v x^2 / sum v
+  for (int t = 0; t < n_targets; ++t) {
+    double m[5] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < B; ++b)
+      for (int k = 0; k < 5; ++k) m[k] += hm[((size_t)t * B + b) * 5 + k];
+    binder_ed[2 * t] = 1.0 - (m[2] / m[0]) / (3.0 * (m[1] / m[0]) * (m[1] / m[0]));
+    binder_ed[2 * t + 1] = 1.0 - (m[4] / m[0]) / (3.0 * (m[3] / m[0]) * (m[3] / m[0]));
   }
-  if (!standardized(x, p, rx.data(), 0, nm)) return set_error(CVF_ERANGE, "stretched order-parameter moments off");
-  if (stats3) {
-    // binder (f90:2580-2608): cumulant of M = x / a1 + media
-    double s = 0;
-    for (int i = 0; i < nm; ++i) s = s + p[i];
-    double u2 = 0, u4 = 0;
-    for (int i = 0; i < nm; ++i) {
-      const double m = (x[i] / a1) + media, m2 = m * m, m4 = m2 * m2;
-      u2 = u2 + m2 * p[i] / s;
-      u4 = u4 + m4 * p[i] / s;
+  // bins: the range over the occupied cells, two empty bins each side
+  std::vector<double2> bins(nta);
+  for (int ta = 0; ta < nta; ++ta) {
+    double xmin = sl(1E12f), xmax = sl(-1E12f);
+    for (int b = 0; b < B; ++b) {
+      xmin = std::min(xmin, hx[2 * ((size_t)ta * B + b)]);
+      xmax = std::max(xmax, hx[2 * ((size_t)ta * B + b) + 1]);
     }
-    stats3[0] = media;
-    stats3[1] = a1;
-    stats3[2] = 1.0 - u4 / (3.0 * u2 * u2);
+    if (!(xmax >= xmin)) return set_error(CVF_ERANGE, "target %d: no cell of positive density", ta / n_angles);
+    double range = xmax - xmin, w = range / nm;
+    range = range + 4 * w;
+    xmin = xmin - 2 * w;
+    w = range / nm;
+    bins[ta] = double2{xmin, w};
+  }
+  HIPR(hipMemcpyAsync(dbin.p, bins.data(), bins.size() * sizeof(double2), hipMemcpyHostToDevice, rw->stream));
+  HIPR(hipMemsetAsync(derr.p, 0, sizeof(uint32_t), rw->stream));
+  HIPR(launch_rw_xhist(g, rw->logp.p, n_targets, n_angles, dsc.p, dbin.p, ph.p, derr.p, rw->stream));
+  std::vector<double> hh((size_t)nta * B * nm);
+  uint32_t err = 0;
+  HIPR(hipMemcpyAsync(hh.data(), ph.p, hh.size() * sizeof(double), hipMemcpyDeviceToHost, rw->stream));
+  HIPR(hipMemcpyAsync(&err, derr.p, sizeof err, hipMemcpyDeviceToHost, rw->stream));
+  HIPR(hipStreamSynchronize(rw->stream));
+  if (err) return set_error(CVF_ERANGE, "an order parameter fell outside its histogram");
+  for (int ta = 0; ta < nta; ++ta) {
+    double* xo = x + (size_t)ta * nm;
+    double* po = p + (size_t)ta * nm;
+    for (int k = 0; k < nm; ++k) {
+      double v = 0;
+      for (int b = 0; b < B; ++b) v += hh[((size_t)ta * B + b) * nm + k];
+      po[k] = v;
+    }
+    if (int rc = finish_order_parameter(bins[ta].x, bins[ta].y, nm, xo, po, stats3 + 3 * ta)) return rc;
+    if (int rc = cvf_rw_ising_scores(xo, po, nm, scores6 + 6 * ta)) return rc;
   }
   return CVF_OK;
 }

@@ -17,8 +17,9 @@
 // and on stderr the constants and the five minima (Liu 2D, Liu 3D, Zero,
 // Height, Zero + Height) with their S, T, P and B.
 //
-// The scan's (T, P) points are reweighted as one batch on the GPU.  Numbers
-// in the files are written with 17 significant digits.
+// The scan's (T, P) points are reweighted as one batch on the GPU, with
+// the cumulants and order-parameter histograms reduced there.  Numbers in
+// the files are written with 17 significant digits.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -211,27 +212,40 @@ int main(int argc, char** argv) {
   std::vector<double> x(nm), p(nm);
   Best best[5];  // liu_2D, liu_3D, zero, height, zero + height
   double alpha = in.Smin;
-  // targets per launch: about 2 GiB of densities
-  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t(1) << 28) / info.n_occupied));
+  // the angles of every scan point (f90:990-995)
+  std::vector<double> angles;
+  for (double S = in.Smin; S < in.Smax; S = S + in.dS) angles.push_back(S);
+  const int na = (int)angles.size();
+  if (na > 4096) return fprintf(stderr, "more than 4096 angles\n"), 2;
+  // targets per pass: at most 4096 (target, angle) pairs and about 2 GiB of
+  // densities, reduced on the device
+  const int64_t chunk = std::max<int64_t>(
+      1, std::min<int64_t>(4096 / na, (int64_t(1) << 28) / info.n_occupied));
+  std::vector<double> ued, xs, ps, sts, scs;
   for (size_t t0 = 0; t0 < TK.size(); t0 += chunk) {
     const int nt = (int)std::min<size_t>(chunk, TK.size() - t0);
-    if (cvf_rw_reweight(rw, nt, TK.data() + t0, PB.data() + t0, nullptr) != 0) die("cvf_rw_reweight", 2);
+    ued.resize(2 * (size_t)nt);
+    xs.resize((size_t)nt * na * nm);
+    ps.resize(xs.size());
+    sts.resize((size_t)nt * na * 3);
+    scs.resize((size_t)nt * na * 6);
+    if (cvf_rw_scan(rw, nt, TK.data() + t0, PB.data() + t0, na, angles.data(), ued.data(), xs.data(), ps.data(),
+                    sts.data(), scs.data()) != 0)
+      die("scan", 255);
     for (int t = 0; t < nt; ++t) {
       const double Tk = TK[t0 + t], Pb = PB[t0 + t];
-      double ue, ur;
-      cvf_rw_binder_ed(rw, t, &ue, &ur);
-      row(f21, {Tk, Pb, ue, ur});
-      for (double S = in.Smin; S < in.Smax; S = S + in.dS) {
+      row(f21, {Tk, Pb, ued[2 * t], ued[2 * t + 1]});
+      for (int a = 0; a < na; ++a) {
+        const size_t ta = (size_t)t * na + a;
+        const double S = angles[a];
         alpha = S;
-        double st[3];
-        if (cvf_rw_order_parameter(rw, t, alpha, x.data(), p.data(), st) != 0) die("order parameter", 255);
-        row(f20, {Tk, Pb, S, st[2]});
+        row(f20, {Tk, Pb, S, sts[3 * ta + 2]});
         // the reference's scale loop over B runs once, at B = 1 (f90:800-852)
         for (double B = 1; B < sl(1.001f); B = B + sl(0.01f)) {
           std::vector<double> xb(nm);
-          for (int i = 0; i < nm; ++i) xb[i] = B * x[i];
+          for (int i = 0; i < nm; ++i) xb[i] = B * xs[ta * nm + i];
           double sc[6];
-          cvf_rw_ising_scores(xb.data(), p.data(), nm, sc);
+          cvf_rw_ising_scores(xb.data(), ps.data() + ta * nm, nm, sc);
           row(f12, {Tk, Pb, S, sc[3]});
           row(f13, {Tk, Pb, S, sc[1]});
           const double vals[5] = {sc[2], sc[3], sc[4], sc[5], sc[4] + sc[5]};

@@ -46,4 +46,33 @@ hipError_t launch_rw_pass(const RwCells& cells, const RwSim* sims, const double*
                           const RwTargets& tg, const RwWork& w, double de, double drho, double* ctes_out,
                           bool dist, hipStream_t stream);
 
+// Grid geometry for the projections: cell = i * nb + w, energy
+// e_min + (i + 1/2) de (kJ/mol), density rho_min + (w + 1/2) drho (kg/m^3).
+struct RwGeom {
+  const uint32_t* cell;
+  int64_t n;
+  int32_t nb, nm;
+  double e_min, de, rho_min, drho;
+};
+
+constexpr int kRwProjBlocks = 64;  // workgroups per target (or target x angle) of a projection
+
+// Per target t (density dens[t][cell]): partial sums over each workgroup's
+// cells of {v, v e^2, v e^4, v rho^2, v rho^4} -> part[t][block][5]
+// (kRwProjBlocks blocks), for the Binder cumulants of the marginals.
+hipError_t launch_rw_moments(const RwGeom& g, const double* dens, int32_t n_t, double* part, hipStream_t st);
+
+// Per (target, angle) pair ta = t * n_a + a: the min and max of
+// x = e sin(a) + rho cos(a) over the cells with density > 0 ->
+// part[ta][block][2].  sc holds (sin, cos) per angle.
+hipError_t launch_rw_xrange(const RwGeom& g, const double* dens, int32_t n_t, int32_t n_a, const double2* sc,
+                            double* part, hipStream_t st);
+
+// Per pair ta, the histogram of x over nm bins of width w from x0
+// (binfo[ta] = (x0, w)), weighted by the density: part[ta][block][nm],
+// summed in a fixed order (reproducible).  A cell with density > 0 outside
+// the bins (and not within 1e-10 of x0, which goes to bin 0) sets *err.
+hipError_t launch_rw_xhist(const RwGeom& g, const double* dens, int32_t n_t, int32_t n_a, const double2* sc,
+                           const double2* binfo, double* part, uint32_t* err, hipStream_t st);
+
 }  // namespace cvf

@@ -181,7 +181,141 @@ __global__ __launch_bounds__(kRwBlock) void k_rw_dist(double* __restrict__ logp,
 
 int32_t rw_sum_blocks(int32_t nblk) { return nblk < kSumBlocks ? nblk : kSumBlocks; }
 
+// ------------------------------------------------------------ projections
+// Each workgroup of a projection takes one contiguous range of cells, so
+// every sum below runs in an order fixed by the launch shape alone.
+__device__ inline void block_range(int64_t n, int64_t& lo, int64_t& hi) {
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  lo = (int64_t)blockIdx.x * per;
+  hi = lo + per < n ? lo + per : n;
+}
+
+__device__ inline void cell_er(const RwGeom& g, uint32_t c, double& e, double& r) {
+  const uint32_t i = c / (uint32_t)g.nb, w = c - i * (uint32_t)g.nb;
+  e = g.e_min + ((int)(i + 1) - 0.5) * g.de;
+  r = g.rho_min + ((int)(w + 1) - 0.5) * g.drho;
+}
+
+__global__ __launch_bounds__(kRwBlock) void k_rw_moments(RwGeom g, const double* __restrict__ dens,
+                                                         double* __restrict__ part) {
+  __shared__ double red[kRwBlock / 64];
+  const int t = blockIdx.y;
+  const double* v = dens + (int64_t)t * g.n;
+  int64_t lo, hi;
+  block_range(g.n, lo, hi);
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int64_t b = lo + threadIdx.x; b < hi; b += kRwBlock) {
+    double e, r;
+    cell_er(g, g.cell[b], e, r);
+    const double x = v[b], e2 = e * e, r2 = r * r;
+    s[0] += x;
+    s[1] += e2 * x;
+    s[2] += e2 * e2 * x;
+    s[3] += r2 * x;
+    s[4] += r2 * r2 * x;
+  }
+  for (int k = 0; k < 5; ++k) {
+    const double m = block_reduce<false>(s[k], red);
+    if (threadIdx.x == 0) part[((int64_t)t * gridDim.x + blockIdx.x) * 5 + k] = m;
+  }
+}
+
+__global__ __launch_bounds__(kRwBlock) void k_rw_xrange(RwGeom g, const double* __restrict__ dens, int32_t n_a,
+                                                        const double2* __restrict__ sc,
+                                                        double* __restrict__ part) {
+  __shared__ double red[kRwBlock / 64];
+  const int ta = blockIdx.y, t = ta / n_a, a = ta - t * n_a;
+  const double* v = dens + (int64_t)t * g.n;
+  const double2 q = sc[a];
+  int64_t lo, hi;
+  block_range(g.n, lo, hi);
+  double mn = INFINITY, mx = -INFINITY;
+  for (int64_t b = lo + threadIdx.x; b < hi; b += kRwBlock) {
+    if (!(v[b] > 0)) continue;
+    double e, r;
+    cell_er(g, g.cell[b], e, r);
+    const double x = e * q.x + r * q.y;
+    mn = fmin(mn, x);
+    mx = fmax(mx, x);
+  }
+  mn = -block_reduce<true>(-mn, red);
+  mx = block_reduce<true>(mx, red);
+  if (threadIdx.x == 0) {
+    part[2 * ((int64_t)ta * gridDim.x + blockIdx.x)] = mn;
+    part[2 * ((int64_t)ta * gridDim.x + blockIdx.x) + 1] = mx;
+  }
+}
+
+// Each wavefront walks its share of the workgroup's range 64 cells at a
+// time; the lanes of a chunk that fall in one bin are summed by a fixed
+// butterfly and added to the wavefront's own LDS histogram by lane 0, bin
+// after bin in order of first appearance.  Neighbouring cells share bins,
+// so a chunk holds few distinct bins.
+__global__ __launch_bounds__(kRwBlock) void k_rw_xhist(RwGeom g, const double* __restrict__ dens, int32_t n_a,
+                                                       const double2* __restrict__ sc,
+                                                       const double2* __restrict__ binfo,
+                                                       double* __restrict__ part, uint32_t* __restrict__ err) {
+  extern __shared__ double hw[];  // [4 waves][nm]
+  const int ta = blockIdx.y, t = ta / n_a, a = ta - t * n_a;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nm = g.nm;
+  for (int k = threadIdx.x; k < 4 * nm; k += kRwBlock) hw[k] = 0;
+  __syncthreads();
+  const double* v = dens + (int64_t)t * g.n;
+  const double2 q = sc[a], bi = binfo[ta];
+  int64_t lo, hi;
+  block_range(g.n, lo, hi);
+  const int64_t per = (hi - lo + 3) / 4;
+  const int64_t wlo = lo + wave * per, whi = wlo + per < hi ? wlo + per : hi;
+  double* h = hw + wave * nm;
+  for (int64_t c0 = wlo; c0 < whi; c0 += 64) {
+    const int64_t b = c0 + lane;
+    int k = -1;
+    double x = 0;
+    if (b < whi && v[b] > 0) {
+      double e, r;
+      cell_er(g, g.cell[b], e, r);
+      const double X = e * q.x + r * q.y, d = (X - bi.x) / bi.y;
+      x = v[b];
+      if (fabs(d) < 1073741824.0 && (int)d >= 0 && (int)d < nm)
+        k = (int)d;
+      else if (fabs(X - bi.x) < (double)1E-10f)
+        k = 0;
+      else
+        atomicOr(err, 1u);
+    }
+    uint64_t left = __ballot(k >= 0);
+    while (left) {
+      const int k0 = __shfl(k, __ffsll((unsigned long long)left) - 1);
+      const double s = wave_sum(k == k0 ? x : 0.0);
+      if (lane == 0) h[k0] += s;
+      left &= ~__ballot(k == k0);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nm; k += kRwBlock)
+    part[((int64_t)ta * gridDim.x + blockIdx.x) * nm + k] = hw[k] + hw[nm + k] + hw[2 * nm + k] + hw[3 * nm + k];
+}
+
 }  // namespace
+
+hipError_t launch_rw_moments(const RwGeom& g, const double* dens, int32_t n_t, double* part, hipStream_t st) {
+  hipLaunchKernelGGL(k_rw_moments, dim3(kRwProjBlocks, (unsigned)n_t), dim3(kRwBlock), 0, st, g, dens, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_rw_xrange(const RwGeom& g, const double* dens, int32_t n_t, int32_t n_a, const double2* sc,
+                            double* part, hipStream_t st) {
+  hipLaunchKernelGGL(k_rw_xrange, dim3(kRwProjBlocks, (unsigned)(n_t * n_a)), dim3(kRwBlock), 0, st, g, dens, n_a,
+                     sc, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_rw_xhist(const RwGeom& g, const double* dens, int32_t n_t, int32_t n_a, const double2* sc,
+                           const double2* binfo, double* part, uint32_t* err, hipStream_t st) {
+  hipLaunchKernelGGL(k_rw_xhist, dim3(kRwProjBlocks, (unsigned)(n_t * n_a)), dim3(kRwBlock),
+                     sizeof(double) * 4 * (size_t)g.nm, st, g, dens, n_a, sc, binfo, part, err);
+  return hipGetLastError();
+}
 
 hipError_t launch_rw_pass(const RwCells& cells, const RwSim* sims, const double* ctes, int32_t n_sim,
                           const RwTargets& tg, const RwWork& w, double de, double drho, double* ctes_out,

@@ -50,6 +50,7 @@ _SIGS = {
     "cvf_rw_ising2d": (C.c_double, [C.c_double]),
     "cvf_rw_ising3d": (C.c_double, [C.c_double]),
     "cvf_rw_errors": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, DP, DP]),
+    "cvf_rw_scan": (C.c_int, [C.c_void_p, C.c_int32, DP, DP, C.c_int32, DP, DP, DP, DP, DP, DP]),
 }
 
 
@@ -165,6 +166,20 @@ class Reweighting:
         x, p, st = np.zeros(nm), np.zeros(nm), np.zeros(3)
         _check(_lib().cvf_rw_order_parameter(self._h, t, alpha, _d(x), _d(p), _d(st)), "cvf_rw_order_parameter")
         return x, p, tuple(st)
+
+    def scan(self, T_K, P_bar, alpha) -> dict:
+        """The scan over (T K, P bar) targets and angles with the projections
+        reduced on the device: binder_ed (n_t, 2); x, p (n_t, n_a, nm);
+        stats (n_t, n_a, 3) = {media, a1, binder}; scores (n_t, n_a, 6)."""
+        T_K, P_bar = _f64(np.atleast_1d(T_K)), _f64(np.atleast_1d(P_bar))
+        alpha = _f64(np.atleast_1d(alpha))
+        nt, na, nm = len(T_K), len(alpha), self.info()["n_bin_mixed"]
+        out = {"binder_ed": np.zeros((nt, 2)), "x": np.zeros((nt, na, nm)), "p": np.zeros((nt, na, nm)),
+               "stats": np.zeros((nt, na, 3)), "scores": np.zeros((nt, na, 6))}
+        _check(_lib().cvf_rw_scan(self._h, nt, _d(T_K), _d(P_bar), na, _d(alpha), _d(out["binder_ed"]),
+                                  _d(out["x"]), _d(out["p"]), _d(out["stats"]), _d(out["scores"])), "cvf_rw_scan")
+        self.n_targets = 0
+        return out
 
     def errors(self, t: int, sim: int, alpha: float) -> tuple[np.ndarray, np.ndarray]:
         nm = self.info()["n_bin_mixed"]

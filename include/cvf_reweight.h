@@ -10,10 +10,11 @@
  * x = e sin(a) + rho cos(a) and compares it with the 3D Ising
  * distribution.  Here the occupied cells live in HBM as a list, the
  * per-cell log-sum over simulations runs as one HIP launch over all cells
- * and all targets (the constants' sweep or a batch of scan points), and
- * the projections, cumulants and Ising scores run on the host over the
- * occupied cells.  The cvf_reweight executable drives these calls over the
- * reference's `input` file and data_L* files.
+ * and all targets (the constants' sweep or a batch of scan points), and a
+ * scan's cumulants and order-parameter histograms are reduced on the device
+ * (cvf_rw_scan); the per-target calls do the same on the host from a copy
+ * of the densities.  The cvf_reweight executable drives these calls over
+ * the reference's `input` file and data_L* files.
  *
  * Units follow the Fortran program: T_hat, P_hat internal; T in K, P in
  * bar; e in kJ/mol; rho in kg/m^3.  All functions return 0 (or a count)
@@ -86,6 +87,18 @@ int cvf_rw_order_parameter(const cvf_reweight *rw, int32_t t, double alpha, doub
  * HeightTest; f90:2042-2182): out6 = {kl_2D, kl_3D, liu_2D, liu_3D, zero,
  * height}. */
 int cvf_rw_ising_scores(const double *x, const double *p, int32_t n, double *out6);
+/* The scan of f90:928-1001 for n_targets (T K, P bar) points and n_angles
+ * angles (n_targets x n_angles <= 4096) in one pass, projections on the
+ * device: binder_ed[2t .. 2t+1] = the energy and density cumulants of
+ * target t; for the pair ta = t * n_angles + a, x and p (n_bin_mixed each,
+ * at ta * n_bin_mixed), stats3[3 ta ..] = {media, a1, binder} and
+ * scores6[6 ta ..] as cvf_rw_order_parameter and cvf_rw_ising_scores give
+ * them.  Sums run in an order fixed by the launch shape (reproducible run
+ * to run).  The densities stay on the device: call cvf_rw_reweight before
+ * the per-target functions. */
+int cvf_rw_scan(cvf_reweight *rw, int32_t n_targets, const double *T_K, const double *P_bar, int32_t n_angles,
+                const double *alpha, double *binder_ed, double *x, double *p, double *stats3, double *scores6);
+
 /* Universal order-parameter distributions (f90:2218-2254). */
 double cvf_rw_ising2d(double z);
 double cvf_rw_ising3d(double z);

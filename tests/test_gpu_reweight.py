@@ -237,3 +237,31 @@ def test_executable_recomputes_on_a_stale_constants_file(tmp_path):
     for n in sorted(p.name for p in a.iterdir()):
         if n.startswith(("Binder", "Confront", "Histogram")):
             assert (a / n).read_text() == (b / n).read_text(), n
+
+
+def test_device_scan_matches_oracle_and_host_path(pair):
+    ora, eng, c_ora, *_ = pair
+    eng.set_constants(c_ora)
+    T, P = targets()
+    angles = [0.4, 1.55, 1.5707, 1.6]
+    out = eng.scan(T, P, angles)
+    again = eng.scan(T, P, angles)
+    for k in out:  # fixed-order sums: identical run to run
+        assert np.array_equal(out[k], again[k]), k
+    p2, p3 = rw.peaks()
+    eng.reweight(T, P)
+    for t in range(len(T)):
+        d_ora = ora.distribution(c_ora, T[t], P[t])
+        assert out["binder_ed"][t] == pytest.approx(ora.binder_ed(d_ora), rel=RTOL)
+        for a, alpha in enumerate(angles):
+            xo, po, mo, ao, rc = ora.normx(d_ora, alpha)
+            assert rc == 0
+            np.testing.assert_allclose(out["x"][t, a], xo, rtol=RTOL, atol=1e-12)
+            np.testing.assert_allclose(out["p"][t, a], po, rtol=RTOL, atol=1e-12)
+            assert tuple(out["stats"][t, a, :2]) == pytest.approx((mo, ao), rel=RTOL)
+            assert out["stats"][t, a, 2] == pytest.approx(rw.binder(xo, po, mo, ao), rel=1e-8)
+            np.testing.assert_allclose(out["scores"][t, a], rw.scores(xo, po, p2, p3), rtol=1e-8, atol=1e-12)
+            # and the host path on the copied densities
+            x, p, st = eng.order_parameter(t, alpha)
+            np.testing.assert_allclose(out["x"][t, a], x, rtol=RTOL, atol=1e-12)
+            np.testing.assert_allclose(out["p"][t, a], p, rtol=RTOL, atol=1e-12)
