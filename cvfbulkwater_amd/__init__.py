@@ -25,7 +25,7 @@ __all__ = [
     "CVFError", "Params", "Obs", "ClusterStats", "Simulation", "load_library",
     "FP64", "FP32_COMPENSATED", "VOLUME_HOST", "VOLUME_DEVICE", "vdw_energy",
     "LIB_PATH", "EXE_PATH", "THERMO_EXE_PATH", "Correlator", "KT_NOT_VALUE", "thermo_averages",
-    "thermo_derivative", "thermo_kt", "REWEIGHT_EXE_PATH",
+    "thermo_derivative", "thermo_kt", "REWEIGHT_EXE_PATH", "SCHED_REFERENCE", "SCHED_FAST",
 ]
 
 PKG = Path(__file__).resolve().parent
@@ -36,6 +36,7 @@ REWEIGHT_EXE_PATH = PKG / "cvf_reweight"
 
 FP64, FP32_COMPENSATED = 0, 1
 VOLUME_HOST, VOLUME_DEVICE = 0, 1
+SCHED_REFERENCE, SCHED_FAST = 0, 1
 
 
 class CVFError(RuntimeError):
@@ -48,7 +49,7 @@ class _Params(C.Structure):
         ("n_metropolis", C.c_int32), ("n_cluster", C.c_int32), ("r_cutoff", C.c_float),
         ("flag_config", C.c_int32), ("flag_chessboard", C.c_int32), ("device_id", C.c_int32),
         ("J_hb", C.c_float), ("J_sig", C.c_float), ("v_hb", C.c_float),
-        ("precision", C.c_int32), ("volume_mode", C.c_int32),
+        ("precision", C.c_int32), ("volume_mode", C.c_int32), ("schedule", C.c_int32),
     ]
 
 
@@ -91,11 +92,12 @@ class Params:
     v_hb: float = 0.6
     precision: int = FP64
     volume_mode: int = VOLUME_HOST
+    schedule: int = SCHED_REFERENCE
 
     def _c(self) -> _Params:
         return _Params(self.L, self.P, self.T, self.seed, self.n_metropolis, self.n_cluster,
                        self.r_cutoff, self.flag_config, self.flag_chessboard, self.device_id,
-                       self.J_hb, self.J_sig, self.v_hb, self.precision, self.volume_mode)
+                       self.J_hb, self.J_sig, self.v_hb, self.precision, self.volume_mode, self.schedule)
 
 
 @dataclass

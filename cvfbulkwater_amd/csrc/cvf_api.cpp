@@ -139,7 +139,7 @@ struct cvf_sim {
   hipStream_t stream = nullptr;  // main stream: uploads, observables, statistics
   hipEvent_t start_ev = nullptr;
   int npts = 0;
-  int nm = 1, nc = 0, precision = CVF_FP64, vol_device = 0;
+  int nm = 1, nc = 0, precision = CVF_FP64, vol_device = 0, sched = CVF_SCHED_REFERENCE;
   std::vector<PointHost> ph;
   size_t total_cells = 0, total_nodes = 0;
   cvf_tuning tune{};
@@ -197,6 +197,11 @@ int validate(const cvf_params& p) {
   if (p.volume_mode != CVF_VOLUME_HOST && p.volume_mode != CVF_VOLUME_DEVICE)
     return fail(CVF_EINVAL, "unknown volume mode %d", p.volume_mode);
   if (!(p.r_cutoff > 0.f)) return fail(CVF_EINVAL, "R_cutoff must be > 0");
+  if (p.schedule != CVF_SCHED_REFERENCE && p.schedule != CVF_SCHED_FAST)
+    return fail(CVF_EINVAL, "unknown schedule %d", p.schedule);
+  // a colour pass would let two cells move one neighbour's counter at once
+  if (p.schedule == CVF_SCHED_FAST && !p.flag_chessboard)
+    return fail(CVF_EINVAL, "the fast schedule needs flag_chessboard = 1");
   return CVF_OK;
 }
 
@@ -541,8 +546,9 @@ int build_groups(cvf_sim* s) {
     g->nb_cell = (uint32_t)tl.size();
     std::vector<int32_t> pl(g->points.begin(), g->points.end());
     if ((rc = upload_vec(g->plist, pl))) return rc;
-    // fused path when every point's slab fits in LDS
-    g->fused = s->tune.fused != 0;
+    // fused path when every point's slab fits in LDS (the reference
+    // schedule's half-sweeps only)
+    g->fused = s->tune.fused != 0 && s->sched == CVF_SCHED_REFERENCE;
     std::vector<BlockEnt> ts;
     g->chunk_lds = 0;
     g->sigma_cells = 0;
@@ -607,7 +613,13 @@ void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Time
   const Launch lh{g.tab_half.p, g.nb_half, (uint32_t)s->tune.block_sigma, stepp, step_off};
   const Launch lc{g.tab_cube.p, g.nb_cube, (uint32_t)s->tune.block_eta, stepp, step_off};
   const Launch ll{g.tab_cell.p, g.nb_cell, (uint32_t)s->tune.block_cell, stepp, step_off};
-  if (g.fused && s->nm > 0) {
+  if (s->sched == CVF_SCHED_FAST) {
+    // eta cube sweep, then n_metropolis x the two colour passes
+    for (int f = 0; f < 2 && g.nb_cube; ++f)
+      timed(CVF_KCLASS_ETA, [&] { launch_eta(st, lc, s->pts.p, s->words.p, f); });
+    for (int pass = 0; pass < 2 * s->nm; ++pass)
+      timed(CVF_KCLASS_SIGMA, [&] { launch_sigma_f(st, lh, s->pts.p, s->words.p, (uint32_t)pass); });
+  } else if (g.fused && s->nm > 0) {
     // 12*n_metropolis half-sweeps in chunks of kChunk, ping-pong between the
     // two word buffers (an even number of chunks ends in `words`)
     const Launch lz{g.tab_slab.p, g.nb_slab, (uint32_t)s->tune.block_chunk, stepp, step_off};
@@ -767,7 +779,7 @@ int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
     if (rc) return rc;
     if (pts[i].n_metropolis != pts[0].n_metropolis || pts[i].n_cluster != pts[0].n_cluster ||
         pts[i].precision != pts[0].precision || pts[i].device_id != pts[0].device_id ||
-        pts[i].volume_mode != pts[0].volume_mode)
+        pts[i].volume_mode != pts[0].volume_mode || pts[i].schedule != pts[0].schedule)
       return fail(CVF_EINVAL, "state points of a batch must share the MC schedule, precision, "
                               "volume mode and device");
   }
@@ -783,6 +795,7 @@ int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
   s->nm = pts[0].n_metropolis;
   s->nc = pts[0].n_cluster;
   s->precision = pts[0].precision;
+  s->sched = pts[0].schedule;
   s->vol_device = pts[0].volume_mode == CVF_VOLUME_DEVICE;
   HIPC(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   HIPC(hipEventCreateWithFlags(&s->start_ev, hipEventDisableTiming));
@@ -1337,7 +1350,7 @@ std::string tune_key(cvf_sim* s) {
     i = j;
   }
   k += "|nm" + std::to_string(s->nm) + "|nc" + std::to_string(s->nc) + "|p" +
-       std::to_string(s->precision) + "|v" + std::to_string(s->vol_device);
+       std::to_string(s->precision) + "|v" + std::to_string(s->vol_device) + "|s" + std::to_string(s->sched);
   return k;
 }
 
@@ -1405,6 +1418,17 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
   const cvf_tuning base = default_tuning(s);
   for (int streams : {1, 2, 4}) {
     if (streams > s->npts) continue;
+    if (s->sched == CVF_SCHED_FAST) {
+      // colour passes never use the fused slab kernel: workgroup sizes only
+      for (int bs : {64, 128, 256, 512}) {
+        cvf_tuning t = base;
+        t.streams = streams;
+        t.fused = 0;
+        t.block_sigma = t.block_eta = bs;
+        cand.push_back(t);
+      }
+      continue;
+    }
     for (int slab : {0, 8, 4, 2, 1})
       for (int bc : {256, 512, 1024}) {
         cvf_tuning t = base;

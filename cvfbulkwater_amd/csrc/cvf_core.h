@@ -41,6 +41,7 @@ enum Stream : uint32_t {
   kStSwFlip = 6,
   kStInitSigma = 7,
   kStInitCube = 8,
+  kStSigmaF = 9,  // colour passes of the fast schedule
 };
 
 struct U4 {

@@ -74,6 +74,9 @@ void launch_vol_chain(hipStream_t st, const PointDev* pts, const int32_t* plist,
                       int precision, int moves);
 void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, int family);
 void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, uint32_t sub);
+// one colour pass of the fast schedule (pass = 2 x Metropolis iteration +
+// colour), over the half-sweep table (N/8 quads per point)
+void launch_sigma_f(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, uint32_t pass);
 // Fused z-slab kernel: optional eta sweep + half-sweeps k0 .. k0+nk-1 of the
 // step on LDS-staged slabs, reading `src` and writing `dst`.
 void launch_chunk(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* src, uint32_t* dst,

@@ -323,6 +323,19 @@ __device__ __forceinline__ void sigma_quads(Ptr W, const uint32_t (&cell)[4 * K]
     if (act[m] && ra[m] < t[m]) W[cell[m]] = (w[m] & ~(7u << (3 * arm))) | (sp[m] << (3 * arm));
 }
 
+// One chessboard sigma move of arm `arm` on the word w held in a register
+// (the rule of sigma_quads), facing neighbour word wn, random word rw.
+__device__ __forceinline__ uint32_t sigma_move_reg(uint32_t w, uint32_t wn, const uint32_t* thr, int arm,
+                                                   uint32_t rw) {
+  uint32_t sp, ra;
+  split_word(rw, 6, sp, ra);
+  const uint32_t sa = sigma_of(w, arm), snb = sigma_of(wn, arm ^ 1);
+  if (sp == sa) return w;
+  const int h = eta_of(w, arm) ? ((int)(sp == snb) - (int)(sa == snb)) : 0;
+  const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
+  return ra < thr[(h + 1) * 11 + d + 5] ? (w & ~(7u << (3 * arm))) | (sp << (3 * arm)) : w;
+}
+
 // Non-chessboard rule (gpu_water3D.cu:1966-1984, 2004-2007): with dN =
 // delta(new, nb) - delta(old, nb), the HB term is
 // [n_c + dN <= 4][n_nb + dN <= 4] delta(new, nb) - [n_c <= 4][n_nb <= 4]
@@ -645,6 +658,64 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   uint32_t* D = dst + e.woff + (uint32_t)z0 * LL;
   const uint32_t* Wi = win + (uint32_t)(-wlo) * LL;
   for (uint32_t i = tid; i < (uint32_t)(T * LL); i += nt) D[i] = Wi[i];
+}
+
+// ------------------------------------------------------- fast schedule
+// One colour pass of the fast schedule (CVF_SCHED_FAST, chessboard only):
+// every cell of colour (x + y + z) % 2 updates its six arms in order, each
+// with the single-move rule.  All neighbours of a cell have the other
+// colour, so no word read by the pass is written by it; a thread keeps its
+// quad's four words in registers through the six arms and writes them once.
+// The quad (x + b L/2, y + a L/2, z), x, y < L/2, shares the colour (L/2 is
+// even) and, per arm, one Philox block keyed on its a=b=0 cell (stream
+// kStSigmaF, sub = 8 pass + arm; cell a,b takes word a + 2b).  Work item
+// t = hx + (L/4) (y + (L/2) z) with x = 2 hx + ((y + z + colour) & 1):
+// N/8 items per point, the half-sweep tables' count.
+__global__ void k_sigma_f(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
+                          uint32_t step_off, const PointDev* __restrict__ pts, uint32_t* __restrict__ words,
+                          uint32_t pass) {
+  const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
+  if (e.point < 0) return;
+  __shared__ uint32_t thr[33];
+  if (threadIdx.x < 33) thr[threadIdx.x] = sigma_thr32(pts[e.point].thr_sigma[threadIdx.x]);
+  __syncthreads();
+  const uint32_t L = e.L, hL = L / 2, qL = L / 4, N = e.N;
+  const uint32_t t = e.start + threadIdx.x;
+  if (t >= N / 8) return;
+  const uint64_t step = *stepp + step_off;
+  const uint64_t seed = seed_of(e);
+  const FastDiv dH{hL, e.mH}, dQ{qL, e.mQ};
+  const uint32_t r1 = fdiv(t, dQ), hx = t - r1 * qL;
+  const uint32_t z = fdiv(r1, dH), y = r1 - z * hL;
+  const uint32_t x = 2 * hx + ((y + z + (pass & 1u)) & 1u);
+  const uint32_t dy = hL * L, dx = hL;
+  const uint32_t c0 = x + L * (y + L * z);
+  const uint32_t c[4] = {c0, c0 + dy, c0 + dx, c0 + dx + dy};
+  const uint32_t cx[4] = {x, x, x + hL, x + hL}, cy[4] = {y, y + hL, y, y + hL};
+  uint32_t* W = words + e.woff;
+  uint32_t w[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) w[m] = W[c[m]];
+  const uint32_t w0[4] = {w[0], w[1], w[2], w[3]};
+#pragma unroll
+  for (int arm = 0; arm < 6; ++arm) {
+    const int ax = arm >> 1;
+    uint32_t wn[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const uint32_t coord = ax == 0 ? cx[m] : ax == 1 ? cy[m] : z;
+      const uint32_t stride = ax == 0 ? 1u : ax == 1 ? L : L * L;
+      wn[m] = W[arm_neighbor(c[m], coord, stride, L, arm)];
+    }
+    const U4 r = draw(seed, kStSigmaF, c0, step, 8 * pass + (uint32_t)arm);
+    w[0] = sigma_move_reg(w[0], wn[0], thr, arm, r.x);
+    w[1] = sigma_move_reg(w[1], wn[1], thr, arm, r.y);
+    w[2] = sigma_move_reg(w[2], wn[2], thr, arm, r.z);
+    w[3] = sigma_move_reg(w[3], wn[3], thr, arm, r.w);
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+    if (w[m] != w0[m]) W[c[m]] = w[m];
 }
 
 // ----------------------------------------------------------- observables
@@ -1014,6 +1085,10 @@ void launch_eta(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words,
 void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, uint32_t sub) {
   launch(k_sigma, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, ln.step_off, pts, words,
          sub);
+}
+void launch_sigma_f(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, uint32_t pass) {
+  launch(k_sigma_f, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, ln.step, ln.step_off, pts, words,
+         pass);
 }
 uint32_t chunk_lds_limit() {
   // gfx950: 160 KB of LDS per CU; keep room for the static arrays

@@ -5,6 +5,8 @@
 // water3D.log and distances (Flag_distances = 0).  The lattice size, a
 // compile-time constant in the reference (35-37), is read from an optional
 // 15th input_data line "Lattice_Size L" or from "-L <L>" (default 32).
+// "-schedule fast" selects the engine's two-colour sigma passes instead of
+// the reference's half-sweeps (same model, different trajectory).
 // All simulation work goes through the C ABI of libcvf.so.
 #include <inttypes.h>
 #include <math.h>
@@ -164,8 +166,16 @@ void read_distances(const char* path, int L, std::vector<float>& d, std::vector<
 
 int main(int argc, char** argv) {
   Input in = read_input_data("input_data");
-  for (int a = 1; a + 1 < argc; ++a)
+  int schedule = CVF_SCHED_REFERENCE;
+  for (int a = 1; a + 1 < argc; ++a) {
     if (!strcmp(argv[a], "-L")) in.L = atoi(argv[a + 1]);
+    if (!strcmp(argv[a], "-schedule")) {
+      if (!strcmp(argv[a + 1], "fast"))
+        schedule = CVF_SCHED_FAST;
+      else if (strcmp(argv[a + 1], "reference"))
+        die("Error: -schedule takes reference or fast");
+    }
+  }
   const int L = in.L, N = L * L * L;
 
   FILE* odata = fopen("data.out", "w");
@@ -188,6 +198,7 @@ int main(int argc, char** argv) {
   p.device_id = in.device;
   p.precision = CVF_FP64;
   p.volume_mode = CVF_VOLUME_HOST;
+  p.schedule = schedule;
 
   // header, gpu_water3D.cu:474-509 (no random-number arrays are stored)
   fprintf(log, "RNG allocated memory: %d MB\n", 0);

@@ -37,6 +37,13 @@ typedef enum cvf_status {
 
 enum { CVF_FP64 = 0, CVF_FP32_COMPENSATED = 1 };
 enum { CVF_VOLUME_HOST = 0, CVF_VOLUME_DEVICE = 1 };
+/* Order of the sigma moves in one Metropolis step.  REFERENCE: the
+ * reference's 12 half-sweeps of randomly drawn (arm, parity)
+ * (gpu_water3D.cu:1383-1417).  FAST (this engine's option, chessboard mode
+ * only): two passes over the cells of colour (x + y + z) % 2, each cell
+ * updating its six arms in order; the same single-move rule and the same
+ * Boltzmann distribution, without the reference's trajectory. */
+enum { CVF_SCHED_REFERENCE = 0, CVF_SCHED_FAST = 1 };
 
 /* Mirrors input_data (gpu_water3D.cu:292-386) plus the reference's
  * compile-time constants (lattice size 35-37, model constants 57-64). */
@@ -48,11 +55,12 @@ typedef struct cvf_params {
   int32_t n_cluster;         /* Cluster_Steps */
   float r_cutoff;            /* R_cutoff */
   int32_t flag_config;       /* 0 random, 1 caller-supplied (cvf_set_config), 2 ordered */
-  int32_t flag_chessboard;   /* must be 1 */
+  int32_t flag_chessboard;   /* 1: eta cube moves; 0: eta fixed, HB gated by NHBMAX */
   int32_t device_id;
   float J_hb, J_sig, v_hb;   /* 0.5, 0.08, 0.6 */
   int32_t precision;         /* CVF_FP64 | CVF_FP32_COMPENSATED */
   int32_t volume_mode;       /* CVF_VOLUME_HOST | CVF_VOLUME_DEVICE */
+  int32_t schedule;          /* CVF_SCHED_REFERENCE | CVF_SCHED_FAST */
 } cvf_params;
 
 /* One sample of measure() (gpu_water3D.cu:2014-2160). */
@@ -81,7 +89,8 @@ void cvf_params_default(cvf_params *p);
 int cvf_create(const cvf_params *p, cvf_sim **out);
 
 /* n independent state points of uneven L advanced by one launch sequence;
- * they must agree on n_metropolis, n_cluster, precision and device_id. */
+ * they must agree on n_metropolis, n_cluster, precision, volume_mode,
+ * schedule and device_id. */
 int cvf_create_batch(const cvf_params *pts, int32_t n, cvf_sim **out);
 
 /* input_config read (gpu_water3D.cu:730-800): sigma in [0,5], eta in {0,1};
@@ -95,7 +104,8 @@ int cvf_get_config(cvf_sim *s, int32_t pt, uint8_t *sigma6N, uint8_t *eta6N);
 
 /* n_steps calls of MonteCarlo_Step (gpu_water3D.cu:1346-1733): eta cube
  * sweep, volume move, n_metropolis x 12 (arm, parity) half-lattice sigma
- * sweeps, n_cluster Swendsen-Wang steps.  Asynchronous. */
+ * sweeps (or, with CVF_SCHED_FAST, n_metropolis x 2 colour passes),
+ * n_cluster Swendsen-Wang steps.  Asynchronous. */
 int cvf_step(cvf_sim *s, int64_t n_steps);
 
 /* measure() (gpu_water3D.cu:2014-2160) for state point pt. */
@@ -142,7 +152,7 @@ typedef struct cvf_tuning {
   int32_t graph_steps;  /* MC steps per replayed hipGraph (1..64) */
   int32_t fused;        /* 1: eta sweep + 6 half-sweeps per kernel on LDS-staged z-slabs
                            where the slab fits in LDS; 0: one kernel per half-sweep */
-  int32_t slab_planes;  /* planes per slab of the fused kernel (<= 8); 0 = automatic */
+  int32_t slab_planes;  /* planes per slab of the fused kernel (<= 16); 0 = automatic */
   int32_t block_chunk;  /* workgroup size of the fused kernel */
 } cvf_tuning;
 

@@ -297,6 +297,9 @@ static void compute_nhb(ora_sim *s) {
 
 ora_sim *ora_create(const ora_params *p) {
   if (p->L < 4 || p->L % 4 != 0) return NULL;
+  /* the colour passes need the chessboard rule: a counter update touches a
+   * neighbour that other cells of the pass also touch */
+  if (p->schedule != 0 && (p->schedule != 1 || !p->flag_chessboard)) return NULL;
   init_solutions();
   ora_sim *s = (ora_sim *)calloc(1, sizeof(ora_sim));
   s->p = *p;
@@ -467,6 +470,44 @@ void ora_schedule(uint64_t seed, uint64_t step, uint32_t sub, int *arm, int *ind
  * delta(old,nb), the HB term is [nhb_i+dN <= 4][nhb_j+dN <= 4] delta(new,nb)
  * - [nhb_i <= 4][nhb_j <= 4] delta(old,nb), and an accepted move adds dN to
  * the uint8 counters of both cells. */
+/* One sigma move of arm `arm` of cell c to proposal `nw`, accepted iff
+ * acc < threshold (the rule of gpu_update, 1966-2007, both branches). */
+static void sigma_move(ora_sim *s, int c, int arm, int nw, uint32_t acc) {
+  int L = s->L, N = s->N;
+  int nb = neighbor(L, c, arm);
+  int nbspin = s->s[(arm ^ 1) * N + nb];
+  int old = s->s[arm * N + c];
+  if (nw == old) return;
+  int dn = (nw == nbspin) - (old == nbspin);
+  int h;
+  if (s->p.flag_chessboard) {
+    h = s->eta[arm * N + c] * dn;
+  } else {
+    int ni = s->nhb[c], nj = s->nhb[nb];
+    h = ((ni + dn <= 4) && (nj + dn <= 4)) * (nw == nbspin) - ((ni <= 4) && (nj <= 4)) * (old == nbspin);
+  }
+  int d = 0;
+  for (int k = 0; k < ARMS; ++k) {
+    if (k == arm) continue;
+    int sk = s->s[k * N + c];
+    d += (nw == sk) - (old == sk);
+  }
+  if ((uint64_t)acc < s->thr_sigma[(h + 1) * 11 + (d + 5)]) {
+    s->s[arm * N + c] = (uint8_t)nw;
+    s->nhb[c] = (uint8_t)(s->nhb[c] + dn);
+    s->nhb[nb] = (uint8_t)(s->nhb[nb] + dn);
+  }
+}
+
+/* the four cells (x + b L/2, y + a L/2, z), a,b in {0,1}, share parities
+ * along every axis and the colour (L/2 is even) and one Philox block keyed
+ * on the a=b=0 cell; cell a,b uses word a + 2b */
+static int quad_site(int L, int c, int x, int y, int *word) {
+  int ay = y >= L / 2, bx = x >= L / 2;
+  *word = ay + 2 * bx;
+  return c - ay * (L / 2) * L - bx * (L / 2);
+}
+
 void ora_sigma_halfsweep(ora_sim *s, uint64_t step, uint32_t sub, int arm, int indices) {
   int L = s->L, N = s->N;
   for (int c = 0; c < N; ++c) {
@@ -475,37 +516,32 @@ void ora_sigma_halfsweep(ora_sim *s, uint64_t step, uint32_t sub, int arm, int i
     int coord = arm < 2 ? x : (arm < 4 ? y : z);
     if (indices && (coord % 2)) continue;
     if (!indices && !(coord % 2)) continue;
-    int nb = neighbor(L, c, arm);
-    int nbspin = s->s[(arm ^ 1) * N + nb];
-    int old = s->s[arm * N + c];
-    /* the four cells (x + b L/2, y + a L/2, z), a,b in {0,1}, are active
-     * together (L/2 is even) and share one Philox block keyed on the a=b=0
-     * cell; cell a,b uses word a + 2b */
-    int ay = y >= L / 2, bx = x >= L / 2;
-    int site = c - ay * (L / 2) * L - bx * (L / 2);
+    int word, site = quad_site(L, c, x, y, &word);
     uint32_t r[4], prop, acc;
     draw(s->p.seed, ORA_ST_SIGMA, (uint32_t)site, step, sub, r);
-    split_word(r[ay + 2 * bx], 6, &prop, &acc);
-    int nw = (int)prop;
-    if (nw == old) continue;
-    int dn = (nw == nbspin) - (old == nbspin);
-    int h;
-    if (s->p.flag_chessboard) {
-      h = s->eta[arm * N + c] * dn;
-    } else {
-      int ni = s->nhb[c], nj = s->nhb[nb];
-      h = ((ni + dn <= 4) && (nj + dn <= 4)) * (nw == nbspin) - ((ni <= 4) && (nj <= 4)) * (old == nbspin);
-    }
-    int d = 0;
-    for (int k = 0; k < ARMS; ++k) {
-      if (k == arm) continue;
-      int sk = s->s[k * N + c];
-      d += (nw == sk) - (old == sk);
-    }
-    if ((uint64_t)acc < s->thr_sigma[(h + 1) * 11 + (d + 5)]) {
-      s->s[arm * N + c] = (uint8_t)nw;
-      s->nhb[c] = (uint8_t)(s->nhb[c] + dn);
-      s->nhb[nb] = (uint8_t)(s->nhb[nb] + dn);
+    split_word(r[word], 6, &prop, &acc);
+    sigma_move(s, c, arm, (int)prop, acc);
+  }
+}
+
+/* The fast schedule's sigma pass (a design of this engine, not the
+ * reference's schedule): every cell of colour (x + y + z) % 2 updates its
+ * six arms in order 0..5 with the same single-move rule.  Cells of one
+ * colour share no bond (every neighbour has the other colour), so the pass
+ * is one parallel sweep.  Randoms: stream ORA_ST_SIGMA_F, the quad's site,
+ * sub = 8 pass + arm, pass = 2 (Metropolis iteration) + colour. */
+void ora_sigma_colour(ora_sim *s, uint64_t step, uint32_t pass) {
+  int L = s->L, N = s->N, colour = (int)(pass & 1);
+  for (int c = 0; c < N; ++c) {
+    int x, y, z;
+    coords(L, c, &x, &y, &z);
+    if (((x + y + z) & 1) != colour) continue;
+    int word, site = quad_site(L, c, x, y, &word);
+    for (int arm = 0; arm < ARMS; ++arm) {
+      uint32_t r[4], prop, acc;
+      draw(s->p.seed, ORA_ST_SIGMA_F, (uint32_t)site, step, 8 * pass + (uint32_t)arm, r);
+      split_word(r[word], 6, &prop, &acc);
+      sigma_move(s, c, arm, (int)prop, acc);
     }
   }
 }
@@ -594,13 +630,19 @@ void ora_step(ora_sim *s) {
   uint64_t t = s->step;
   if (s->p.flag_chessboard) ora_eta_sweep(s, t);
   ora_volume_move(s, t);
-  for (int j = 0; j < s->p.n_metropolis; ++j)
+  for (int j = 0; j < s->p.n_metropolis; ++j) {
+    if (s->p.schedule == 1) {
+      ora_sigma_colour(s, t, (uint32_t)(2 * j));
+      ora_sigma_colour(s, t, (uint32_t)(2 * j + 1));
+      continue;
+    }
     for (int k = 0; k < 2 * ARMS; ++k) {
       int arm, ind;
       uint32_t sub = (uint32_t)(j * 2 * ARMS + k);
       ora_schedule(s->p.seed, t, sub, &arm, &ind);
       ora_sigma_halfsweep(s, t, sub, arm, ind);
     }
+  }
   for (int jc = 0; jc < s->p.n_cluster; ++jc) ora_sw_step(s, t, (uint32_t)jc);
   s->step = t + 1;
 }

@@ -44,7 +44,7 @@ extern "C" {
 
 enum {
   ORA_ST_SCHED = 1, ORA_ST_SIGMA = 2, ORA_ST_ETA = 3, ORA_ST_VOLUME = 4,
-  ORA_ST_SWBOND = 5, ORA_ST_SWFLIP = 6, ORA_ST_INIT_SIGMA = 7, ORA_ST_INIT_CUBE = 8
+  ORA_ST_SWBOND = 5, ORA_ST_SWFLIP = 6, ORA_ST_INIT_SIGMA = 7, ORA_ST_INIT_CUBE = 8, ORA_ST_SIGMA_F = 9
 };
 
 typedef struct ora_params {
@@ -56,6 +56,7 @@ typedef struct ora_params {
   float J_hb, J_sig, v_hb;
   int32_t flag_config;     /* 0 random, 1 from arrays (ora_set_config), 2 ordered */
   int32_t flag_chessboard; /* 1: eta cube moves; 0: eta fixed, HB gated by NHBMAX */
+  int32_t schedule;        /* 0: the reference's half-sweeps; 1: two-colour cell passes */
 } ora_params;
 
 typedef struct ora_obs {
@@ -97,6 +98,8 @@ void ora_steps(ora_sim *s, int64_t n);
 /* Individual phases, for fine-grained parity tests. */
 void ora_eta_sweep(ora_sim *s, uint64_t step);
 void ora_sigma_halfsweep(ora_sim *s, uint64_t step, uint32_t sub, int arm, int indices);
+/* colour pass of the fast schedule: pass = 2 (Metropolis iteration) + colour */
+void ora_sigma_colour(ora_sim *s, uint64_t step, uint32_t pass);
 void ora_schedule(uint64_t seed, uint64_t step, uint32_t sub, int *arm, int *indices);
 void ora_volume_move(ora_sim *s, uint64_t step);
 void ora_sw_step(ora_sim *s, uint64_t step, uint32_t jc);

@@ -15,7 +15,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "_build" / "libcvf_oracle.so"
 
-ST_SCHED, ST_SIGMA, ST_ETA, ST_VOLUME, ST_SWBOND, ST_SWFLIP, ST_INIT_SIGMA, ST_INIT_CUBE = range(1, 9)
+ST_SCHED, ST_SIGMA, ST_ETA, ST_VOLUME, ST_SWBOND, ST_SWFLIP, ST_INIT_SIGMA, ST_INIT_CUBE, ST_SIGMA_F = range(1, 10)
 
 
 class _Params(C.Structure):
@@ -23,7 +23,7 @@ class _Params(C.Structure):
         ("L", C.c_int32), ("P", C.c_float), ("T", C.c_float), ("seed", C.c_uint64),
         ("n_metropolis", C.c_int32), ("n_cluster", C.c_int32), ("r_cutoff", C.c_float),
         ("J_hb", C.c_float), ("J_sig", C.c_float), ("v_hb", C.c_float), ("flag_config", C.c_int32),
-        ("flag_chessboard", C.c_int32),
+        ("flag_chessboard", C.c_int32), ("schedule", C.c_int32),
     ]
 
 
@@ -72,6 +72,7 @@ def lib() -> C.CDLL:
             "ora_steps": (None, [P, C.c_int64]),
             "ora_eta_sweep": (None, [P, C.c_uint64]),
             "ora_sigma_halfsweep": (None, [P, C.c_uint64, C.c_uint32, C.c_int, C.c_int]),
+            "ora_sigma_colour": (None, [P, C.c_uint64, C.c_uint32]),
             "ora_schedule": (None, [C.c_uint64, C.c_uint64, C.c_uint32, I32P, I32P]),
             "ora_volume_move": (None, [P, C.c_uint64]),
             "ora_sw_step": (None, [P, C.c_uint64, C.c_uint32]),
@@ -145,11 +146,12 @@ def cluster_stats_labels(L: int, labels: np.ndarray):
 
 class OracleSim:
     def __init__(self, L=32, P=0.45, T=0.815, seed=44648604, n_metropolis=1, n_cluster=0,
-                 r_cutoff=6.0, J_hb=0.5, J_sig=0.08, v_hb=0.6, flag_config=0, flag_chessboard=1):
+                 r_cutoff=6.0, J_hb=0.5, J_sig=0.08, v_hb=0.6, flag_config=0, flag_chessboard=1,
+                 schedule=0):
         self.L = L
         self.N = L ** 3
         p = _Params(L, P, T, seed, n_metropolis, n_cluster, r_cutoff, J_hb, J_sig, v_hb, flag_config,
-                    flag_chessboard)
+                    flag_chessboard, schedule)
         self._h = lib().ora_create(C.byref(p))
         if not self._h:
             raise ValueError("ora_create failed")
@@ -204,6 +206,9 @@ class OracleSim:
 
     def sigma_halfsweep(self, step: int, sub: int, arm: int, indices: int):
         lib().ora_sigma_halfsweep(self._h, step, sub, arm, indices)
+
+    def sigma_colour(self, step: int, pass_: int):
+        lib().ora_sigma_colour(self._h, step, pass_)
 
     def volume_move(self, step: int):
         lib().ora_volume_move(self._h, step)
