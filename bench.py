@@ -321,6 +321,32 @@ def cluster_form(steps: int, warmup: int, ev: HipEvents) -> dict:
             "value": 32 ** 3 / (ms * 1e-3)}
 
 
+def fast_schedule_form(steps: int, warmup: int, ev: HipEvents, tune_steps: int) -> dict:
+    """The engine's fast schedule (two-colour cell passes, CVF_SCHED_FAST) on
+    the headline batch and on the plain L = 32 point: the same model and
+    work per step (6 sigma proposals per molecule, eta sweep, volume move),
+    another order of the sigma moves."""
+    from cvfbulkwater_amd import FP64, SCHED_FAST, VOLUME_HOST, Params, Simulation
+
+    pts = batch_params(fp32=True)
+    for p in pts:
+        p.schedule = SCHED_FAST
+    cells = sum(p.L ** 3 for p in pts)
+    sim = Simulation(pts)
+    tuning = sim.autotune(tune_steps) if tune_steps else sim.tuning
+    ms = time_steps(sim, steps, warmup, ev)
+    sim.close()
+    plain = Simulation(Params(L=32, T=0.815, P=0.45, seed=44648604, precision=FP64, volume_mode=VOLUME_HOST,
+                              schedule=SCHED_FAST))
+    ptuning = plain.autotune(tune_steps) if tune_steps else plain.tuning
+    pms = time_steps(plain, max(steps, 1000), warmup, ev)
+    plain.close()
+    return {"workload": "fast schedule: the headline batch and the plain L=32 point with two-colour "
+                        "cell passes instead of the reference's 12 half-sweeps",
+            "batch": {"value": cells / (ms * 1e-3), "ms_per_step": ms, "tuning": tuning},
+            "plain": {"value": 32 ** 3 / (pms * 1e-3), "ms_per_step": pms, "tuning": ptuning}}
+
+
 def nonchess_form(steps: int, warmup: int, ev: HipEvents, cpu_seconds: float = 0.0) -> dict:
     """Flag_chessboard = 0 at the plain shape: L = 32, T = 0.815, P = 0.45,
     eta fixed, hydrogen bonds gated by the per-molecule counters, FP64,
@@ -476,7 +502,8 @@ def main() -> int:
                          "(skips the autotuner)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-steps", type=int, default=5000)
-    ap.add_argument("--only", choices=["batch", "plain", "precision", "cluster", "reweight"], default=None,
+    ap.add_argument("--only", choices=["batch", "plain", "precision", "cluster", "reweight", "fast"],
+                    default=None,
                     help="run one leg only (for profiling)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (calibrates the "
@@ -540,6 +567,8 @@ def main() -> int:
         out["cluster_only"] = cluster_form(args.plain_steps, args.warmup, ev)
     if args.only in (None, "precision"):
         out["precision"] = precision_form(args.precision_steps, 10, ev)
+    if args.only in (None, "fast"):
+        out["fast"] = fast_schedule_form(args.steps, args.warmup, ev, args.tune_steps)
     if args.only in (None, "reweight"):
         out["reweight"] = reweight_form(with_cpu=not args.no_cpu_baseline)
 
@@ -598,6 +627,10 @@ def main() -> int:
         line["precision_form"] = out["precision"]
         if "value" not in line:
             line.update(value=None, ms_per_step=out["precision"].get("ms_per_step", {}).get("fp32c"))
+    if "fast" in out:
+        line["fast_schedule"] = out["fast"]
+        if "value" not in line:
+            line.update(value=out["fast"]["batch"]["value"], ms_per_step=out["fast"]["batch"]["ms_per_step"])
     if "reweight" in out:
         line["reweight_form"] = out["reweight"]
         if "value" not in line:
