@@ -16,7 +16,8 @@ from typing import Sequence
 
 import numpy as np
 
-from . import (FP32_COMPENSATED, THERMO_EXE_PATH, VOLUME_DEVICE, Params, Simulation, thermo_averages)
+from . import (FP32_COMPENSATED, SCHED_REFERENCE, THERMO_EXE_PATH, VOLUME_DEVICE, Params, Simulation,
+               thermo_averages)
 
 
 def awk_num(v: float) -> str:
@@ -33,14 +34,15 @@ def label(x: float) -> str:
 
 def scan_series(L: int, points: Sequence[tuple[float, float]], equil_steps: int, samples: int, interval: int,
                 seed: int = 44648604, precision: int = FP32_COMPENSATED, n_cluster: int = 0,
-                tuning: dict | None = None) -> np.ndarray:
+                tuning: dict | None = None, schedule: int = SCHED_REFERENCE) -> np.ndarray:
     """Observables (e, h, v, N_HB, m, r_cell) of every point after each
     sampling step: shape (points, samples, 6).  Sampling follows the
     reference's production loop (gpu_water3D.cu:1204-1243): a sample after
     production step i whenever i % interval == 0, i.e. after steps 1,
-    1 + interval, 1 + 2 interval, ..."""
+    1 + interval, 1 + 2 interval, ...  schedule=SCHED_FAST runs the
+    colour-pass schedule (same model, another trajectory)."""
     pts = [Params(L=L, T=T, P=P, seed=seed + 1000003 * i, precision=precision, volume_mode=VOLUME_DEVICE,
-                  n_cluster=n_cluster) for i, (T, P) in enumerate(points)]
+                  n_cluster=n_cluster, schedule=schedule) for i, (T, P) in enumerate(points)]
     sim = Simulation(pts)
     if tuning:
         sim.tuning = tuning
@@ -65,14 +67,14 @@ def write_data_out(path: Path, series: np.ndarray, interval: int) -> None:
 def isobar_scan(out_dir: str | Path, L: int, temperatures: Sequence[float], pressures: Sequence[float],
                 equil_steps: int, samples: int, interval: int = 10, seed: int = 44648604,
                 precision: int = FP32_COMPENSATED, n_cluster: int = 0, data_out: bool = False,
-                tuning: dict | None = None) -> Path:
+                tuning: dict | None = None, schedule: int = SCHED_REFERENCE) -> Path:
     """Scan the (T, P) grid in one batch and write the analysis files of
     CVF_ThermodynamicAnalysis into out_dir.  With data_out=True the
     L<L>_T<T>_P<P>/data.out files are written as well."""
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
     points = [(T, P) for P in pressures for T in temperatures]
-    series = scan_series(L, points, equil_steps, samples, interval, seed, precision, n_cluster, tuning)
+    series = scan_series(L, points, equil_steps, samples, interval, seed, precision, n_cluster, tuning, schedule)
     rows = []
     for (T, P), s in zip(points, series):
         a = thermo_averages(s)
