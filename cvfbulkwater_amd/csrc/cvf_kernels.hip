@@ -228,22 +228,6 @@ __device__ __forceinline__ void eta_cube_move(const uint32_t (&w)[8], const uint
   }
 }
 
-__device__ __forceinline__ void eta_cube(uint32_t* __restrict__ W, const uint64_t* thr, const uint64_t* etab,
-                                         uint32_t L, uint32_t ox, uint32_t oy, uint32_t oz, uint32_t rw) {
-  const uint32_t x1 = ox + 1 == L ? 0 : ox + 1;
-  const uint32_t y1 = oy + 1 == L ? 0 : oy + 1;
-  const uint32_t z1 = oz + 1 == L ? 0 : oz + 1;
-  uint32_t idx[8], w[8];
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const uint32_t x = (b & 1) ? x1 : ox, y = (b & 2) ? y1 : oy, z = (b & 4) ? z1 : oz;
-    idx[b] = x + L * (y + L * z);
-  }
-#pragma unroll
-  for (int b = 0; b < 8; ++b) w[b] = W[idx[b]];
-  eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t v) { W[idx[b]] = v; });
-}
-
 // One thread per quad of cubes of the family with origins
 // (ox + b L/2, oy + a L/2, oz), a,b in {0,1}: one Philox block, word a + 2b.
 __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
@@ -271,10 +255,25 @@ __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint6
   const uint32_t cube_ref = ox / 2 + hL * (oy / 2) + hL * hL * oz;  // gpu_water3D.cu:3405
   const U4 r = draw(seed_of(e), kStEta, cube_ref, step, 0);
   uint32_t* W = words + e.woff;
-  eta_cube(W, thr, etab, L, ox, oy, oz, r.x);
-  eta_cube(W, thr, etab, L, ox, oy + hL, oz, r.y);
-  eta_cube(W, thr, etab, L, ox + hL, oy, oz, r.z);
-  eta_cube(W, thr, etab, L, ox + hL, oy + hL, oz, r.w);
+  // the four cubes share no corner: all 32 corner loads go out before any
+  // cube move, so their latencies overlap
+  const uint32_t z1 = oz + 1 == L ? 0 : oz + 1;
+  uint32_t idx[4][8], w[4][8];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t cx = ox + ((q & 2) ? hL : 0), cy = oy + ((q & 1) ? hL : 0);
+    const uint32_t x1 = cx + 1 == L ? 0 : cx + 1, y1 = cy + 1 == L ? 0 : cy + 1;
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      idx[q][b] = ((b & 1) ? x1 : cx) + L * (((b & 2) ? y1 : cy) + L * ((b & 4) ? z1 : oz));
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) w[q][b] = W[idx[q][b]];
+  const uint32_t rq[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) eta_cube_move(w[q], thr, etab, rq[q], [&](int b, uint32_t v) { W[idx[q][b]] = v; });
 }
 
 // --------------------------------------------------------- sigma half-sweep
