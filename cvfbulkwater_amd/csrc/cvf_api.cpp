@@ -535,7 +535,8 @@ int build_groups(cvf_sim* s) {
     HIPC(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     HIPC(hipEventCreateWithFlags(&g->done, hipEventDisableTiming));
     const auto th = make_table(s->ph, g->points, (uint32_t)s->tune.block_sigma, cells_sigma);
-    const auto tc = make_table(s->ph, g->points, (uint32_t)s->tune.block_eta, cells_eta);
+    // k_eta runs four threads per cube quad: block_eta / 4 quads per workgroup
+    const auto tc = make_table(s->ph, g->points, (uint32_t)s->tune.block_eta / 4, cells_eta);
     const auto tl = make_table(s->ph, g->points, (uint32_t)s->tune.block_cell, cells_all);
     int rc;
     if ((rc = upload_vec(g->tab_half, th)) || (rc = upload_vec(g->tab_cube, tc)) ||

@@ -228,13 +228,38 @@ __device__ __forceinline__ void eta_cube_move(const uint32_t (&w)[8], const uint
   }
 }
 
-// One thread per quad of cubes of the family with origins
-// (ox + b L/2, oy + a L/2, oz), a,b in {0,1}: one Philox block, word a + 2b.
+// Four threads per quad of cubes of the family with origins
+// (ox + b L/2, oy + a L/2, oz), a,b in {0,1}: the quad's Philox block (each
+// of the four lanes forms it), word a + 2b for cube a,b, one cube per
+// thread.  A workgroup covers blockDim/4 table items.  The corner loads are
+// issued before the workgroup's barrier on the acceptance tables (lanes past
+// the point's end load the entry's first quad and store nothing).
 __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
                       uint32_t step_off,
                       const PointDev* __restrict__ pts, uint32_t* __restrict__ words, int family) {
   const BlockEnt e = tab[xcd_logical_block(blockIdx.x, nb)];
   if (e.point < 0) return;
+  const uint64_t step = *stepp + step_off;
+  const uint32_t L = e.L, hL = L / 2, qL = L / 4;
+  const uint32_t t = e.start + (threadIdx.x >> 2), q = threadIdx.x & 3u;
+  const bool live = t < qL * qL * hL;
+  const uint32_t tl = live ? t : e.start;
+  const FastDiv dQ{qL, e.mQ};
+  const uint32_t r1 = fdiv(tl, dQ);  // t = i + qL (j + qL k), i,j < L/4, k < L/2
+  const uint32_t i = tl - r1 * qL;
+  const uint32_t k = fdiv(r1, dQ);
+  const uint32_t j = r1 - k * qL;
+  const uint32_t ox = 2 * i + family, oy = 2 * j + family, oz = 2 * k + family;
+  // cube q: (ox + (q & 2 ? L/2 : 0), oy + (q & 1 ? L/2 : 0), oz)
+  const uint32_t cx = ox + ((q & 2u) ? hL : 0u), cy = oy + ((q & 1u) ? hL : 0u);
+  const uint32_t x1 = cx + 1 == L ? 0 : cx + 1, y1 = cy + 1 == L ? 0 : cy + 1, z1 = oz + 1 == L ? 0 : oz + 1;
+  uint32_t* W = words + e.woff;
+  uint32_t idx[8], w[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b)
+    idx[b] = ((b & 1) ? x1 : cx) + L * (((b & 2) ? y1 : cy) + L * ((b & 4) ? z1 : oz));
+#pragma unroll
+  for (int b = 0; b < 8; ++b) w[b] = W[idx[b]];
   __shared__ uint64_t thr[9];
   __shared__ uint64_t etab[9];
   if (threadIdx.x < 9) {
@@ -242,38 +267,11 @@ __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint6
     etab[threadIdx.x] = eta_table_entry(threadIdx.x);
   }
   __syncthreads();
-  const uint32_t L = e.L, hL = L / 2, qL = L / 4;
-  const uint32_t t = e.start + threadIdx.x;
-  if (t >= qL * qL * hL) return;
-  const uint64_t step = *stepp + step_off;
-  const FastDiv dQ{qL, e.mQ};
-  const uint32_t r1 = fdiv(t, dQ);  // t = i + qL (j + qL k), i,j < L/4, k < L/2
-  const uint32_t i = t - r1 * qL;
-  const uint32_t k = fdiv(r1, dQ);
-  const uint32_t j = r1 - k * qL;
-  const uint32_t ox = 2 * i + family, oy = 2 * j + family, oz = 2 * k + family;
+  if (!live) return;
   const uint32_t cube_ref = ox / 2 + hL * (oy / 2) + hL * hL * oz;  // gpu_water3D.cu:3405
   const U4 r = draw(seed_of(e), kStEta, cube_ref, step, 0);
-  uint32_t* W = words + e.woff;
-  // the four cubes share no corner: all 32 corner loads go out before any
-  // cube move, so their latencies overlap
-  const uint32_t z1 = oz + 1 == L ? 0 : oz + 1;
-  uint32_t idx[4][8], w[4][8];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const uint32_t cx = ox + ((q & 2) ? hL : 0), cy = oy + ((q & 1) ? hL : 0);
-    const uint32_t x1 = cx + 1 == L ? 0 : cx + 1, y1 = cy + 1 == L ? 0 : cy + 1;
-#pragma unroll
-    for (int b = 0; b < 8; ++b)
-      idx[q][b] = ((b & 1) ? x1 : cx) + L * (((b & 2) ? y1 : cy) + L * ((b & 4) ? z1 : oz));
-  }
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int b = 0; b < 8; ++b) w[q][b] = W[idx[q][b]];
-  const uint32_t rq[4] = {r.x, r.y, r.z, r.w};
-#pragma unroll
-  for (int q = 0; q < 4; ++q) eta_cube_move(w[q], thr, etab, rq[q], [&](int b, uint32_t v) { W[idx[q][b]] = v; });
+  const uint32_t rw = q == 0 ? r.x : q == 1 ? r.y : q == 2 ? r.z : r.w;
+  eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t v) { W[idx[b]] = v; });
 }
 
 // --------------------------------------------------------- sigma half-sweep
@@ -536,7 +534,7 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
                         uint32_t step_off,
                         const PointDev* __restrict__ pts, const uint32_t* __restrict__ src,
                         uint32_t* __restrict__ dst, uint32_t k0, uint32_t nk, int with_eta) {
-  extern __shared__ uint32_t win[];
+  extern __shared__ __align__(16) uint32_t win[];
   __shared__ uint32_t thr_s[33];
   __shared__ uint64_t thr_e[9];
   __shared__ uint64_t etab[9];
@@ -561,7 +559,7 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   }
   __syncthreads();
   const int L = (int)e.L, hL = L / 2, qL = L / 4, LL = L * L, T = (int)e.T, z0 = (int)e.start;
-  const FastDiv dH{(uint32_t)hL, e.mH}, dQ{(uint32_t)qL, e.mQ}, dLL{(uint32_t)LL, e.mLL};
+  const FastDiv dH{(uint32_t)hL, e.mH}, dQ{(uint32_t)qL, e.mQ};
   const FastDiv dP8{(uint32_t)(LL / 8), e.mP8}, dP4{(uint32_t)(LL / 4), e.mP4},
       dC{(uint32_t)(qL * qL), e.mC};
   // planes relative to z0: window [wlo, whi)
@@ -575,12 +573,26 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   const int wlo = -h_lo - ext, whi = T + h_hi + ext, np = whi - wlo;
   if (tid < (uint32_t)np) gofs[tid] = wrap_plane(z0 + wlo + (int)tid, L) * (uint32_t)LL;
   __syncthreads();
-  // stage the window
-  const uint32_t* S = src + e.woff;
+  // stage the window in 16-byte pieces (point offsets are 256-byte aligned
+  // and L^2 is a multiple of 16 words), four loads in flight per thread
   {
-    const uint32_t Q = (uint32_t)LL, dp = fdiv(nt, dLL), du = nt - dp * Q;
-    for (PlaneIter it = plane_first(tid, Q, dLL); it.pi < (uint32_t)np; plane_next(it, Q, dp, du))
-      win[it.pi * Q + it.u] = S[gofs[it.pi] + it.u];
+    const uint4* S4 = reinterpret_cast<const uint4*>(src + e.woff);
+    uint4* W4 = reinterpret_cast<uint4*>(win);
+    const uint32_t Q = (uint32_t)(LL / 4), n = (uint32_t)np * Q;
+    for (uint32_t i0 = tid; i0 < n; i0 += 4 * nt) {
+      // pieces past the end redo the last one (same value, same place), so
+      // the four loads are unconditional and issued together
+      uint4 v[4];
+      uint32_t ix[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ix[u] = min(i0 + (uint32_t)u * nt, n - 1);
+        const uint32_t pi = mulhi32(ix[u], dP4.m);  // L^2/4 >= 4
+        v[u] = S4[(gofs[pi] >> 2) + (ix[u] - pi * Q)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) W4[ix[u]] = v[u];
+    }
   }
   __syncthreads();
   if (with_eta) {
@@ -653,10 +665,21 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
     }
     __syncthreads();
   }
-  // the slab's own planes go to the other buffer
-  uint32_t* D = dst + e.woff + (uint32_t)z0 * LL;
-  const uint32_t* Wi = win + (uint32_t)(-wlo) * LL;
-  for (uint32_t i = tid; i < (uint32_t)(T * LL); i += nt) D[i] = Wi[i];
+  // the slab's own planes go to the other buffer, 16-byte pieces
+  uint4* D4 = reinterpret_cast<uint4*>(dst + e.woff + (uint32_t)z0 * LL);
+  const uint4* Wi4 = reinterpret_cast<const uint4*>(win + (uint32_t)(-wlo) * LL);
+  const uint32_t n4 = (uint32_t)(T * LL) / 4;
+  for (uint32_t i0 = tid; i0 < n4; i0 += 4 * nt) {
+    uint4 v[4];
+    uint32_t ix[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ix[u] = min(i0 + (uint32_t)u * nt, n4 - 1);
+      v[u] = Wi4[ix[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) D4[ix[u]] = v[u];
+  }
 }
 
 // ------------------------------------------------------- fast schedule
