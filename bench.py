@@ -594,15 +594,14 @@ def main() -> int:
         if valu is not None:
             # the kernel is bound by integer VALU issue, not HBM: its wave64
             # instructions per MC step against the chip's issue peak
-            # (256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction)
+            # (256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction;
+            # on gfx950 v_mad_u64_u32 and v_mul_hi_u32 issue at that same rate)
             per_step = out["prof"][{"k_chunk": "chunk", "k_sigma": "sigma"}[out["kname"]]][1] / args.profile_steps
             peak = 256 * 4 * 2.4e9 / 4
             rate = valu[0] * per_step / (out["batch_ms"] * 1e-3)
             line["valu_issue"] = {"kernel": out["kname"], "insts_per_step": valu[0] * per_step,
                                   "achieved": rate / 1e9, "peak": peak / 1e9, "unit": "G wave-instr/s",
-                                  "frac": rate / peak,
-                                  "frac_int64_quarter_rate": (valu[0] + 3 * valu[1]) * per_step /
-                                                             (out["batch_ms"] * 1e-3) / peak}
+                                  "frac": rate / peak}
     if "plain_rate" in out:
         line["plain"] = {"workload": "plain drop-in form: L=32, T=0.815, P=0.45, FP64 vdW, host "
                                      "volume move", "value": out["plain_rate"],

@@ -421,29 +421,29 @@ BlockEnt make_entry(const PointHost& P, int point, uint32_t start) {
 uint32_t slab_lds_bytes(uint32_t L, uint32_t T) { return (T + kChunk + 2) * L * L * 4; }
 
 // Planes per slab: the requested count if it divides L and fits, else the
-// largest divisor of L up to the cap that fits in LDS; 0 if none does.
-uint32_t choose_slab(uint32_t L, int requested, uint32_t cap = kMaxSlab) {
+// thickest divisor of L up to kMaxSlab that fits in LDS and keeps the slab
+// within max_cells cells, else the thinnest divisor that fits; 0 if none.
+uint32_t choose_slab(uint32_t L, int requested, uint64_t max_cells) {
   if (requested > 0 && L % (uint32_t)requested == 0 &&
       slab_lds_bytes(L, (uint32_t)requested) <= chunk_lds_limit())
     return (uint32_t)requested;
-  for (uint32_t T = cap; T >= 1; --T)
-    if (L % T == 0 && slab_lds_bytes(L, T) <= chunk_lds_limit()) return T;
-  return 0;
+  uint32_t thinnest = 0;
+  for (uint32_t T = kMaxSlab; T >= 1; --T)
+    if (L % T == 0 && slab_lds_bytes(L, T) <= chunk_lds_limit()) {
+      if ((uint64_t)T * L * L <= max_cells) return T;
+      thinnest = T;
+    }
+  return thinnest;
 }
 
-// Automatic slab thickness cap: thick slabs (less halo work) while the
-// handle still has at least one slab per CU (256 on MI355X); small handles
-// (one L=32 point) get one-plane slabs so that more CUs work.
-uint32_t auto_slab_cap(const cvf_sim* s) {
-  for (uint32_t cap = kMaxSlab; cap > 1; cap /= 2) {
-    uint64_t slabs = 0;
-    for (const auto& P : s->ph) {
-      const uint32_t T = choose_slab(P.L, 0, cap);
-      if (T) slabs += P.L / T;
-    }
-    if (slabs >= 256) return cap;
-  }
-  return 1;
+// Automatic slab size: at most the handle's cells per CU (256 on MI355X).
+// A fused launch lasts as long as its largest workgroup, so thick slabs
+// (less halo work) are taken only while no slab outgrows the CUs' even
+// share; small handles (one L=32 point) get one-plane slabs.
+uint64_t auto_slab_cells(const cvf_sim* s) {
+  uint64_t total = 0;
+  for (const auto& P : s->ph) total += P.N;
+  return std::max<uint64_t>(total / 256, 1);
 }
 
 // Block table over `count(P)` work items per point of `points`, `block`
@@ -515,7 +515,7 @@ int upload_group_steps(cvf_sim* s) {
 // largest-first onto the group with the fewest cells.
 int build_groups(cvf_sim* s) {
   HIPC(hipDeviceSynchronize());
-  const uint32_t slab_cap = auto_slab_cap(s);
+  const uint64_t slab_cells = auto_slab_cells(s);
   s->groups.clear();
   const int ng = std::max(1, std::min(s->tune.streams, s->npts));
   for (int g = 0; g < ng; ++g) s->groups.emplace_back(new Group);
@@ -555,7 +555,7 @@ int build_groups(cvf_sim* s) {
     g->sigma_cells = 0;
     for (int p : g->points) {
       const PointHost& P = s->ph[p];
-      const uint32_t T = choose_slab(P.L, s->tune.slab_planes, slab_cap);
+      const uint32_t T = choose_slab(P.L, s->tune.slab_planes, slab_cells);
       if (!T || !P.prm.flag_chessboard) {
         g->fused = false;
         break;
