@@ -287,6 +287,34 @@ __device__ __forceinline__ uint32_t sigma_thr32(uint64_t t) {
   return t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
 }
 
+// Acceptance-table index (h + 1) * 11 + d + 5 of a chessboard sigma move of
+// arm `arm` from sa to sp (sp != sa; any in-range index otherwise) on word w
+// with facing neighbour word wn, where h = eta ? delta(sp, snb) -
+// delta(sa, snb) : 0 and d = count_eq(w, sp) - count_eq(w, sa) + 1.
+// Branch-free: with sp != sa at most one of the deltas is 1, so (h + 1) * 11
+// is a two-way select; d + 5 = #(fields != sa) + #(fields == sp), each from
+// the zero-field test ((y & 011) + 011 | y) & 100 on every 3-bit field of
+// y = w ^ (s * 001) (no carry leaves a field), the counts chained through
+// the popcount's addend.
+// popcount(x) + acc in one v_bcnt_u32_b32 (written out so that the compiler
+// does not spread the table's x4 scaling over the terms of the sum)
+__device__ __forceinline__ uint32_t popc_add(uint32_t x, uint32_t acc) {
+  uint32_t r;
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+  return r;
+}
+
+__device__ __forceinline__ uint32_t sigma_thr_index(uint32_t w, uint32_t wn, int arm, uint32_t sa, uint32_t sp) {
+  constexpr uint32_t kLo2 = 0x1B6DBu, kHi = 0x24924u;  // bits 0-1, bit 2 of each field
+  const uint32_t snb = sigma_of(wn, arm ^ 1);
+  const bool eta = (w >> (kEtaShift + arm)) & 1u;
+  const uint32_t h11 = eta && sp == snb ? 22u : eta && sa == snb ? 0u : 11u;
+  const uint32_t ya = w ^ (sa * kFieldLsb), yp = w ^ (sp * kFieldLsb);
+  const uint32_t ne_a = (((ya & kLo2) + kLo2) | ya) & kHi;
+  const uint32_t eq_p = ~(((yp & kLo2) + kLo2) | yp) & kHi;
+  return popc_add(ne_a, popc_add(eq_p, h11));
+}
+
 // The sigma rule for the cells of K quads: words at W[cell[m]], facing
 // neighbours at W[nb[m]] (LDS or global memory); quads with ok[k] false are
 // skipped (their indices must still be in range).  The cells and their
@@ -309,11 +337,9 @@ __device__ __forceinline__ void sigma_quads(Ptr W, const uint32_t (&cell)[4 * K]
     const U4& q = r[m / 4];
     const uint32_t rw = (m & 3) == 0 ? q.x : (m & 3) == 1 ? q.y : (m & 3) == 2 ? q.z : q.w;
     split_word(rw, 6, sp[m], ra[m]);
-    const uint32_t sa = sigma_of(w[m], arm), snb = sigma_of(wn[m], arm ^ 1);
+    const uint32_t sa = sigma_of(w[m], arm);
     act[m] = ok[m / 4] && sp[m] != sa;
-    const int h = eta_of(w[m], arm) ? ((int)(sp[m] == snb) - (int)(sa == snb)) : 0;
-    const int d = count_eq(w[m], sp[m]) - count_eq(w[m], sa) + 1;
-    t[m] = thr[(h + 1) * 11 + d + 5];
+    t[m] = thr[sigma_thr_index(w[m], wn[m], arm, sa, sp[m])];
   }
 #pragma unroll
   for (int m = 0; m < 4 * K; ++m)
@@ -326,11 +352,9 @@ __device__ __forceinline__ uint32_t sigma_move_reg(uint32_t w, uint32_t wn, cons
                                                    uint32_t rw) {
   uint32_t sp, ra;
   split_word(rw, 6, sp, ra);
-  const uint32_t sa = sigma_of(w, arm), snb = sigma_of(wn, arm ^ 1);
-  if (sp == sa) return w;
-  const int h = eta_of(w, arm) ? ((int)(sp == snb) - (int)(sa == snb)) : 0;
-  const int d = count_eq(w, sp) - count_eq(w, sa) + 1;
-  return ra < thr[(h + 1) * 11 + d + 5] ? (w & ~(7u << (3 * arm))) | (sp << (3 * arm)) : w;
+  const uint32_t sa = sigma_of(w, arm);
+  return sp != sa && ra < thr[sigma_thr_index(w, wn, arm, sa, sp)] ? (w & ~(7u << (3 * arm))) | (sp << (3 * arm))
+                                                                    : w;
 }
 
 // Non-chessboard rule (gpu_water3D.cu:1966-1984, 2004-2007): with dN =
