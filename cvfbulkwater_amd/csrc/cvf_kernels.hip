@@ -315,39 +315,39 @@ __device__ __forceinline__ uint32_t sigma_thr_index(uint32_t w, uint32_t wn, int
   return popc_add(ne_a, popc_add(eq_p, h11));
 }
 
-// The sigma rule for the cells of K quads: words at W[cell[m]], facing
-// neighbours at W[nb[m]] (LDS or global memory); quads with ok[k] false are
-// skipped (their indices must still be in range).  The cells and their
-// neighbours are distinct words (active cells against the other parity)
-// and only the cells are written, so all loads are issued first and the
-// updates are computed without waiting on one another.
-template <int K, class Ptr>
-__device__ __forceinline__ void sigma_quads(Ptr W, const uint32_t (&cell)[4 * K], const uint32_t (&nb)[4 * K],
-                                            const uint32_t* thr, int arm, const U4 (&r)[K], const bool (&ok)[K]) {
-  uint32_t w[4 * K], wn[4 * K];
+// The sigma rule for the four cells of a quad: words at W[cell[m]], facing
+// neighbours at W[nb[m]] (LDS or global memory), random words from draw().
+// The cells and their neighbours are distinct words (active cells against
+// the other parity) and only the cells are written, so all eight loads are
+// issued first, the Philox block is computed while they are in flight, and
+// the four updates proceed without waiting on one another.
+template <class Ptr, class Draw>
+__device__ __forceinline__ void sigma_quad(Ptr W, const uint32_t (&cell)[4], const uint32_t (&nb)[4],
+                                           const uint32_t* thr, int arm, Draw&& draw) {
+  uint32_t w[4], wn[4];
 #pragma unroll
-  for (int m = 0; m < 4 * K; ++m) {
+  for (int m = 0; m < 4; ++m) {
     w[m] = W[cell[m]];
     wn[m] = W[nb[m]];
   }
-  uint32_t sp[4 * K], ra[4 * K], t[4 * K];
-  bool act[4 * K];
+  const U4 q = draw();
+  uint32_t sp[4], ra[4], t[4];
+  bool act[4];
 #pragma unroll
-  for (int m = 0; m < 4 * K; ++m) {
-    const U4& q = r[m / 4];
-    const uint32_t rw = (m & 3) == 0 ? q.x : (m & 3) == 1 ? q.y : (m & 3) == 2 ? q.z : q.w;
+  for (int m = 0; m < 4; ++m) {
+    const uint32_t rw = m == 0 ? q.x : m == 1 ? q.y : m == 2 ? q.z : q.w;
     split_word(rw, 6, sp[m], ra[m]);
     const uint32_t sa = sigma_of(w[m], arm);
-    act[m] = ok[m / 4] && sp[m] != sa;
+    act[m] = sp[m] != sa;
     t[m] = thr[sigma_thr_index(w[m], wn[m], arm, sa, sp[m])];
   }
 #pragma unroll
-  for (int m = 0; m < 4 * K; ++m)
+  for (int m = 0; m < 4; ++m)
     if (act[m] && ra[m] < t[m]) W[cell[m]] = (w[m] & ~(7u << (3 * arm))) | (sp[m] << (3 * arm));
 }
 
 // One chessboard sigma move of arm `arm` on the word w held in a register
-// (the rule of sigma_quads), facing neighbour word wn, random word rw.
+// (the rule of sigma_quad), facing neighbour word wn, random word rw.
 __device__ __forceinline__ uint32_t sigma_move_reg(uint32_t w, uint32_t wn, const uint32_t* thr, int arm,
                                                    uint32_t rw) {
   uint32_t sp, ra;
@@ -443,17 +443,15 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
     for (int m = 0; m < 4; ++m) nbc[m] = arm_neighbor(c[m], z, L * L, L, arm);
   }
   uint32_t* W = words + e.woff;
-  const U4 r = draw(seed, kStSigma, c0, step, sub);
   if (e.flags & kEntNoChess) {
+    const U4 r = draw(seed, kStSigma, c0, step, sub);
     sigma_update_nc(W, thr, c[0], nbc[0], arm, r.x);
     sigma_update_nc(W, thr, c[1], nbc[1], arm, r.y);
     sigma_update_nc(W, thr, c[2], nbc[2], arm, r.z);
     sigma_update_nc(W, thr, c[3], nbc[3], arm, r.w);
     return;
   }
-  const U4 rq[1] = {r};
-  const bool ok[1] = {true};
-  sigma_quads<1>(W, c, nbc, thr, arm, rq, ok);
+  sigma_quad(W, c, nbc, thr, arm, [&] { return draw(seed, kStSigma, c0, step, sub); });
 }
 
 // Cube update on LDS planes: Wp is the cube's origin plane (L x L words,
@@ -667,9 +665,7 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
         const uint32_t wz = wz0 + it.pi;
         uint32_t cell[4], nb[4];
         quad_plane_cells(wz * LL, (uint32_t)L, x, y, arm, ax, cell, nb);
-        const U4 r[1] = {draw(seed, kStSigma, x + L * y + gofs[wz], step, sub)};
-        const bool ok[1] = {true};
-        sigma_quads<1>(win, cell, nb, thr_s, arm, r, ok);
+        sigma_quad(win, cell, nb, thr_s, arm, [&] { return draw(seed, kStSigma, x + L * y + gofs[wz], step, sub); });
       }
     } else {
       // planes whose global z has the kept parity
@@ -682,9 +678,7 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
         const uint32_t wz = wz0 + 2 * it.pi;
         uint32_t cell[4], nb[4];
         quad_z_cells(wz * LL, (uint32_t)L, (int)LL, x, y, arm, cell, nb);
-        const U4 r[1] = {draw(seed, kStSigma, x + L * y + gofs[wz], step, sub)};
-        const bool ok[1] = {true};
-        sigma_quads<1>(win, cell, nb, thr_s, arm, r, ok);
+        sigma_quad(win, cell, nb, thr_s, arm, [&] { return draw(seed, kStSigma, x + L * y + gofs[wz], step, sub); });
       }
     }
     __syncthreads();
