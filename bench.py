@@ -196,7 +196,8 @@ def calibrate_copy(mib: float, reps: int = 10) -> int:
 
 def tuning_signature(kname: str, t: dict) -> str:
     if kname == "k_chunk":
-        return f"streams:{t['streams']},block:{t['block_chunk']},fused:1,slab:{t['slab_planes']}"
+        whole = ",chunk:12" if t.get("chunk_sweeps") == 12 else ""
+        return f"streams:{t['streams']},block:{t['block_chunk']},fused:1,slab:{t['slab_planes']}{whole}"
     return f"streams:{t['streams']},block:{t['block_sigma']},fused:0"
 
 

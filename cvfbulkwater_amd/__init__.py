@@ -64,7 +64,7 @@ class _Obs(C.Structure):
 class _Tuning(C.Structure):
     _fields_ = [("streams", C.c_int32), ("block_sigma", C.c_int32), ("block_eta", C.c_int32),
                 ("block_cell", C.c_int32), ("graph_steps", C.c_int32), ("fused", C.c_int32),
-                ("slab_planes", C.c_int32), ("block_chunk", C.c_int32)]
+                ("slab_planes", C.c_int32), ("block_chunk", C.c_int32), ("chunk_sweeps", C.c_int32)]
 
 
 class _CStat(C.Structure):
@@ -293,7 +293,7 @@ class Simulation:
                "cvf_set_shells")
 
     TUNING_FIELDS = ("streams", "block_sigma", "block_eta", "block_cell", "graph_steps", "fused",
-                     "slab_planes", "block_chunk")
+                     "slab_planes", "block_chunk", "chunk_sweeps")
 
     @property
     def tuning(self) -> dict:

@@ -84,6 +84,7 @@ struct Group {
   uint32_t nb_half = 0, nb_cube = 0, nb_cell = 0, nb_slab = 0;
   bool fused = false;        // eta + sigma through the fused slab kernel
   uint32_t chunk_lds = 0;    // dynamic LDS bytes of a fused launch
+  uint32_t chunk_lds_long = 0;  // the same for kMaxChunk half-sweeps; 0 if that does not fit
   size_t sigma_cells = 0;    // cells swept per fused launch
   // captured step sequences, longest first: graph_steps, then the powers of
   // two below it, so any step count is a few graph launches
@@ -418,7 +419,7 @@ BlockEnt make_entry(const PointHost& P, int point, uint32_t start) {
 
 // LDS words of a fused launch for slabs of T planes: T + one plane per
 // half-sweep of the chunk (worst-case halo) + one per side for the cubes.
-uint32_t slab_lds_bytes(uint32_t L, uint32_t T) { return (T + kChunk + 2) * L * L * 4; }
+uint32_t slab_lds_bytes(uint32_t L, uint32_t T, int chunk = kChunk) { return (T + chunk + 2) * L * L * 4; }
 
 // Planes per slab: the requested count if it divides L and fits, else the
 // thickest divisor of L up to kMaxSlab that fits in LDS and keeps the slab
@@ -552,6 +553,7 @@ int build_groups(cvf_sim* s) {
     g->fused = s->tune.fused != 0 && s->sched == CVF_SCHED_REFERENCE;
     std::vector<BlockEnt> ts;
     g->chunk_lds = 0;
+    bool long_fits = true;
     g->sigma_cells = 0;
     for (int p : g->points) {
       const PointHost& P = s->ph[p];
@@ -561,6 +563,8 @@ int build_groups(cvf_sim* s) {
         break;
       }
       g->chunk_lds = std::max(g->chunk_lds, slab_lds_bytes(P.L, T));
+      g->chunk_lds_long = std::max(g->chunk_lds_long, slab_lds_bytes(P.L, T, kMaxChunk));
+      long_fits = long_fits && slab_lds_bytes(P.L, T, kMaxChunk) <= chunk_lds_limit();
       g->sigma_cells += P.N;
       for (uint32_t z0 = 0; z0 < P.L; z0 += T) {
         BlockEnt en = make_entry(P, p, z0);
@@ -568,6 +572,7 @@ int build_groups(cvf_sim* s) {
         ts.push_back(en);
       }
     }
+    if (!long_fits) g->chunk_lds_long = 0;
     if (g->fused) {
       // largest slabs first (the kernel takes entries in launch order, and
       // each XCD's dispatcher hands out its share in that order)
@@ -607,8 +612,12 @@ struct Untimed {
 // Lattice part of one MC step of group g (MonteCarlo_Step,
 // gpu_water3D.cu:1346-1733): the step is the group's counter + step_off.
 // `timed(class, launch)` wraps every launch (profiling) or just makes it.
+// The fused launches ping-pong between the two word arrays: `par` says which
+// one holds the state (1: words2) and is updated; with `whole`, each launch
+// runs a whole Metropolis pass (kMaxChunk half-sweeps) instead of kChunk.
 template <class Timed>
-void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Timed&& timed) {
+void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Timed&& timed, int& par,
+                     bool whole = false) {
   Group& g = *s->groups[gi];
   const uint64_t* stepp = s->cur_steps.p + gi;
   const Launch lh{g.tab_half.p, g.nb_half, (uint32_t)s->tune.block_sigma, stepp, step_off};
@@ -621,16 +630,17 @@ void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Time
     for (int pass = 0; pass < 2 * s->nm; ++pass)
       timed(CVF_KCLASS_SIGMA, [&] { launch_sigma_f(st, lh, s->pts.p, s->words.p, (uint32_t)pass); });
   } else if (g.fused && s->nm > 0) {
-    // 12*n_metropolis half-sweeps in chunks of kChunk, ping-pong between the
-    // two word buffers (an even number of chunks ends in `words`)
+    // 12*n_metropolis half-sweeps in launches of kChunk (kMaxChunk when whole)
     const Launch lz{g.tab_slab.p, g.nb_slab, (uint32_t)s->tune.block_chunk, stepp, step_off};
-    const int nchunks = 12 * s->nm / kChunk;
-    for (int c = 0; c < nchunks; ++c) {
-      const uint32_t* src = (c & 1) ? s->words2.p : s->words.p;
-      uint32_t* dst = (c & 1) ? s->words.p : s->words2.p;
+    const int len = whole ? kMaxChunk : kChunk;
+    const uint32_t lds = whole ? g.chunk_lds_long : g.chunk_lds;
+    for (int c = 0; c < 12 * s->nm / len; ++c) {
+      const uint32_t* src = par ? s->words2.p : s->words.p;
+      uint32_t* dst = par ? s->words.p : s->words2.p;
       timed(CVF_KCLASS_CHUNK, [&] {
-        launch_chunk(st, lz, s->pts.p, src, dst, (uint32_t)(c * kChunk), kChunk, c == 0, g.chunk_lds);
+        launch_chunk(st, lz, s->pts.p, src, dst, (uint32_t)(c * len), (uint32_t)len, c == 0, lds);
       });
+      par ^= 1;
     }
   } else {
     for (int f = 0; f < 2 && g.nb_cube; ++f)
@@ -651,10 +661,20 @@ constexpr uint32_t kHistSpan = 4096;  // per-step histograms per copy back
 // kernel with the device volume moves of all of them (the chain never reads
 // the lattice) that also moves the group's step counter on.  With rec, the
 // sigma histogram of point s->rec_pt follows every step.
+//
+// Whole-pass fused launches (chunk_sweeps = 12) leave the state in words2
+// after an odd count, so they are used only where nothing reads the lattice
+// between steps (no SW step, no per-step histogram), and a sequence with an
+// odd launch count runs its last step in kChunk launches: every sequence
+// ends with the state in `words`.
 int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps, bool rec = false) {
   Group& g = *s->groups[gi];
+  const bool whole_ok = s->tune.chunk_sweeps == kMaxChunk && g.fused && g.chunk_lds_long != 0 &&
+                        s->sched == CVF_SCHED_REFERENCE && s->nc == 0 && !rec;
+  int par = 0;
   for (int i = 0; i < nsteps; ++i) {
-    enqueue_lattice(s, gi, st, (uint32_t)i, Untimed{});
+    const bool whole = whole_ok && !(i == nsteps - 1 && ((nsteps * s->nm) & 1));
+    enqueue_lattice(s, gi, st, (uint32_t)i, Untimed{}, par, whole);
     if (rec) {
       const PointHost& P = s->ph[s->rec_pt];
       launch_sigma_hist_slot(st, s->words.p + P.woff, P.N, s->hist_series.p, s->cur_steps.p + gi, (uint32_t)i,
@@ -756,6 +776,8 @@ int check_tuning(const cvf_tuning& t) {
   if (t.slab_planes < 0 || t.slab_planes > kMaxSlab)
     return fail(CVF_EINVAL, "slab_planes must be in [0, %d]", kMaxSlab);
   if (t.fused != 0 && t.fused != 1) return fail(CVF_EINVAL, "fused must be 0 or 1");
+  if (t.chunk_sweeps != 0 && t.chunk_sweeps != kChunk && t.chunk_sweeps != kMaxChunk)
+    return fail(CVF_EINVAL, "chunk_sweeps must be 0, %d or %d", kChunk, kMaxChunk);
   return CVF_OK;
 }
 
@@ -769,6 +791,7 @@ cvf_tuning default_tuning(const cvf_sim* s) {
   t.fused = 1;
   t.slab_planes = 0;
   t.block_chunk = 512;
+  t.chunk_sweeps = kChunk;
   return t;
 }
 
@@ -1285,7 +1308,8 @@ int cvf_profile_steps(cvf_sim* s, int64_t n_steps, double* kernel_ms, int64_t* l
     for (size_t gi = 0; gi < s->groups.size(); ++gi) {
       Group& g = *s->groups[gi];
       hipStream_t st = g.stream;
-      enqueue_lattice(s, (int)gi, st, 0, timed);
+      int par = 0;
+      enqueue_lattice(s, (int)gi, st, 0, timed, par);
       timed(CVF_KCLASS_BEGIN, [&] {
         launch_vol_chain(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, s->cur_steps.p + gi, 1,
                          s->dist.p, s->freq.p, s->precision, s->vol_device);
@@ -1355,6 +1379,14 @@ std::string tune_key(cvf_sim* s) {
   return k;
 }
 
+// Some stream group can run whole-pass fused launches (chunk_sweeps = 12).
+bool whole_passes_possible(const cvf_sim* s) {
+  if (s->sched != CVF_SCHED_REFERENCE || s->nc != 0 || s->nm <= 0) return false;
+  for (const auto& g : s->groups)
+    if (g->fused && g->chunk_lds_long != 0) return true;
+  return false;
+}
+
 bool cache_lookup(const std::string& key, cvf_tuning* t) {
   const char* path = getenv("CVF_TUNE_CACHE");
   if (!path || !*path) return false;
@@ -1365,8 +1397,8 @@ bool cache_lookup(const std::string& key, cvf_tuning* t) {
   while (fgets(line, sizeof line, f)) {
     char k[768];
     cvf_tuning c;
-    if (sscanf(line, "%767s %d %d %d %d %d %d %d %d", k, &c.streams, &c.block_sigma, &c.block_eta,
-               &c.block_cell, &c.graph_steps, &c.fused, &c.slab_planes, &c.block_chunk) == 9 &&
+    if (sscanf(line, "%767s %d %d %d %d %d %d %d %d %d", k, &c.streams, &c.block_sigma, &c.block_eta,
+               &c.block_cell, &c.graph_steps, &c.fused, &c.slab_planes, &c.block_chunk, &c.chunk_sweeps) == 10 &&
         key == k) {
       *t = c;
       found = true;
@@ -1381,8 +1413,8 @@ void cache_store(const std::string& key, const cvf_tuning& c) {
   if (!path || !*path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %d %d %d %d\n", key.c_str(), c.streams, c.block_sigma, c.block_eta,
-          c.block_cell, c.graph_steps, c.fused, c.slab_planes, c.block_chunk);
+  fprintf(f, "%s %d %d %d %d %d %d %d %d %d\n", key.c_str(), c.streams, c.block_sigma, c.block_eta,
+          c.block_cell, c.graph_steps, c.fused, c.slab_planes, c.block_chunk, c.chunk_sweeps);
   fclose(f);
 }
 
@@ -1431,14 +1463,18 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
       continue;
     }
     for (int slab : {0, 8, 4, 2, 1})
-      for (int bc : {256, 512, 1024}) {
-        cvf_tuning t = base;
-        t.streams = streams;
-        t.fused = 1;
-        t.slab_planes = slab;
-        t.block_chunk = bc;
-        cand.push_back(t);
-      }
+      for (int bc : {256, 512, 1024})
+        for (int len : {kChunk, kMaxChunk}) {
+          // whole-pass launches only where the wider windows can fit
+          if (len == kMaxChunk && (slab == 8 || s->nc > 0)) continue;
+          cvf_tuning t = base;
+          t.streams = streams;
+          t.fused = 1;
+          t.slab_planes = slab;
+          t.block_chunk = bc;
+          t.chunk_sweeps = len;
+          cand.push_back(t);
+        }
     for (int bs : {64, 256}) {
       cvf_tuning t = base;
       t.streams = streams;
@@ -1453,6 +1489,9 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
   double best_ms = 1e30;
   for (const auto& t : cand) {
     if ((rc = cvf_set_tuning(s, &t))) break;
+    // a whole-pass candidate that no stream group can take launches exactly
+    // as its 6-sweep twin: not timed, so the result names what runs
+    if (t.chunk_sweeps == kMaxChunk && !whole_passes_possible(s)) continue;
     if ((rc = cvf_step(s, 2))) break;
     HIPC(hipEventRecord(a, s->stream));
     if ((rc = cvf_step(s, steps_per_trial))) break;

@@ -50,7 +50,8 @@ struct BlockEnt {
 
 constexpr uint32_t kEntNoChess = 1;
 
-constexpr int kChunk = 6;    // half-sweeps per fused kernel
+constexpr int kChunk = 6;      // half-sweeps per fused kernel (default)
+constexpr int kMaxChunk = 12;  // half-sweeps per fused kernel at most: a whole Metropolis pass
 constexpr uint32_t kObserveSpan = 4096;  // cells per k_observe workgroup (256 threads)
 constexpr int kMaxSlab = 16;  // planes per slab of the fused kernel
 

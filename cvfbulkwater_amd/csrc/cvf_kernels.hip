@@ -516,8 +516,8 @@ __device__ __forceinline__ void quad_z_cells(uint32_t base, uint32_t L, int LL, 
 
 // ------------------------------------------------------ fused slab kernel
 // One workgroup per z-slab of T full x-y planes of one state point.  The
-// slab plus a z-halo is staged in LDS, the optional eta sweep and kChunk
-// consecutive sigma half-sweeps run there, and the slab's own planes are
+// slab plus a z-halo is staged in LDS, the optional eta sweep and nk
+// (kChunk or kMaxChunk) consecutive sigma half-sweeps run there, and the slab's own planes are
 // written to the other buffer of a ping-pong pair.  The halo is sized from
 // the chunk's own schedule: walking the chunk backwards, every +z (-z)
 // half-sweep widens the region that must be valid beforehand by one plane
@@ -550,7 +550,7 @@ __device__ __forceinline__ void plane_next(PlaneIter& it, uint32_t Q, uint32_t d
   }
 }
 
-constexpr int kMaxWin = kMaxSlab + kChunk + 2;  // window planes incl. halos
+constexpr int kMaxWin = kMaxSlab + kMaxChunk + 2;  // window planes incl. halos
 
 __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
                         uint32_t step_off,
@@ -560,7 +560,7 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   __shared__ uint32_t thr_s[33];
   __shared__ uint64_t thr_e[9];
   __shared__ uint64_t etab[9];
-  __shared__ uint32_t sch[kChunk];
+  __shared__ uint32_t sch[kMaxChunk];
   __shared__ uint32_t gofs[kMaxWin];  // L^2 * global z of each window plane
   // the table is ordered by decreasing slab work and used in launch order,
   // so every XCD's dispatcher takes the large slabs first

@@ -8,7 +8,7 @@ engine from integer totals).
 import numpy as np
 import pytest
 
-from cvfbulkwater_amd import FP32_COMPENSATED, VOLUME_DEVICE, VOLUME_HOST, Params, Simulation
+from cvfbulkwater_amd import FP32_COMPENSATED, VOLUME_DEVICE, VOLUME_HOST, CVFError, Params, Simulation
 from oracle import py as O
 
 pytestmark = pytest.mark.gpu
@@ -201,6 +201,44 @@ def test_fused_and_unfused_paths_edge_cases(fused, L, nm, nc, steps):
         if nc:
             assert np.array_equal(eng.labels(), ora.labels()), f"labels step {t}"
     assert_same_obs(eng, ora, "end")
+
+
+@pytest.mark.parametrize("nm,graph_steps,steps", [(1, 16, 37), (1, 3, 20), (2, 5, 13), (3, 4, 9)])
+def test_whole_pass_launches_match_oracle(nm, graph_steps, steps):
+    """chunk_sweeps = 12: one fused launch per Metropolis pass.  The state
+    alternates between the two word arrays and a sequence with an odd launch
+    count runs its last step in 6-sweep launches; bit-exact with the oracle
+    after calls of 1, 2, 3 and `steps` steps (several graph sizes each)."""
+    kw = dict(L=16, T=0.7, P=0.4, seed=2024, n_metropolis=nm, n_cluster=0)
+    eng = Simulation(Params(volume_mode=VOLUME_DEVICE, **kw))
+    eng.tuning = {"fused": 1, "chunk_sweeps": 12, "graph_steps": graph_steps, "slab_planes": 1}
+    assert eng.tuning["chunk_sweeps"] == 12
+    ora = O.OracleSim(**kw)
+    done = 0
+    for n in (1, 2, 3, steps):
+        eng.step(n)
+        ora.step(n)
+        done += n
+        assert_same_state(eng, ora, f"after {done} steps")
+    assert_same_obs(eng, ora, "end")
+
+
+def test_whole_pass_launches_in_a_batch_with_a_wide_point():
+    """A whole-pass tuning on a batch whose L=64 point cannot hold the wider
+    window: that stream group keeps 6-sweep launches, the other takes whole
+    passes, and every point stays equal to its default-tuning run."""
+    pts = [Params(L=L, T=0.6 + 0.05 * i, P=0.2, seed=31 + i, volume_mode=VOLUME_DEVICE)
+           for i, L in enumerate((64, 8, 12, 16))]
+    ref, alt = Simulation(pts), Simulation(pts)
+    alt.tuning = {"fused": 1, "chunk_sweeps": 12, "streams": 2, "slab_planes": 0}
+    ref.step(11)
+    alt.step(11)
+    for i in range(len(pts)):
+        a, b = ref.get_config(i), alt.get_config(i)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
+        assert ref.volume(i) == alt.volume(i)
+    with pytest.raises(CVFError):
+        alt.tuning = {"chunk_sweeps": 8}
 
 
 def test_lattice_above_the_fused_limit_runs_unfused():
