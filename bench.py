@@ -503,7 +503,7 @@ def main() -> int:
                          "(skips the autotuner)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-steps", type=int, default=5000)
-    ap.add_argument("--only", choices=["batch", "plain", "precision", "cluster", "reweight", "fast"],
+    ap.add_argument("--only", choices=["batch", "plain", "plain1", "precision", "cluster", "reweight", "fast"],
                     default=None,
                     help="run one leg only (for profiling)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
@@ -551,14 +551,15 @@ def main() -> int:
         achieved = k_bytes / (k_ms * 1e-3) / 1e9
         out.update(batch_ms=ms, batch_cells=cells, batch_rate=cells / (ms * 1e-3), prof=prof, tuning=tuning,
                    kname=kname, sig_ms=k_ms, sig_bytes=k_bytes, achieved=achieved)
-    if args.only in (None, "plain"):
+    if args.only in (None, "plain", "plain1"):
         sim = Simulation(Params(L=32, T=0.815, P=0.45, seed=44648604, precision=FP64,
                                 volume_mode=VOLUME_HOST))
         ptuning = sim.autotune(args.tune_steps) if args.tune_steps else sim.tuning
         pms = time_steps(sim, args.plain_steps, args.warmup, ev)
         sim.close()
-        out.update(plain_ms=pms, plain_rate=32 ** 3 / (pms * 1e-3), plain_tuning=ptuning,
-                   seq=plain_per_L(max(args.plain_steps // 4, 100), args.warmup, ev),
+        out.update(plain_ms=pms, plain_rate=32 ** 3 / (pms * 1e-3), plain_tuning=ptuning)
+    if args.only in (None, "plain"):
+        out.update(seq=plain_per_L(max(args.plain_steps // 4, 100), args.warmup, ev),
                    cluster=cluster_form(max(args.plain_steps // 4, 100), args.warmup, ev),
                    sampled=plain_sampled(args.plain_steps, args.warmup, ev),
                    nonchess=nonchess_form(max(args.plain_steps // 4, 100), args.warmup, ev,
@@ -608,6 +609,7 @@ def main() -> int:
                                      "volume move", "value": out["plain_rate"],
                          "ms_per_step": out["plain_ms"], "steps": args.plain_steps,
                          "tuning": out["plain_tuning"]}
+    if "seq" in out:
         line["cluster_form"] = out["cluster"]
         line["plain_sampled"] = out["sampled"]
         line["nonchess_form"] = out["nonchess"]
@@ -615,6 +617,7 @@ def main() -> int:
         line["plain_sequential_128"] = {
             "workload": "the 128 batch state points as sequential plain runs (FP64, host volume "
                         "move), from the per-L plain step times", **out["seq"]}
+    if "plain_rate" in out:
         if "batch_rate" in out:
             line["batched_over_plain"] = out["batch_rate"] / out["plain_rate"]
             line["batched_over_sequential_plain"] = out["batch_rate"] / out["seq"]["value"]
