@@ -85,6 +85,7 @@ struct Group {
   bool fused = false;        // eta + sigma through the fused slab kernel
   uint32_t chunk_lds = 0;    // dynamic LDS bytes of a fused launch
   uint32_t chunk_lds_long = 0;  // the same for kMaxChunk half-sweeps; 0 if that does not fit
+  uint32_t chunk_lds_eta = 0;   // the same for the eta sweep alone (fast schedule)
   size_t sigma_cells = 0;    // cells swept per fused launch
   // captured step sequences, longest first: graph_steps, then the powers of
   // two below it, so any step count is a few graph launches
@@ -548,11 +549,13 @@ int build_groups(cvf_sim* s) {
     g->nb_cell = (uint32_t)tl.size();
     std::vector<int32_t> pl(g->points.begin(), g->points.end());
     if ((rc = upload_vec(g->plist, pl))) return rc;
-    // fused path when every point's slab fits in LDS (the reference
-    // schedule's half-sweeps only)
-    g->fused = s->tune.fused != 0 && s->sched == CVF_SCHED_REFERENCE;
+    // fused path when every point's slab fits in LDS: the reference
+    // schedule's eta sweep and half-sweeps, or the fast schedule's eta sweep
+    g->fused = s->tune.fused != 0;
     std::vector<BlockEnt> ts;
     g->chunk_lds = 0;
+    g->chunk_lds_long = 0;
+    g->chunk_lds_eta = 0;
     bool long_fits = true;
     g->sigma_cells = 0;
     for (int p : g->points) {
@@ -564,6 +567,7 @@ int build_groups(cvf_sim* s) {
       }
       g->chunk_lds = std::max(g->chunk_lds, slab_lds_bytes(P.L, T));
       g->chunk_lds_long = std::max(g->chunk_lds_long, slab_lds_bytes(P.L, T, kMaxChunk));
+      g->chunk_lds_eta = std::max(g->chunk_lds_eta, slab_lds_bytes(P.L, T, 0));
       long_fits = long_fits && slab_lds_bytes(P.L, T, kMaxChunk) <= chunk_lds_limit();
       g->sigma_cells += P.N;
       for (uint32_t z0 = 0; z0 < P.L; z0 += T) {
@@ -613,27 +617,39 @@ struct Untimed {
 // gpu_water3D.cu:1346-1733): the step is the group's counter + step_off.
 // `timed(class, launch)` wraps every launch (profiling) or just makes it.
 // The fused launches ping-pong between the two word arrays: `par` says which
-// one holds the state (1: words2) and is updated; with `whole`, each launch
-// runs a whole Metropolis pass (kMaxChunk half-sweeps) instead of kChunk.
+// one holds the state (1: words2) and is updated.  `variant` selects the
+// step's other fused form: with the reference schedule, whole-pass launches
+// (kMaxChunk half-sweeps each instead of kChunk); with the fast schedule, the
+// eta sweep as one slab launch instead of one in-place launch per family.
 template <class Timed>
 void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Timed&& timed, int& par,
-                     bool whole = false) {
+                     bool variant = false) {
   Group& g = *s->groups[gi];
   const uint64_t* stepp = s->cur_steps.p + gi;
   const Launch lh{g.tab_half.p, g.nb_half, (uint32_t)s->tune.block_sigma, stepp, step_off};
   const Launch lc{g.tab_cube.p, g.nb_cube, (uint32_t)s->tune.block_eta, stepp, step_off};
   const Launch ll{g.tab_cell.p, g.nb_cell, (uint32_t)s->tune.block_cell, stepp, step_off};
   if (s->sched == CVF_SCHED_FAST) {
-    // eta cube sweep, then n_metropolis x the two colour passes
-    for (int f = 0; f < 2 && g.nb_cube; ++f)
-      timed(CVF_KCLASS_ETA, [&] { launch_eta(st, lc, s->pts.p, s->words.p, f); });
+    // eta cube sweep, then n_metropolis x the two colour passes (in place,
+    // on the array holding the state)
+    uint32_t* cur = par ? s->words2.p : s->words.p;
+    if (variant) {
+      uint32_t* nxt = par ? s->words.p : s->words2.p;
+      const Launch lz{g.tab_slab.p, g.nb_slab, (uint32_t)s->tune.block_chunk, stepp, step_off};
+      timed(CVF_KCLASS_CHUNK, [&] { launch_chunk(st, lz, s->pts.p, cur, nxt, 0, 0, 1, g.chunk_lds_eta); });
+      par ^= 1;
+      cur = nxt;
+    } else {
+      for (int f = 0; f < 2 && g.nb_cube; ++f)
+        timed(CVF_KCLASS_ETA, [&] { launch_eta(st, lc, s->pts.p, cur, f); });
+    }
     for (int pass = 0; pass < 2 * s->nm; ++pass)
-      timed(CVF_KCLASS_SIGMA, [&] { launch_sigma_f(st, lh, s->pts.p, s->words.p, (uint32_t)pass); });
+      timed(CVF_KCLASS_SIGMA, [&] { launch_sigma_f(st, lh, s->pts.p, cur, (uint32_t)pass); });
   } else if (g.fused && s->nm > 0) {
-    // 12*n_metropolis half-sweeps in launches of kChunk (kMaxChunk when whole)
+    // 12*n_metropolis half-sweeps in launches of kChunk (kMaxChunk in the variant form)
     const Launch lz{g.tab_slab.p, g.nb_slab, (uint32_t)s->tune.block_chunk, stepp, step_off};
-    const int len = whole ? kMaxChunk : kChunk;
-    const uint32_t lds = whole ? g.chunk_lds_long : g.chunk_lds;
+    const int len = variant ? kMaxChunk : kChunk;
+    const uint32_t lds = variant ? g.chunk_lds_long : g.chunk_lds;
     for (int c = 0; c < 12 * s->nm / len; ++c) {
       const uint32_t* src = par ? s->words2.p : s->words.p;
       uint32_t* dst = par ? s->words.p : s->words2.p;
@@ -662,19 +678,28 @@ constexpr uint32_t kHistSpan = 4096;  // per-step histograms per copy back
 // the lattice) that also moves the group's step counter on.  With rec, the
 // sigma histogram of point s->rec_pt follows every step.
 //
-// Whole-pass fused launches (chunk_sweeps = 12) leave the state in words2
-// after an odd count, so they are used only where nothing reads the lattice
-// between steps (no SW step, no per-step histogram), and a sequence with an
-// odd launch count runs its last step in kChunk launches: every sequence
-// ends with the state in `words`.
+// The steps' variant forms (whole-pass launches with chunk_sweeps = 12; the
+// fast schedule's slab eta sweep) can leave the state in words2, so they are
+// used only where nothing reads the lattice between steps (no SW step, no
+// per-step histogram), and the last step of a sequence takes its plain form
+// when the variant would end there in words2: every sequence ends with the
+// state in `words`.
 int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps, bool rec = false) {
   Group& g = *s->groups[gi];
-  const bool whole_ok = s->tune.chunk_sweeps == kMaxChunk && g.fused && g.chunk_lds_long != 0 &&
-                        s->sched == CVF_SCHED_REFERENCE && s->nc == 0 && !rec;
+  const bool ok = g.fused && s->nc == 0 && !rec && s->nm > 0;
+  bool variant_ok = false;
+  int flips = 0;  // word-array swaps of one variant step
+  if (s->sched == CVF_SCHED_REFERENCE) {
+    variant_ok = ok && s->tune.chunk_sweeps == kMaxChunk && g.chunk_lds_long != 0;
+    flips = s->nm & 1;
+  } else if (s->sched == CVF_SCHED_FAST) {
+    variant_ok = ok && g.nb_cube != 0;
+    flips = 1;
+  }
   int par = 0;
   for (int i = 0; i < nsteps; ++i) {
-    const bool whole = whole_ok && !(i == nsteps - 1 && ((nsteps * s->nm) & 1));
-    enqueue_lattice(s, gi, st, (uint32_t)i, Untimed{}, par, whole);
+    const bool variant = variant_ok && (i < nsteps - 1 || (par ^ flips) == 0);
+    enqueue_lattice(s, gi, st, (uint32_t)i, Untimed{}, par, variant);
     if (rec) {
       const PointHost& P = s->ph[s->rec_pt];
       launch_sigma_hist_slot(st, s->words.p + P.woff, P.N, s->hist_series.p, s->cur_steps.p + gi, (uint32_t)i,
@@ -1452,14 +1477,19 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
   for (int streams : {1, 2, 4}) {
     if (streams > s->npts) continue;
     if (s->sched == CVF_SCHED_FAST) {
-      // colour passes never use the fused slab kernel: workgroup sizes only
-      for (int bs : {64, 128, 256, 512}) {
-        cvf_tuning t = base;
-        t.streams = streams;
-        t.fused = 0;
-        t.block_sigma = t.block_eta = bs;
-        cand.push_back(t);
-      }
+      // colour-pass workgroup sizes, the eta sweep per family or as one slab
+      // launch (fused)
+      for (int bs : {64, 128, 256, 512})
+        for (int fused : {0, 1})
+          for (int bc : {512, 1024}) {
+            if (!fused && bc != 512) continue;
+            cvf_tuning t = base;
+            t.streams = streams;
+            t.fused = fused;
+            t.block_chunk = bc;
+            t.block_sigma = t.block_eta = bs;
+            cand.push_back(t);
+          }
       continue;
     }
     for (int slab : {0, 8, 4, 2, 1})

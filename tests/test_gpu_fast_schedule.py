@@ -65,13 +65,34 @@ def test_fast_schedule_tuning_does_not_change_results(tuning):
         assert ref.volume(i) == alt.volume(i)
 
 
+@pytest.mark.parametrize("Ls,nm,graph_steps", [([8, 16, 12], 1, 16), ([24], 2, 3), ([4, 20], 1, 5)])
+def test_fast_schedule_slab_eta_matches_oracle(Ls, nm, graph_steps):
+    """fused = 1 with the fast schedule: the eta sweep of each step as one
+    LDS slab launch into the other word array (the colour passes then run on
+    that array); sequences with an odd step count end with the per-family
+    eta kernels.  Bit-exact with the oracle after calls of 1, 2, 3 and 9
+    steps."""
+    pts = fast_points(Ls, nc=0, nm=nm)
+    eng = Simulation(pts)
+    eng.tuning = {"fused": 1, "graph_steps": graph_steps}
+    oras = [O.OracleSim(L=p.L, T=p.T, P=p.P, seed=p.seed, n_metropolis=nm, n_cluster=0, schedule=1)
+            for p in pts]
+    for n in (1, 2, 3, 9):
+        eng.step(n)
+        for i, o in enumerate(oras):
+            o.step(n)
+            a, b = eng.get_config(i), o.get_config()
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (Ls[i], n)
+            assert eng.volume(i) == o.volume()
+
+
 def test_fast_schedule_autotune_restores_state():
     pts = fast_points([16, 24, 32], precision=FP32_COMPENSATED)
     a, b = Simulation(pts), Simulation(pts)
     a.step(5)
     b.step(5)
     chosen = b.autotune(4)
-    assert chosen["fused"] == 0
+    assert b.tuning == chosen
     a.step(12)
     b.step(12)
     for i in range(len(pts)):
