@@ -86,6 +86,7 @@ struct Group {
   uint32_t chunk_lds = 0;    // dynamic LDS bytes of a fused launch
   uint32_t chunk_lds_long = 0;  // the same for kMaxChunk half-sweeps; 0 if that does not fit
   uint32_t chunk_lds_eta = 0;   // the same for the eta sweep alone (fast schedule)
+  bool any_nc = false;          // some point runs the non-chessboard rule
   size_t sigma_cells = 0;    // cells swept per fused launch
   // captured step sequences, longest first: graph_steps, then the powers of
   // two below it, so any step count is a few graph launches
@@ -419,19 +420,22 @@ BlockEnt make_entry(const PointHost& P, int point, uint32_t start) {
 }
 
 // LDS words of a fused launch for slabs of T planes: T + one plane per
-// half-sweep of the chunk (worst-case halo) + one per side for the cubes.
-uint32_t slab_lds_bytes(uint32_t L, uint32_t T, int chunk = kChunk) { return (T + chunk + 2) * L * L * 4; }
+// half-sweep of the chunk (worst-case halo) + one per side for the cubes;
+// with the non-chessboard rule (no cubes) two planes per half-sweep.
+uint32_t slab_lds_bytes(uint32_t L, uint32_t T, int chunk = kChunk, bool nc = false) {
+  return (T + (nc ? 2 * chunk : chunk + 2)) * L * L * 4;
+}
 
 // Planes per slab: the requested count if it divides L and fits, else the
 // thickest divisor of L up to kMaxSlab that fits in LDS and keeps the slab
 // within max_cells cells, else the thinnest divisor that fits; 0 if none.
-uint32_t choose_slab(uint32_t L, int requested, uint64_t max_cells) {
+uint32_t choose_slab(uint32_t L, int requested, uint64_t max_cells, bool nc) {
   if (requested > 0 && L % (uint32_t)requested == 0 &&
-      slab_lds_bytes(L, (uint32_t)requested) <= chunk_lds_limit())
+      slab_lds_bytes(L, (uint32_t)requested, kChunk, nc) <= chunk_lds_limit())
     return (uint32_t)requested;
   uint32_t thinnest = 0;
   for (uint32_t T = kMaxSlab; T >= 1; --T)
-    if (L % T == 0 && slab_lds_bytes(L, T) <= chunk_lds_limit()) {
+    if (L % T == 0 && slab_lds_bytes(L, T, kChunk, nc) <= chunk_lds_limit()) {
       if ((uint64_t)T * L * L <= max_cells) return T;
       thinnest = T;
     }
@@ -556,19 +560,22 @@ int build_groups(cvf_sim* s) {
     g->chunk_lds = 0;
     g->chunk_lds_long = 0;
     g->chunk_lds_eta = 0;
+    g->any_nc = false;
     bool long_fits = true;
     g->sigma_cells = 0;
     for (int p : g->points) {
       const PointHost& P = s->ph[p];
-      const uint32_t T = choose_slab(P.L, s->tune.slab_planes, slab_cells);
-      if (!T || !P.prm.flag_chessboard) {
+      const bool nc = !P.prm.flag_chessboard;
+      g->any_nc = g->any_nc || nc;
+      const uint32_t T = choose_slab(P.L, s->tune.slab_planes, slab_cells, nc);
+      if (!T) {
         g->fused = false;
         break;
       }
-      g->chunk_lds = std::max(g->chunk_lds, slab_lds_bytes(P.L, T));
-      g->chunk_lds_long = std::max(g->chunk_lds_long, slab_lds_bytes(P.L, T, kMaxChunk));
+      g->chunk_lds = std::max(g->chunk_lds, slab_lds_bytes(P.L, T, kChunk, nc));
+      g->chunk_lds_long = std::max(g->chunk_lds_long, slab_lds_bytes(P.L, T, kMaxChunk, nc));
       g->chunk_lds_eta = std::max(g->chunk_lds_eta, slab_lds_bytes(P.L, T, 0));
-      long_fits = long_fits && slab_lds_bytes(P.L, T, kMaxChunk) <= chunk_lds_limit();
+      long_fits = long_fits && slab_lds_bytes(P.L, T, kMaxChunk, nc) <= chunk_lds_limit();
       g->sigma_cells += P.N;
       for (uint32_t z0 = 0; z0 < P.L; z0 += T) {
         BlockEnt en = make_entry(P, p, z0);
@@ -636,7 +643,7 @@ void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Time
     if (variant) {
       uint32_t* nxt = par ? s->words.p : s->words2.p;
       const Launch lz{g.tab_slab.p, g.nb_slab, (uint32_t)s->tune.block_chunk, stepp, step_off};
-      timed(CVF_KCLASS_CHUNK, [&] { launch_chunk(st, lz, s->pts.p, cur, nxt, 0, 0, 1, g.chunk_lds_eta); });
+      timed(CVF_KCLASS_CHUNK, [&] { launch_chunk(st, lz, s->pts.p, cur, nxt, 0, 0, 1, g.chunk_lds_eta, false); });
       par ^= 1;
       cur = nxt;
     } else {
@@ -654,7 +661,7 @@ void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Time
       const uint32_t* src = par ? s->words2.p : s->words.p;
       uint32_t* dst = par ? s->words.p : s->words2.p;
       timed(CVF_KCLASS_CHUNK, [&] {
-        launch_chunk(st, lz, s->pts.p, src, dst, (uint32_t)(c * len), (uint32_t)len, c == 0, lds);
+        launch_chunk(st, lz, s->pts.p, src, dst, (uint32_t)(c * len), (uint32_t)len, c == 0, lds, g.any_nc);
       });
       par ^= 1;
     }

@@ -79,9 +79,10 @@ void launch_sigma(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* word
 // colour), over the half-sweep table (N/8 quads per point)
 void launch_sigma_f(hipStream_t st, Launch ln, const PointDev* pts, uint32_t* words, uint32_t pass);
 // Fused z-slab kernel: optional eta sweep + half-sweeps k0 .. k0+nk-1 of the
-// step on LDS-staged slabs, reading `src` and writing `dst`.
+// step on LDS-staged slabs, reading `src` and writing `dst`; any_nc: the
+// table holds non-chessboard slabs.
 void launch_chunk(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* src, uint32_t* dst,
-                  uint32_t k0, uint32_t nk, int with_eta, uint32_t lds_bytes);
+                  uint32_t k0, uint32_t nk, int with_eta, uint32_t lds_bytes, bool any_nc);
 // largest dynamic LDS a fused launch may use
 uint32_t chunk_lds_limit();
 void launch_zero_u32(hipStream_t st, uint32_t* p, size_t n);

@@ -383,6 +383,16 @@ __device__ __forceinline__ void sigma_update_nc(uint32_t* __restrict__ W, const 
   }
 }
 
+// The non-chessboard rule on the four cells of a quad (cells and facing
+// neighbours all distinct words).
+__device__ __forceinline__ void sigma_quad_nc(uint32_t* W, const uint32_t (&c)[4], const uint32_t (&nbc)[4],
+                                              const uint32_t* thr, int arm, const U4& r) {
+  sigma_update_nc(W, thr, c[0], nbc[0], arm, r.x);
+  sigma_update_nc(W, thr, c[1], nbc[1], arm, r.y);
+  sigma_update_nc(W, thr, c[2], nbc[2], arm, r.z);
+  sigma_update_nc(W, thr, c[3], nbc[3], arm, r.w);
+}
+
 // One thread per quad of active cells (x + b L/2, y + a L/2, z), a,b in
 // {0,1}, x,y < L/2: the four have the same parity along every axis (L/2 is
 // even) and share one Philox block keyed on the a=b=0 cell (word a + 2b).
@@ -444,11 +454,7 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   }
   uint32_t* W = words + e.woff;
   if (e.flags & kEntNoChess) {
-    const U4 r = draw(seed, kStSigma, c0, step, sub);
-    sigma_update_nc(W, thr, c[0], nbc[0], arm, r.x);
-    sigma_update_nc(W, thr, c[1], nbc[1], arm, r.y);
-    sigma_update_nc(W, thr, c[2], nbc[2], arm, r.z);
-    sigma_update_nc(W, thr, c[3], nbc[3], arm, r.w);
+    sigma_quad_nc(W, c, nbc, thr, arm, draw(seed, kStSigma, c0, step, sub));
     return;
   }
   sigma_quad(W, c, nbc, thr, arm, [&] { return draw(seed, kStSigma, c0, step, sub); });
@@ -550,8 +556,10 @@ __device__ __forceinline__ void plane_next(PlaneIter& it, uint32_t Q, uint32_t d
   }
 }
 
-constexpr int kMaxWin = kMaxSlab + kMaxChunk + 2;  // window planes incl. halos
+constexpr int kMaxWin = kMaxSlab + 2 * kMaxChunk;  // window planes incl. halos (either rule)
 
+// kAnyNc: the launch may hold non-chessboard slabs (compiled out otherwise).
+template <bool kAnyNc>
 __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
                         uint32_t step_off,
                         const PointDev* __restrict__ pts, const uint32_t* __restrict__ src,
@@ -584,14 +592,18 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   const FastDiv dH{(uint32_t)hL, e.mH}, dQ{(uint32_t)qL, e.mQ};
   const FastDiv dP8{(uint32_t)(LL / 8), e.mP8}, dP4{(uint32_t)(LL / 4), e.mP4},
       dC{(uint32_t)(qL * qL), e.mC};
-  // planes relative to z0: window [wlo, whi)
+  // planes relative to z0: window [wlo, whi).  A z half-sweep of the
+  // non-chessboard rule also writes the HB counter of each updated cell's
+  // facing neighbour, so it needs one more valid plane on both sides, and
+  // such a point has no cube moves.
+  const bool nc = kAnyNc && (e.flags & kEntNoChess) != 0;
   int h_lo = 0, h_hi = 0;
   for (uint32_t k = 0; k < nk; ++k) {
     const uint32_t arm = sch[k] & 7u;
-    h_hi += arm == 4;
-    h_lo += arm == 5;
+    h_hi += nc ? arm >= 4 : arm == 4;
+    h_lo += nc ? arm >= 4 : arm == 5;
   }
-  const int ext = with_eta ? 1 : 0;
+  const int ext = with_eta && !nc ? 1 : 0;
   const int wlo = -h_lo - ext, whi = T + h_hi + ext, np = whi - wlo;
   if (tid < (uint32_t)np) gofs[tid] = wrap_plane(z0 + wlo + (int)tid, L) * (uint32_t)LL;
   __syncthreads();
@@ -617,7 +629,7 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
     }
   }
   __syncthreads();
-  if (with_eta) {
+  if (ext) {
     // cubes with both planes inside the window, family by family
     const uint32_t Q = (uint32_t)(qL * qL), dp = fdiv(nt, dC), du = nt - dp * Q;
     for (int fam = 0; fam < 2; ++fam) {
@@ -645,9 +657,12 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
     const int arm = (int)(sch[k] & 7u);
     const uint32_t q = (sch[k] >> 3) ? 0u : 1u;
     const int ax = arm >> 1;
-    rem_hi -= arm == 4;
-    rem_lo -= arm == 5;
-    const int rlo = -rem_lo, rhi = T + rem_hi;  // planes updated by this half-sweep
+    rem_hi -= nc ? arm >= 4 : arm == 4;
+    rem_lo -= nc ? arm >= 4 : arm == 5;
+    // planes updated by this half-sweep; with the non-chessboard rule a +z
+    // (-z) sweep also updates the active plane just below (above) the region
+    // valid after it, for the counters it writes inside that region
+    const int rlo = -rem_lo - (nc && arm == 4), rhi = T + rem_hi + (nc && arm == 5);
     const uint32_t sub = k0 + k;
     if (ax < 2) {
       const uint32_t Q = (uint32_t)(LL / 8), dp = fdiv(nt, dP8), du = nt - dp * Q;
@@ -665,7 +680,11 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
         const uint32_t wz = wz0 + it.pi;
         uint32_t cell[4], nb[4];
         quad_plane_cells(wz * LL, (uint32_t)L, x, y, arm, ax, cell, nb);
-        sigma_quad(win, cell, nb, thr_s, arm, [&] { return draw(seed, kStSigma, x + L * y + gofs[wz], step, sub); });
+        const auto rnd = [&] { return draw(seed, kStSigma, x + L * y + gofs[wz], step, sub); };
+        if (nc)
+          sigma_quad_nc(win, cell, nb, thr_s, arm, rnd());
+        else
+          sigma_quad(win, cell, nb, thr_s, arm, rnd);
       }
     } else {
       // planes whose global z has the kept parity
@@ -678,7 +697,11 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
         const uint32_t wz = wz0 + 2 * it.pi;
         uint32_t cell[4], nb[4];
         quad_z_cells(wz * LL, (uint32_t)L, (int)LL, x, y, arm, cell, nb);
-        sigma_quad(win, cell, nb, thr_s, arm, [&] { return draw(seed, kStSigma, x + L * y + gofs[wz], step, sub); });
+        const auto rnd = [&] { return draw(seed, kStSigma, x + L * y + gofs[wz], step, sub); };
+        if (nc)
+          sigma_quad_nc(win, cell, nb, thr_s, arm, rnd());
+        else
+          sigma_quad(win, cell, nb, thr_s, arm, rnd);
       }
     }
     __syncthreads();
@@ -1134,8 +1157,9 @@ uint32_t chunk_lds_limit() {
   // gfx950: 160 KB of LDS per CU; keep room for the static arrays
   return 160 * 1024 - 1024;
 }
-void launch_chunk(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* src, uint32_t* dst,
-                  uint32_t k0, uint32_t nk, int with_eta, uint32_t lds_bytes) {
+template <bool kAnyNc>
+void launch_chunk_t(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* src, uint32_t* dst, uint32_t k0,
+                    uint32_t nk, int with_eta, uint32_t lds_bytes) {
   // the dynamic-LDS limit is a per-device function attribute: set it once
   // on each device the library launches on
   static std::atomic<uint64_t> attr_set{0};
@@ -1143,16 +1167,23 @@ void launch_chunk(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t
   (void)hipGetDevice(&dev);
   const uint64_t bit = 1ull << (dev & 63);
   if (!(attr_set.load(std::memory_order_acquire) & bit)) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chunk),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chunk<kAnyNc>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)chunk_lds_limit());
     attr_set.fetch_or(bit, std::memory_order_acq_rel);
   }
   if (g_t0)
-    hipExtLaunchKernelGGL(k_chunk, dim3(ln.nblocks), dim3(ln.block), lds_bytes, st, g_t0, g_t1, 0,
+    hipExtLaunchKernelGGL(k_chunk<kAnyNc>, dim3(ln.nblocks), dim3(ln.block), lds_bytes, st, g_t0, g_t1, 0,
                           ln.tab, ln.nblocks, ln.step, ln.step_off, pts, src, dst, k0, nk, with_eta);
   else
-    hipLaunchKernelGGL(k_chunk, dim3(ln.nblocks), dim3(ln.block), lds_bytes, st, ln.tab, ln.nblocks,
+    hipLaunchKernelGGL(k_chunk<kAnyNc>, dim3(ln.nblocks), dim3(ln.block), lds_bytes, st, ln.tab, ln.nblocks,
                        ln.step, ln.step_off, pts, src, dst, k0, nk, with_eta);
+}
+void launch_chunk(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t* src, uint32_t* dst,
+                  uint32_t k0, uint32_t nk, int with_eta, uint32_t lds_bytes, bool any_nc) {
+  if (any_nc)
+    launch_chunk_t<true>(st, ln, pts, src, dst, k0, nk, with_eta, lds_bytes);
+  else
+    launch_chunk_t<false>(st, ln, pts, src, dst, k0, nk, with_eta, lds_bytes);
 }
 void launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
   size_t blocks = (n + 255) / 256;

@@ -151,6 +151,29 @@ def test_nonchessboard_mode_every_step(L, nc, flag_config):
     assert eng.volume() == ora.volume()
 
 
+@pytest.mark.parametrize("tuning", [{"fused": 1, "slab_planes": 1}, {"fused": 1, "slab_planes": 2, "block_chunk": 256},
+                                    {"fused": 1, "slab_planes": 1, "chunk_sweeps": 12, "graph_steps": 3},
+                                    {"fused": 0}])
+@pytest.mark.parametrize("L", [4, 12])
+def test_nonchessboard_fused_slabs_match_oracle(tuning, L):
+    """The non-chessboard rule in the fused slab kernel (its z half-sweeps
+    write the facing neighbour's HB counter, so each widens the window on
+    both sides): sigma, eta and N_HB equal to the oracle after calls of 1, 2,
+    5 and 12 steps, for thin slabs (the window repeats planes at L=4),
+    whole-pass launches and the unfused kernels."""
+    kw = dict(L=L, T=0.6, P=0.45, seed=78, n_cluster=0)
+    eng = Simulation(Params(flag_chessboard=0, volume_mode=VOLUME_DEVICE, **kw))
+    eng.tuning = tuning
+    ora = O.OracleSim(flag_chessboard=0, **kw)
+    done = 0
+    for n in (1, 2, 5, 12):
+        eng.step(n)
+        ora.step(n)
+        done += n
+        assert_same_state(eng, ora, f"after {done} steps")
+        assert_same_obs(eng, ora, f"after {done} steps")
+
+
 def test_nonchessboard_input_config_keeps_eta():
     L, N = 8, 512
     rng = np.random.default_rng(9)
