@@ -671,10 +671,12 @@ void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Time
     for (int sub = 0; sub < 12 * s->nm; ++sub)
       timed(CVF_KCLASS_SIGMA, [&] { launch_sigma(st, lh, s->pts.p, s->words.p, (uint32_t)sub); });
   }
+  // Swendsen-Wang steps on the array holding the state
+  uint32_t* cur = par ? s->words2.p : s->words.p;
   for (int jc = 0; jc < s->nc; ++jc) {
-    timed(CVF_KCLASS_SW_LOCAL, [&] { launch_sw_local(st, ll, s->words.p, s->parent.p, s->cov.p, (uint32_t)jc); });
+    timed(CVF_KCLASS_SW_LOCAL, [&] { launch_sw_local(st, ll, cur, s->parent.p, s->cov.p, (uint32_t)jc); });
     timed(CVF_KCLASS_SW_UNION, [&] { launch_sw_union(st, ll, s->parent.p, s->cov.p); });
-    timed(CVF_KCLASS_SW_FLIP, [&] { launch_sw_flip(st, ll, s->words.p, s->parent.p, (uint32_t)jc); });
+    timed(CVF_KCLASS_SW_FLIP, [&] { launch_sw_flip(st, ll, cur, s->parent.p, (uint32_t)jc); });
   }
 }
 
@@ -686,14 +688,14 @@ constexpr uint32_t kHistSpan = 4096;  // per-step histograms per copy back
 // sigma histogram of point s->rec_pt follows every step.
 //
 // The steps' variant forms (whole-pass launches with chunk_sweeps = 12; the
-// fast schedule's slab eta sweep) can leave the state in words2, so they are
-// used only where nothing reads the lattice between steps (no SW step, no
-// per-step histogram), and the last step of a sequence takes its plain form
+// fast schedule's slab eta sweep) can leave the state in words2; the kernels
+// that read the lattice between steps (SW, the per-step histogram) take the
+// array holding it, and the last step of a sequence takes its plain form
 // when the variant would end there in words2: every sequence ends with the
 // state in `words`.
 int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps, bool rec = false) {
   Group& g = *s->groups[gi];
-  const bool ok = g.fused && s->nc == 0 && !rec && s->nm > 0;
+  const bool ok = g.fused && s->nm > 0;
   bool variant_ok = false;
   int flips = 0;  // word-array swaps of one variant step
   if (s->sched == CVF_SCHED_REFERENCE) {
@@ -709,8 +711,8 @@ int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps, bool rec = fal
     enqueue_lattice(s, gi, st, (uint32_t)i, Untimed{}, par, variant);
     if (rec) {
       const PointHost& P = s->ph[s->rec_pt];
-      launch_sigma_hist_slot(st, s->words.p + P.woff, P.N, s->hist_series.p, s->cur_steps.p + gi, (uint32_t)i,
-                             s->hist_base.p, kHistSpan);
+      launch_sigma_hist_slot(st, (par ? s->words2.p : s->words.p) + P.woff, P.N, s->hist_series.p,
+                             s->cur_steps.p + gi, (uint32_t)i, s->hist_base.p, kHistSpan);
     }
   }
   launch_vol_chain(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, s->cur_steps.p + gi,
@@ -823,7 +825,12 @@ cvf_tuning default_tuning(const cvf_sim* s) {
   t.fused = 1;
   t.slab_planes = 0;
   t.block_chunk = 512;
-  t.chunk_sweeps = kChunk;
+  // small handles are bound by per-launch cost: a whole Metropolis pass per
+  // fused launch where the window fits (not with the non-chessboard rule,
+  // whose 24-plane halo reserve costs more than the launch it saves)
+  bool any_nc = false;
+  for (const auto& P : s->ph) any_nc = any_nc || !P.prm.flag_chessboard;
+  t.chunk_sweeps = s->total_cells < (1u << 20) && !any_nc ? kMaxChunk : kChunk;
   return t;
 }
 
@@ -1413,7 +1420,7 @@ std::string tune_key(cvf_sim* s) {
 
 // Some stream group can run whole-pass fused launches (chunk_sweeps = 12).
 bool whole_passes_possible(const cvf_sim* s) {
-  if (s->sched != CVF_SCHED_REFERENCE || s->nc != 0 || s->nm <= 0) return false;
+  if (s->sched != CVF_SCHED_REFERENCE || s->nm <= 0) return false;
   for (const auto& g : s->groups)
     if (g->fused && g->chunk_lds_long != 0) return true;
   return false;
@@ -1494,6 +1501,7 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
             t.streams = streams;
             t.fused = fused;
             t.block_chunk = bc;
+            t.chunk_sweeps = kChunk;  // no half-sweeps in fused launches here
             t.block_sigma = t.block_eta = bs;
             cand.push_back(t);
           }
@@ -1503,7 +1511,7 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
       for (int bc : {256, 512, 1024})
         for (int len : {kChunk, kMaxChunk}) {
           // whole-pass launches only where the wider windows can fit
-          if (len == kMaxChunk && (slab == 8 || s->nc > 0)) continue;
+          if (len == kMaxChunk && slab == 8) continue;
           cvf_tuning t = base;
           t.streams = streams;
           t.fused = 1;
@@ -1516,6 +1524,7 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
       cvf_tuning t = base;
       t.streams = streams;
       t.fused = 0;
+      t.chunk_sweeps = kChunk;
       t.block_sigma = t.block_eta = bs;
       cand.push_back(t);
     }
@@ -1528,7 +1537,7 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
     if ((rc = cvf_set_tuning(s, &t))) break;
     // a whole-pass candidate that no stream group can take launches exactly
     // as its 6-sweep twin: not timed, so the result names what runs
-    if (t.chunk_sweeps == kMaxChunk && !whole_passes_possible(s)) continue;
+    if (t.fused && t.chunk_sweeps == kMaxChunk && !whole_passes_possible(s)) continue;
     if ((rc = cvf_step(s, 2))) break;
     HIPC(hipEventRecord(a, s->stream));
     if ((rc = cvf_step(s, steps_per_trial))) break;

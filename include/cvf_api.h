@@ -155,8 +155,8 @@ typedef struct cvf_tuning {
   int32_t slab_planes;  /* planes per slab of the fused kernel (<= 16); 0 = automatic */
   int32_t block_chunk;  /* workgroup size of the fused kernel */
   int32_t chunk_sweeps; /* half-sweeps per fused launch: 6, or 12 (a whole Metropolis pass per
-                           launch where the wider window still fits in LDS and no Swendsen-Wang
-                           step or per-step histogram reads the lattice between steps); 0 = 6 */
+                           launch where the wider window still fits in LDS); 0 = 6.  The default
+                           is 12 for handles under 2^20 cells with the chessboard rule, else 6 */
 } cvf_tuning;
 
 int cvf_get_tuning(cvf_sim *s, cvf_tuning *t);

@@ -226,13 +226,15 @@ def test_fused_and_unfused_paths_edge_cases(fused, L, nm, nc, steps):
     assert_same_obs(eng, ora, "end")
 
 
-@pytest.mark.parametrize("nm,graph_steps,steps", [(1, 16, 37), (1, 3, 20), (2, 5, 13), (3, 4, 9)])
-def test_whole_pass_launches_match_oracle(nm, graph_steps, steps):
+@pytest.mark.parametrize("nm,nc,graph_steps,steps", [(1, 0, 16, 37), (1, 0, 3, 20), (2, 0, 5, 13), (3, 0, 4, 9),
+                                                     (1, 1, 16, 21), (3, 1, 4, 9)])
+def test_whole_pass_launches_match_oracle(nm, nc, graph_steps, steps):
     """chunk_sweeps = 12: one fused launch per Metropolis pass.  The state
-    alternates between the two word arrays and a sequence with an odd launch
-    count runs its last step in 6-sweep launches; bit-exact with the oracle
-    after calls of 1, 2, 3 and `steps` steps (several graph sizes each)."""
-    kw = dict(L=16, T=0.7, P=0.4, seed=2024, n_metropolis=nm, n_cluster=0)
+    alternates between the two word arrays (the SW kernels follow it) and a
+    sequence with an odd launch count runs its last step in 6-sweep launches;
+    bit-exact with the oracle (and the SW labels) after calls of 1, 2, 3 and
+    `steps` steps (several graph sizes each)."""
+    kw = dict(L=16, T=0.7, P=0.4, seed=2024, n_metropolis=nm, n_cluster=nc)
     eng = Simulation(Params(volume_mode=VOLUME_DEVICE, **kw))
     eng.tuning = {"fused": 1, "chunk_sweeps": 12, "graph_steps": graph_steps, "slab_planes": 1}
     assert eng.tuning["chunk_sweeps"] == 12
@@ -243,6 +245,8 @@ def test_whole_pass_launches_match_oracle(nm, graph_steps, steps):
         ora.step(n)
         done += n
         assert_same_state(eng, ora, f"after {done} steps")
+        if nc:
+            assert np.array_equal(eng.labels(), ora.labels()), f"labels after {done} steps"
     assert_same_obs(eng, ora, "end")
 
 

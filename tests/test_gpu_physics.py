@@ -227,13 +227,17 @@ def test_sigma_histogram_matches_config():
         assert np.array_equal(sim.sigma_histogram(i), np.bincount(s, minlength=6))
 
 
-@pytest.mark.parametrize("n_cluster", [0, 1])
-def test_step_histograms_equal_per_step_histograms(n_cluster):
+@pytest.mark.parametrize("n_cluster,whole", [(0, False), (1, False), (1, True)])
+def test_step_histograms_equal_per_step_histograms(n_cluster, whole):
     # the device-recorded per-step histograms equal stepping one at a time and
     # reading the histogram after each step, for every point of a batch and
-    # across the copy-back span (4096 steps)
+    # across the copy-back span (4096 steps); with whole-pass launches the
+    # state alternates between the word arrays and the histogram and SW
+    # kernels follow it
     pts = batch_points([8, 12], nc=n_cluster)
     a, b = Simulation(pts), Simulation(pts)
+    if whole:
+        a.tuning = {"fused": 1, "chunk_sweeps": 12, "slab_planes": 1, "graph_steps": 5}
     n = 4100 if n_cluster == 0 else 40
     rec = a.step_histograms(n, pt=1)
     ref = []
