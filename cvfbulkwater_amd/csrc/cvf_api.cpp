@@ -1105,7 +1105,7 @@ int cvf_observe(cvf_sim* s, int32_t pt, cvf_obs* o) {
   HIPC(hipSetDevice(s->device));
   if (s->obs_at != s->steps) {
     launch_zero_u32(s->stream, (uint32_t*)s->counters.p, 4 * (size_t)s->npts);
-    launch_observe(s->stream, Launch{s->tab_cell_all.p, s->nb_cell_all, 256, nullptr}, s->words.p,
+    launch_observe(s->stream, Launch{s->tab_cell_all.p, s->nb_cell_all, 256, nullptr, 0}, s->words.p,
                    s->counters.p);
     s->obs_counters.resize(2 * (size_t)s->npts);
     HIPC(hipMemcpyAsync(s->obs_counters.data(), s->counters.p, 16 * (size_t)s->npts,

@@ -560,7 +560,8 @@ constexpr int kMaxWin = kMaxSlab + 2 * kMaxChunk;  // window planes incl. halos 
 
 // kAnyNc: the launch may hold non-chessboard slabs (compiled out otherwise).
 template <bool kAnyNc>
-__global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t nb, const uint64_t* __restrict__ stepp,
+__global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t /* entries, launch order */,
+                        const uint64_t* __restrict__ stepp,
                         uint32_t step_off,
                         const PointDev* __restrict__ pts, const uint32_t* __restrict__ src,
                         uint32_t* __restrict__ dst, uint32_t k0, uint32_t nk, int with_eta) {
