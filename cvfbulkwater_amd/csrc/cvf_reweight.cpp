@@ -203,7 +203,7 @@ struct cvf_reweight {
 
   hipStream_t stream = nullptr;
   DMem<double> en, vn, lh, c0, c1, logp, pmax, mx, ps, lz, norm;
-  DMem<uint32_t> dcell;  // the occupied cells' grid indices, for the projections
+  DMem<uint32_t> dcell;  // the occupied cells as i << 16 | w, for the projections
   DMem<RwSim> sims;
   DMem<double2> tgt;
   int32_t cap_t = 0, n_t = 0;
@@ -364,7 +364,12 @@ int cvf_rw_create(int32_t L, int32_t n_sim, const double* T_hat, const double* P
   HIPR(rw->sims.alloc(n_sim));
   HIPR(rw->dcell.alloc(nc));
   {
-    std::vector<uint32_t> c32(rw->cell.begin(), rw->cell.end());  // nb^2 < 2^31
+    // (energy row, density column) in 16 bits each: nb^2 < 2^31, so nb < 2^16
+    std::vector<uint32_t> c32(nc);
+    for (size_t k = 0; k < nc; ++k) {
+      const uint64_t i = rw->cell[k] / (uint64_t)rw->nb, w = rw->cell[k] - i * (uint64_t)rw->nb;
+      c32[k] = (uint32_t)(i << 16 | w);
+    }
     HIPR(hipMemcpy(rw->dcell.p, c32.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice));
   }
   std::vector<RwSim> hs(n_sim);
@@ -599,20 +604,18 @@ int cvf_rw_scan(cvf_reweight* rw, int32_t n_targets, const double* T_K, const do
   HIPR(hipMemcpyAsync(dbin.p, bins.data(), bins.size() * sizeof(double2), hipMemcpyHostToDevice, rw->stream));
   HIPR(hipMemsetAsync(derr.p, 0, sizeof(uint32_t), rw->stream));
   HIPR(launch_rw_xhist(g, rw->logp.p, n_targets, n_angles, dsc.p, dbin.p, ph.p, derr.p, rw->stream));
-  std::vector<double> hh((size_t)nta * B * nm);
+  // the workgroups' partial histograms summed on the device, in workgroup order
+  DMem<double> phs;
+  HIPR(phs.alloc((size_t)nta * nm));
+  HIPR(launch_rw_xhist_sum(ph.p, nta, nm, phs.p, rw->stream));
   uint32_t err = 0;
-  HIPR(hipMemcpyAsync(hh.data(), ph.p, hh.size() * sizeof(double), hipMemcpyDeviceToHost, rw->stream));
+  HIPR(hipMemcpyAsync(p, phs.p, (size_t)nta * nm * sizeof(double), hipMemcpyDeviceToHost, rw->stream));
   HIPR(hipMemcpyAsync(&err, derr.p, sizeof err, hipMemcpyDeviceToHost, rw->stream));
   HIPR(hipStreamSynchronize(rw->stream));
   if (err) return set_error(CVF_ERANGE, "an order parameter fell outside its histogram");
   for (int ta = 0; ta < nta; ++ta) {
     double* xo = x + (size_t)ta * nm;
     double* po = p + (size_t)ta * nm;
-    for (int k = 0; k < nm; ++k) {
-      double v = 0;
-      for (int b = 0; b < B; ++b) v += hh[((size_t)ta * B + b) * nm + k];
-      po[k] = v;
-    }
     if (int rc = finish_order_parameter(bins[ta].x, bins[ta].y, nm, xo, po, stats3 + 3 * ta)) return rc;
     if (int rc = cvf_rw_ising_scores(xo, po, nm, scores6 + 6 * ta)) return rc;
   }

@@ -46,7 +46,7 @@ hipError_t launch_rw_pass(const RwCells& cells, const RwSim* sims, const double*
                           const RwTargets& tg, const RwWork& w, double de, double drho, double* ctes_out,
                           bool dist, hipStream_t stream);
 
-// Grid geometry for the projections: cell = i * nb + w, energy
+// Grid geometry for the projections: cell = i << 16 | w (nb < 2^16), energy
 // e_min + (i + 1/2) de (kJ/mol), density rho_min + (w + 1/2) drho (kg/m^3).
 struct RwGeom {
   const uint32_t* cell;
@@ -74,5 +74,9 @@ hipError_t launch_rw_xrange(const RwGeom& g, const double* dens, int32_t n_t, in
 // the bins (and not within 1e-10 of x0, which goes to bin 0) sets *err.
 hipError_t launch_rw_xhist(const RwGeom& g, const double* dens, int32_t n_t, int32_t n_a, const double2* sc,
                            const double2* binfo, double* part, uint32_t* err, hipStream_t st);
+
+// out[ta][k] = sum over the kRwProjBlocks workgroups b, in order, of
+// part[ta][b][k] (n_ta pairs, nm bins).
+hipError_t launch_rw_xhist_sum(const double* part, int32_t n_ta, int32_t nm, double* out, hipStream_t st);
 
 }  // namespace cvf

@@ -191,7 +191,7 @@ __device__ inline void block_range(int64_t n, int64_t& lo, int64_t& hi) {
 }
 
 __device__ inline void cell_er(const RwGeom& g, uint32_t c, double& e, double& r) {
-  const uint32_t i = c / (uint32_t)g.nb, w = c - i * (uint32_t)g.nb;
+  const uint32_t i = c >> 16, w = c & 0xFFFFu;
   e = g.e_min + ((int)(i + 1) - 0.5) * g.de;
   r = g.rho_min + ((int)(w + 1) - 0.5) * g.drho;
 }
@@ -246,11 +246,38 @@ __global__ __launch_bounds__(kRwBlock) void k_rw_xrange(RwGeom g, const double* 
   }
 }
 
+// Sum of v over the wavefront by DPP steps (pairs, quads, half rows, rows,
+// then the row broadcasts), an order fixed by the lane layout; the total
+// ends in lane 63.
+template <int kCtrl, int kRowMask>
+__device__ inline double dpp_add(double x) {
+  const uint64_t u = __double_as_longlong(x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, kCtrl, kRowMask, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), kCtrl, kRowMask, 0xF, false);
+  return x + __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ inline double wave_sum_dpp(double x) {
+  x = dpp_add<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+  x = dpp_add<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+  x = dpp_add<0x141, 0xF>(x);  // row_half_mirror
+  x = dpp_add<0x140, 0xF>(x);  // row_mirror
+  x = dpp_add<0x142, 0xA>(x);  // row_bcast:15 into rows 1, 3
+  x = dpp_add<0x143, 0xC>(x);  // row_bcast:31 into rows 2, 3
+  return x;
+}
+
+__device__ inline double readlane_f64(double v, int i) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, i);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), i);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Each wavefront walks its share of the workgroup's range 64 cells at a
 // time; the lanes of a chunk that fall in one bin are summed by a fixed
-// butterfly and added to the wavefront's own LDS histogram by lane 0, bin
-// after bin in order of first appearance.  Neighbouring cells share bins,
-// so a chunk holds few distinct bins.
+// DPP reduction and added to the wavefront's own LDS histogram, bin after
+// bin in order of first appearance.  Neighbouring cells share bins, so a
+// chunk holds few distinct bins.
 __global__ __launch_bounds__(kRwBlock) void k_rw_xhist(RwGeom g, const double* __restrict__ dens, int32_t n_a,
                                                        const double2* __restrict__ sc,
                                                        const double2* __restrict__ binfo,
@@ -269,13 +296,16 @@ __global__ __launch_bounds__(kRwBlock) void k_rw_xhist(RwGeom g, const double* _
   double* h = hw + wave * nm;
   for (int64_t c0 = wlo; c0 < whi; c0 += 64) {
     const int64_t b = c0 + lane;
+    const int64_t bc = b < whi ? b : whi - 1;  // loads issued unconditionally
+    const double vb = v[bc];
+    const uint32_t cb = g.cell[bc];
     int k = -1;
     double x = 0;
-    if (b < whi && v[b] > 0) {
+    if (b < whi && vb > 0) {
       double e, r;
-      cell_er(g, g.cell[b], e, r);
+      cell_er(g, cb, e, r);
       const double X = e * q.x + r * q.y, d = (X - bi.x) / bi.y;
-      x = v[b];
+      x = vb;
       if (fabs(d) < 1073741824.0 && (int)d >= 0 && (int)d < nm)
         k = (int)d;
       else if (fabs(X - bi.x) < (double)1E-10f)
@@ -285,8 +315,8 @@ __global__ __launch_bounds__(kRwBlock) void k_rw_xhist(RwGeom g, const double* _
     }
     uint64_t left = __ballot(k >= 0);
     while (left) {
-      const int k0 = __shfl(k, __ffsll((unsigned long long)left) - 1);
-      const double s = wave_sum(k == k0 ? x : 0.0);
+      const int k0 = __builtin_amdgcn_readlane(k, __ffsll((unsigned long long)left) - 1);
+      const double s = readlane_f64(wave_sum_dpp(k == k0 ? x : 0.0), 63);
       if (lane == 0) h[k0] += s;
       left &= ~__ballot(k == k0);
     }
@@ -296,7 +326,21 @@ __global__ __launch_bounds__(kRwBlock) void k_rw_xhist(RwGeom g, const double* _
     part[((int64_t)ta * gridDim.x + blockIdx.x) * nm + k] = hw[k] + hw[nm + k] + hw[2 * nm + k] + hw[3 * nm + k];
 }
 
+__global__ void k_rw_xhist_sum(const double* __restrict__ part, int32_t nm, double* __restrict__ out) {
+  const int ta = blockIdx.x;
+  for (int k = threadIdx.x; k < nm; k += blockDim.x) {
+    double v = 0;
+    for (int b = 0; b < kRwProjBlocks; ++b) v += part[((int64_t)ta * kRwProjBlocks + b) * nm + k];
+    out[(int64_t)ta * nm + k] = v;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_rw_xhist_sum(const double* part, int32_t n_ta, int32_t nm, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_rw_xhist_sum, dim3((unsigned)n_ta), dim3(64), 0, st, part, nm, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_rw_moments(const RwGeom& g, const double* dens, int32_t n_t, double* part, hipStream_t st) {
   hipLaunchKernelGGL(k_rw_moments, dim3(kRwProjBlocks, (unsigned)n_t), dim3(kRwBlock), 0, st, g, dens, part);
