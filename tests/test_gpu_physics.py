@@ -253,6 +253,33 @@ def test_step_histograms_equal_per_step_histograms(n_cluster, whole):
     assert a.step_histograms(0).shape == (0, 6)
 
 
+@pytest.mark.parametrize("schedule,tuning,classes", [
+    (0, {"fused": 1, "chunk_sweeps": 12}, {"chunk": 2, "begin": 1}),
+    (0, {"fused": 0}, {"eta": 2, "sigma": 12, "begin": 1}),
+    (1, {"fused": 1}, {"eta": 2, "sigma": 2, "begin": 1}),
+])
+def test_profile_steps_matches_plain_steps(schedule, tuning, classes):
+    """cvf_profile_steps (every launch timed by its own dispatch events) runs
+    the plain form of each step, whatever the tuning's variant forms, and
+    leaves the same state as cvf_step; launch counts per class as expected."""
+    from cvfbulkwater_amd import SCHED_FAST
+
+    pts = [Params(L=L, T=0.7, P=0.3, seed=90 + L, schedule=SCHED_FAST if schedule else 0,
+                  volume_mode=VOLUME_DEVICE) for L in (8, 16)]
+    a, b = Simulation(pts), Simulation(pts)
+    a.tuning = tuning
+    b.tuning = tuning
+    prof = a.profile_steps(5)
+    b.step(5)
+    for i in range(len(pts)):
+        x, y = a.get_config(i), b.get_config(i)
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+        assert a.volume(i) == b.volume(i)
+    for cls, per_step in classes.items():
+        ms, n = prof[cls]
+        assert n == 5 * per_step and ms > 0, (cls, prof[cls])
+
+
 def test_config5_precision_form_errors():
     """L=96, T=0.65, P=0.30 with SW, FP32-compensated vdW against FP64
     everywhere from the same seed: the lattice trajectory is identical (N_HB
