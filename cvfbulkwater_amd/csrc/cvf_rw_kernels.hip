@@ -196,6 +196,8 @@ __device__ inline void cell_er(const RwGeom& g, uint32_t c, double& e, double& r
   r = g.rho_min + ((int)(w + 1) - 0.5) * g.drho;
 }
 
+constexpr int kRwBatch = 8;  // cells per thread whose loads a projection issues together
+
 __global__ __launch_bounds__(kRwBlock) void k_rw_moments(RwGeom g, const double* __restrict__ dens,
                                                          double* __restrict__ part) {
   __shared__ double red[kRwBlock / 64];
@@ -204,15 +206,29 @@ __global__ __launch_bounds__(kRwBlock) void k_rw_moments(RwGeom g, const double*
   int64_t lo, hi;
   block_range(g.n, lo, hi);
   double s[5] = {0, 0, 0, 0, 0};
-  for (int64_t b = lo + threadIdx.x; b < hi; b += kRwBlock) {
-    double e, r;
-    cell_er(g, g.cell[b], e, r);
-    const double x = v[b], e2 = e * e, r2 = r * r;
-    s[0] += x;
-    s[1] += e2 * x;
-    s[2] += e2 * e2 * x;
-    s[3] += r2 * x;
-    s[4] += r2 * r2 * x;
+  // kRwBatch cells per thread per trip, their loads issued first, then
+  // added in cell order
+  for (int64_t b0 = lo + threadIdx.x; b0 < hi; b0 += kRwBlock * kRwBatch) {
+    double xs[kRwBatch];
+    uint32_t cs[kRwBatch];
+#pragma unroll
+    for (int u = 0; u < kRwBatch; ++u) {
+      const int64_t bb = b0 + kRwBlock * u, bc = bb < hi ? bb : hi - 1;
+      xs[u] = v[bc];
+      cs[u] = g.cell[bc];
+    }
+#pragma unroll
+    for (int u = 0; u < kRwBatch; ++u) {
+      if (b0 + kRwBlock * u >= hi) break;
+      double e, r;
+      cell_er(g, cs[u], e, r);
+      const double x = xs[u], e2 = e * e, r2 = r * r;
+      s[0] += x;
+      s[1] += e2 * x;
+      s[2] += e2 * e2 * x;
+      s[3] += r2 * x;
+      s[4] += r2 * r2 * x;
+    }
   }
   for (int k = 0; k < 5; ++k) {
     const double m = block_reduce<false>(s[k], red);
@@ -230,13 +246,24 @@ __global__ __launch_bounds__(kRwBlock) void k_rw_xrange(RwGeom g, const double* 
   int64_t lo, hi;
   block_range(g.n, lo, hi);
   double mn = INFINITY, mx = -INFINITY;
-  for (int64_t b = lo + threadIdx.x; b < hi; b += kRwBlock) {
-    if (!(v[b] > 0)) continue;
-    double e, r;
-    cell_er(g, g.cell[b], e, r);
-    const double x = e * q.x + r * q.y;
-    mn = fmin(mn, x);
-    mx = fmax(mx, x);
+  for (int64_t b0 = lo + threadIdx.x; b0 < hi; b0 += kRwBlock * kRwBatch) {
+    double xs[kRwBatch];
+    uint32_t cs[kRwBatch];
+#pragma unroll
+    for (int u = 0; u < kRwBatch; ++u) {
+      const int64_t bb = b0 + kRwBlock * u, bc = bb < hi ? bb : hi - 1;
+      xs[u] = v[bc];
+      cs[u] = g.cell[bc];
+    }
+#pragma unroll
+    for (int u = 0; u < kRwBatch; ++u) {
+      if (b0 + kRwBlock * u >= hi || !(xs[u] > 0)) continue;
+      double e, r;
+      cell_er(g, cs[u], e, r);
+      const double x = e * q.x + r * q.y;
+      mn = fmin(mn, x);
+      mx = fmax(mx, x);
+    }
   }
   mn = -block_reduce<true>(-mn, red);
   mx = block_reduce<true>(mx, red);
@@ -246,84 +273,62 @@ __global__ __launch_bounds__(kRwBlock) void k_rw_xrange(RwGeom g, const double* 
   }
 }
 
-// Sum of v over the wavefront by DPP steps (pairs, quads, half rows, rows,
-// then the row broadcasts), an order fixed by the lane layout; the total
-// ends in lane 63.
-template <int kCtrl, int kRowMask>
-__device__ inline double dpp_add(double x) {
-  const uint64_t u = __double_as_longlong(x);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, kCtrl, kRowMask, 0xF, false);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), kCtrl, kRowMask, 0xF, false);
-  return x + __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-__device__ inline double wave_sum_dpp(double x) {
-  x = dpp_add<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
-  x = dpp_add<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
-  x = dpp_add<0x141, 0xF>(x);  // row_half_mirror
-  x = dpp_add<0x140, 0xF>(x);  // row_mirror
-  x = dpp_add<0x142, 0xA>(x);  // row_bcast:15 into rows 1, 3
-  x = dpp_add<0x143, 0xC>(x);  // row_bcast:31 into rows 2, 3
-  return x;
-}
+// One wavefront per workgroup; every lane adds its cells (lane, lane + 64,
+// ... of the workgroup's contiguous range) into its own LDS row of nm bins,
+// and lane k then sums column k over the 64 rows in lane order.  Every sum
+// runs in an order fixed by the launch shape, with one LDS read-modify-write
+// per cell and no cross-lane traffic.
+constexpr int kRwXhistBatch = 16;
 
-__device__ inline double readlane_f64(double v, int i) {
-  const uint64_t u = __double_as_longlong(v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, i);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), i);
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
-// Each wavefront walks its share of the workgroup's range 64 cells at a
-// time; the lanes of a chunk that fall in one bin are summed by a fixed
-// DPP reduction and added to the wavefront's own LDS histogram, bin after
-// bin in order of first appearance.  Neighbouring cells share bins, so a
-// chunk holds few distinct bins.
-__global__ __launch_bounds__(kRwBlock) void k_rw_xhist(RwGeom g, const double* __restrict__ dens, int32_t n_a,
-                                                       const double2* __restrict__ sc,
-                                                       const double2* __restrict__ binfo,
-                                                       double* __restrict__ part, uint32_t* __restrict__ err) {
-  extern __shared__ double hw[];  // [4 waves][nm]
+__global__ __launch_bounds__(64) void k_rw_xhist(RwGeom g, const double* __restrict__ dens, int32_t n_a,
+                                                 const double2* __restrict__ sc,
+                                                 const double2* __restrict__ binfo, double* __restrict__ part,
+                                                 uint32_t* __restrict__ err) {
+  extern __shared__ double hl[];  // [64 lanes][nm]
   const int ta = blockIdx.y, t = ta / n_a, a = ta - t * n_a;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nm = g.nm;
-  for (int k = threadIdx.x; k < 4 * nm; k += kRwBlock) hw[k] = 0;
-  __syncthreads();
+  const int lane = threadIdx.x, nm = g.nm;
+  double* h = hl + lane * nm;
+  for (int k = 0; k < nm; ++k) h[k] = 0;
   const double* v = dens + (int64_t)t * g.n;
   const double2 q = sc[a], bi = binfo[ta];
   int64_t lo, hi;
   block_range(g.n, lo, hi);
-  const int64_t per = (hi - lo + 3) / 4;
-  const int64_t wlo = lo + wave * per, whi = wlo + per < hi ? wlo + per : hi;
-  double* h = hw + wave * nm;
-  for (int64_t c0 = wlo; c0 < whi; c0 += 64) {
-    const int64_t b = c0 + lane;
-    const int64_t bc = b < whi ? b : whi - 1;  // loads issued unconditionally
-    const double vb = v[bc];
-    const uint32_t cb = g.cell[bc];
-    int k = -1;
-    double x = 0;
-    if (b < whi && vb > 0) {
+  // kRwXhistBatch cells per lane per trip, all their loads issued first
+  // (clamped indices), then added in cell order
+  for (int64_t b0 = lo + lane; b0 < hi; b0 += 64 * kRwXhistBatch) {
+    double xs[kRwXhistBatch];
+    uint32_t cs[kRwXhistBatch];
+#pragma unroll
+    for (int u = 0; u < kRwXhistBatch; ++u) {
+      const int64_t bb = b0 + 64 * u, bc = bb < hi ? bb : hi - 1;
+      xs[u] = v[bc];
+      cs[u] = g.cell[bc];
+    }
+#pragma unroll
+    for (int u = 0; u < kRwXhistBatch; ++u) {
+      const double x = xs[u];
+      if (b0 + 64 * u >= hi || !(x > 0)) continue;
       double e, r;
-      cell_er(g, cb, e, r);
+      cell_er(g, cs[u], e, r);
       const double X = e * q.x + r * q.y, d = (X - bi.x) / bi.y;
-      x = vb;
+      int k;
       if (fabs(d) < 1073741824.0 && (int)d >= 0 && (int)d < nm)
         k = (int)d;
       else if (fabs(X - bi.x) < (double)1E-10f)
         k = 0;
-      else
+      else {
         atomicOr(err, 1u);
-    }
-    uint64_t left = __ballot(k >= 0);
-    while (left) {
-      const int k0 = __builtin_amdgcn_readlane(k, __ffsll((unsigned long long)left) - 1);
-      const double s = readlane_f64(wave_sum_dpp(k == k0 ? x : 0.0), 63);
-      if (lane == 0) h[k0] += s;
-      left &= ~__ballot(k == k0);
+        continue;
+      }
+      h[k] += x;
     }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < nm; k += kRwBlock)
-    part[((int64_t)ta * gridDim.x + blockIdx.x) * nm + k] = hw[k] + hw[nm + k] + hw[2 * nm + k] + hw[3 * nm + k];
+  for (int k = lane; k < nm; k += 64) {
+    double s = 0;
+    for (int l = 0; l < 64; ++l) s += hl[l * nm + k];
+    part[((int64_t)ta * gridDim.x + blockIdx.x) * nm + k] = s;
+  }
 }
 
 __global__ void k_rw_xhist_sum(const double* __restrict__ part, int32_t nm, double* __restrict__ out) {
@@ -356,8 +361,15 @@ hipError_t launch_rw_xrange(const RwGeom& g, const double* dens, int32_t n_t, in
 
 hipError_t launch_rw_xhist(const RwGeom& g, const double* dens, int32_t n_t, int32_t n_a, const double2* sc,
                            const double2* binfo, double* part, uint32_t* err, hipStream_t st) {
-  hipLaunchKernelGGL(k_rw_xhist, dim3(kRwProjBlocks, (unsigned)(n_t * n_a)), dim3(kRwBlock),
-                     sizeof(double) * 4 * (size_t)g.nm, st, g, dens, n_a, sc, binfo, part, err);
+  // 64 rows of nm <= 215 bins (n_bin2D <= 46340): up to 110 KB of LDS
+  const size_t lds = sizeof(double) * 64 * (size_t)g.nm;
+  if (lds > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rw_xhist),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_rw_xhist, dim3(kRwProjBlocks, (unsigned)(n_t * n_a)), dim3(64), lds, st, g, dens, n_a, sc,
+                     binfo, part, err);
   return hipGetLastError();
 }
 
