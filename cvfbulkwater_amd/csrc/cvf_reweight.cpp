@@ -76,6 +76,39 @@ struct DMem {
   }
 };
 
+// Pinned host memory (hipHostMalloc), grown on demand: the densities'
+// device-to-host copy then runs at the link's rate instead of through the
+// runtime's staging of pageable memory.
+template <class T>
+struct HMem {
+  T* p = nullptr;
+  size_t n = 0, cap = 0;
+  HMem() = default;
+  HMem(const HMem&) = delete;
+  HMem& operator=(const HMem&) = delete;
+  ~HMem() {
+    if (p) (void)hipHostFree(p);
+  }
+  hipError_t resize(size_t m) {
+    if (m > cap) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      cap = n = 0;
+      const hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(m, 1) * sizeof(T), hipHostMallocDefault);
+      if (e != hipSuccess) {
+        p = nullptr;
+        return e;
+      }
+      cap = m;
+    }
+    n = m;
+    return hipSuccess;
+  }
+  T* data() { return p; }
+  const T* data() const { return p; }
+  size_t size() const { return n; }
+};
+
 // norm 1, mean 0, variance 1 with uniform bins of width *w (f90:1484-1508);
 // returns the mean that was removed
 double standardize(double* x, double* p, int n, double* w) {
@@ -207,7 +240,7 @@ struct cvf_reweight {
   DMem<RwSim> sims;
   DMem<double2> tgt;
   int32_t cap_t = 0, n_t = 0;
-  std::vector<double> dist;  // [target][cell] densities of the last reweighting
+  HMem<double> dist;  // [target][cell] densities of the last reweighting (pinned)
 
   int32_t nblk() const { return (int32_t)((cell.size() + kRwBlock - 1) / kRwBlock); }
   RwCells cells() const { return RwCells{en.p, vn.p, lh.p, (int64_t)cell.size()}; }
@@ -453,7 +486,8 @@ int cvf_rw_reweight(cvf_reweight* rw, int32_t n_targets, const double* T_K, cons
   HIPR(launch_rw_pass(rw->cells(), rw->sims.p, rw->c0.p, rw->n_sim, RwTargets{rw->tgt.p, n_targets}, rw->work(),
                       rw->de, rw->drho, nullptr, true, rw->stream));
   const size_t nc = rw->cell.size();
-  rw->dist.resize((size_t)n_targets * nc);
+  rw->n_t = 0;
+  HIPR(rw->dist.resize((size_t)n_targets * nc));
   rw->n_t = n_targets;
   HIPR(hipMemcpyAsync(rw->dist.data(), rw->logp.p, rw->dist.size() * sizeof(double), hipMemcpyDeviceToHost,
                       rw->stream));
