@@ -239,6 +239,33 @@ def test_executable_recomputes_on_a_stale_constants_file(tmp_path):
             assert (a / n).read_text() == (b / n).read_text(), n
 
 
+def test_device_scan_wide_grid_matches_host_path():
+    """Bin factor 60: n_bin_mixed above 128, so the projection kernel's 64 LDS
+    rows of bins take more than 64 KB.  The device scan repeats bit for bit
+    and equals the host projections of the copied densities (the dense grid
+    of the oracle would not fit a test at this size)."""
+    data = rw_synth.samples(TS, 3000)
+    eng = Reweighting(8, TS, [0.5] * 6, data, 60.0)
+    try:
+        assert eng.info()["n_bin_mixed"] > 128
+        eng.solve_constants(300)
+        T, P = targets()
+        angles = [0.4, 1.55]
+        out = eng.scan(T, P, angles)
+        again = eng.scan(T, P, angles)
+        for k in out:
+            assert np.array_equal(out[k], again[k]), k
+        eng.reweight(T, P)
+        for t in range(len(T)):
+            for a, alpha in enumerate(angles):
+                x, p, st = eng.order_parameter(t, alpha)
+                np.testing.assert_allclose(out["x"][t, a], x, rtol=RTOL, atol=1e-12)
+                np.testing.assert_allclose(out["p"][t, a], p, rtol=RTOL, atol=1e-12)
+                assert tuple(out["stats"][t, a]) == pytest.approx(tuple(st), rel=RTOL)
+    finally:
+        eng.close()
+
+
 def test_device_scan_matches_oracle_and_host_path(pair):
     ora, eng, c_ora, *_ = pair
     eng.set_constants(c_ora)
