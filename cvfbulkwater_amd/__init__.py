@@ -171,6 +171,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cvf_thermo_derivative": (C.c_int, [DP, DP, DP, C.POINTER(C.c_int32), I32, I32, DP, DP, DP]),
         "cvf_thermo_kt": (C.c_int, [DP, DP, DP, DP, I32, C.c_double, DP, DP]),
         "cvf_set_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
+        "cvf_get_plan": (C.c_int, [P, I32, I32P, I32P, I32P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -307,6 +308,13 @@ class Simulation:
         cur.update(value)
         t = _Tuning(*[int(cur[k]) for k in self.TUNING_FIELDS])
         _check(self._lib, self._lib.cvf_set_tuning(self._h, C.byref(t)), "cvf_set_tuning")
+
+    def plan(self, pt: int = 0) -> dict:
+        """Launch plan of point pt: stream group, planes per fused z-slab (0:
+        unfused kernels) and sigma half-sweeps per fused launch."""
+        g, t, k = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(self._lib, self._lib.cvf_get_plan(self._h, pt, C.byref(g), C.byref(t), C.byref(k)), "cvf_get_plan")
+        return {"group": g.value, "slab_planes": t.value, "chunk_sweeps": k.value}
 
     def autotune(self, steps_per_trial: int = 20) -> dict:
         """Time candidate launch configurations and keep the fastest (state unchanged)."""

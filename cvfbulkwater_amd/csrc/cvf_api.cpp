@@ -70,6 +70,8 @@ struct PointHost {
   std::vector<int32_t> freq;
   VolState vol;  // host copy (authoritative in host-volume mode)
   VolParams vp;
+  int32_t group = 0;    // stream group under the current tuning
+  uint32_t slab_T = 0;  // planes per slab of its fused launches (0: unfused kernels)
 };
 
 // A group of state points advanced by its own stream: its own block tables,
@@ -584,6 +586,12 @@ int build_groups(cvf_sim* s) {
       }
     }
     if (!long_fits) g->chunk_lds_long = 0;
+    for (int p : g->points) {
+      s->ph[p].group = (int32_t)(&g - &s->groups[0]);
+      s->ph[p].slab_T = 0;
+    }
+    if (g->fused)
+      for (const BlockEnt& en : ts) s->ph[en.point].slab_T = en.T;
     if (g->fused) {
       // largest slabs first (the kernel takes entries in launch order, and
       // each XCD's dispatcher hands out its share in that order)
@@ -1380,6 +1388,24 @@ int cvf_profile_steps(cvf_sim* s, int64_t n_steps, double* kernel_ms, int64_t* l
   return CVF_OK;
 }
 
+int cvf_get_plan(cvf_sim* s, int32_t pt, int32_t* group, int32_t* slab_planes, int32_t* chunk_sweeps) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  const PointHost& P = s->ph[pt];
+  const Group& g = *s->groups[P.group];
+  if (group) *group = P.group;
+  if (slab_planes) *slab_planes = (int32_t)P.slab_T;
+  if (chunk_sweeps) {
+    // half-sweeps per fused launch of the reference schedule: whole passes
+    // only where every window of the group holds the wider halo
+    const bool whole = s->sched == CVF_SCHED_REFERENCE && s->tune.chunk_sweeps == kMaxChunk &&
+                       g.chunk_lds_long != 0;
+    const bool none = P.slab_T == 0 || s->nm == 0 || s->sched != CVF_SCHED_REFERENCE;
+    *chunk_sweeps = none ? 0 : whole ? kMaxChunk : kChunk;
+  }
+  return CVF_OK;
+}
+
 int cvf_get_tuning(cvf_sim* s, cvf_tuning* t) {
   if (!s || !t) return fail(CVF_EINVAL, "null argument");
   *t = s->tune;
@@ -1413,7 +1439,11 @@ std::string tune_key(cvf_sim* s) {
     k += std::to_string(Ls[i]) + "x" + std::to_string(j - i) + ",";
     i = j;
   }
-  k += "|nm" + std::to_string(s->nm) + "|nc" + std::to_string(s->nc) + "|p" +
+  // the non-chessboard rule changes the window halos and the default
+  // launch length, so its points are part of the shape
+  int n_nochess = 0;
+  for (const auto& P : s->ph) n_nochess += !P.prm.flag_chessboard;
+  k += "|nm" + std::to_string(s->nm) + "|nc" + std::to_string(s->nc) + "|ncb" + std::to_string(n_nochess) + "|p" +
        std::to_string(s->precision) + "|v" + std::to_string(s->vol_device) + "|s" + std::to_string(s->sched);
   return k;
 }

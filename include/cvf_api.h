@@ -160,6 +160,13 @@ typedef struct cvf_tuning {
 } cvf_tuning;
 
 int cvf_get_tuning(cvf_sim *s, cvf_tuning *t);
+
+/* Launch plan of state point pt under the current tuning (no reference
+ * counterpart; the reference has one launch shape): its stream group, the
+ * planes per z-slab of its fused launches (0 when its group runs the
+ * unfused kernels) and the sigma half-sweeps per fused launch of the
+ * reference schedule (6 or 12; 0 when none run fused). */
+int cvf_get_plan(cvf_sim *s, int32_t pt, int32_t *group, int32_t *slab_planes, int32_t *chunk_sweeps);
 /* Rebuilds the groups, block tables and graphs; synchronises the handle. */
 int cvf_set_tuning(cvf_sim *s, const cvf_tuning *t);
 

@@ -177,6 +177,11 @@ def test_host_and_device_volume_modes_agree():
     {"streams": 2, "fused": 1, "slab_planes": 4, "block_chunk": 256},
     {"streams": 1, "fused": 1, "slab_planes": 2, "block_chunk": 1024, "graph_steps": 3},
     {"streams": 4, "fused": 1, "slab_planes": 1, "block_chunk": 64},
+    # thick slabs, as the automatic plan gives large batches; T = L at L = 8,
+    # 12 and 16 (a window repeats its own planes)
+    {"streams": 2, "fused": 1, "slab_planes": 8, "block_chunk": 1024},
+    {"streams": 1, "fused": 1, "slab_planes": 12, "block_chunk": 512},
+    {"streams": 2, "fused": 1, "slab_planes": 16, "block_chunk": 1024, "chunk_sweeps": 12},
 ])
 def test_tuning_does_not_change_results(tuning):
     """Launch configuration (streams, workgroup sizes, steps per graph) never
