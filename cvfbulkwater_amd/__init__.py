@@ -172,6 +172,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cvf_thermo_kt": (C.c_int, [DP, DP, DP, DP, I32, C.c_double, DP, DP]),
         "cvf_set_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
         "cvf_get_plan": (C.c_int, [P, I32, I32P, I32P, I32P]),
+        "cvf_vdw_energy_noalloc": (C.c_int, [I32, I32, F32P, I32P, I32, C.c_float, DP, I32, I32, DP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -358,17 +359,20 @@ class Simulation:
 
 
 def vdw_energy(L: int, dist: np.ndarray, freq: np.ndarray, r_cutoff: float, r: np.ndarray,
-               precision: int = FP64, device_id: int = 0) -> np.ndarray:
-    """E_vdW(r) of a shell table evaluated on the GPU (FP64 or FP32-compensated)."""
+               precision: int = FP64, device_id: int = 0, noalloc: bool = False) -> np.ndarray:
+    """E_vdW(r) of a shell table evaluated on the GPU (FP64 or FP32-compensated).
+    noalloc: through cvf_vdw_energy_noalloc (fixed device scratch, no allocation)."""
     lib = load_library()
     d = np.ascontiguousarray(dist, np.float32)
     f = np.ascontiguousarray(freq, np.int32)
     rr = np.ascontiguousarray(r, np.float64)
     out = np.empty_like(rr)
-    _check(lib, lib.cvf_vdw_energy(device_id, L, d.ctypes.data_as(C.POINTER(C.c_float)),
+    fn, name = (lib.cvf_vdw_energy_noalloc, "cvf_vdw_energy_noalloc") if noalloc else (lib.cvf_vdw_energy,
+                                                                                      "cvf_vdw_energy")
+    _check(lib, fn(device_id, L, d.ctypes.data_as(C.POINTER(C.c_float)),
                                    f.ctypes.data_as(C.POINTER(C.c_int32)), d.size, r_cutoff,
                                    rr.ctypes.data_as(C.POINTER(C.c_double)), rr.size, precision,
-                                   out.ctypes.data_as(C.POINTER(C.c_double))), "cvf_vdw_energy")
+                                   out.ctypes.data_as(C.POINTER(C.c_double))), name)
     return out
 
 

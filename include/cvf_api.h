@@ -229,6 +229,20 @@ int cvf_vdw_energy(int32_t device_id, int32_t L, const float *dist, const int32_
                    int32_t nshell, float r_cutoff, const double *r, int32_t n_r,
                    int32_t precision, double *energy);
 
+/* Allocation-free form of cvf_vdw_energy: the same kernel and results, with
+ * the shell table, radii and energies staged in a fixed device array that
+ * each GPU holds until the process ends instead of memory allocated per
+ * call.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  At most CVF_NOALLOC_MAX_SHELLS
+ * shells and CVF_NOALLOC_MAX_R radii per call (CVF_EINVAL beyond).  Calls
+ * share their GPU's array: calls that overlap on one GPU race. */
+#define CVF_NOALLOC_MAX_SHELLS 512
+#define CVF_NOALLOC_MAX_R 4096
+int cvf_vdw_energy_noalloc(int32_t device_id, int32_t L, const float *dist, const int32_t *freq,
+                           int32_t nshell, float r_cutoff, const double *r, int32_t n_r,
+                           int32_t precision, double *energy);
+
 #ifdef __cplusplus
 }
 #endif
