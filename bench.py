@@ -195,46 +195,48 @@ def calibrate_copy(mib: float, reps: int = 10) -> int:
 
 
 def tuning_signature(kname: str, t: dict) -> str:
+    if kname == "k_resident":
+        return f"per_cu:{t['streams']},block:{t['block_chunk']},fused:2,graph:{t['graph_steps']}"
     if kname == "k_chunk":
         whole = ",chunk:12" if t.get("chunk_sweeps") == 12 else ""
         return f"streams:{t['streams']},block:{t['block_chunk']},fused:1,slab:{t['slab_planes']}{whole}"
     return f"streams:{t['streams']},block:{t['block_sigma']},fused:0"
 
 
+def _profile_line(tag: str, kname: str, tuning: dict):
+    """The first line "<tag> kernel=<kname> tuning=<signature> ..." of the
+    counter summaries kept in profiles/, as a dict, or None."""
+    sig = tuning_signature(kname, tuning)
+    for path in sorted((ROOT / "profiles").glob("*.txt")):
+        for line in path.read_text().splitlines():
+            if not line.startswith(tag + " "):
+                continue
+            kv = dict(f.split("=", 1) for f in line.split()[1:] if "=" in f)
+            if kv.get("kernel") == kname and kv.get("tuning") == sig:
+                return kv
+    return None
+
+
 def measured_traffic(kname: str, tuning: dict):
     """HBM bytes per dispatch of the roofline kernel under this launch
-    configuration, from the FETCH_SIZE / WRITE_SIZE summary kept in
+    configuration, from the FETCH_SIZE / WRITE_SIZE summaries kept in
     profiles/ (lines "traffic kernel=... tuning=... bytes=..."), or None
     when no counter run covers the configuration."""
-    path = ROOT / "profiles" / "batch128_traffic.txt"
-    if not path.exists():
-        return None
-    sig = tuning_signature(kname, tuning)
-    for line in path.read_text().splitlines():
-        if not line.startswith("traffic "):
-            continue
-        kv = dict(f.split("=", 1) for f in line.split()[1:])
-        if kv.get("kernel") == kname and kv.get("tuning") == sig:
-            return float(kv["bytes"])
-    return None
+    kv = _profile_line("traffic", kname, tuning)
+    return float(kv["bytes"]) if kv else None
 
 
-def measured_valu(kname: str, tuning: dict):
-    """VALU wave instructions (all, and 64-bit integer ones) per dispatch of
-    the roofline kernel under this launch configuration, from the SQ counter
-    summary kept in profiles/ (lines "valu kernel=... tuning=... insts=...
-    int64=..."), or None."""
-    path = ROOT / "profiles" / "k_chunk_batch128_sq_counters.txt"
-    if not path.exists():
+def measured_valu_per_step(kname: str, tuning: dict, dispatches_per_step: float):
+    """VALU wave instructions per MC step of the roofline kernel under this
+    launch configuration, from the SQ counter summaries kept in profiles/
+    (lines "valu kernel=... tuning=... insts=<per dispatch>" or
+    "insts_per_step=..."), or None."""
+    kv = _profile_line("valu", kname, tuning)
+    if not kv:
         return None
-    sig = tuning_signature(kname, tuning)
-    for line in path.read_text().splitlines():
-        if not line.startswith("valu "):
-            continue
-        kv = dict(f.split("=", 1) for f in line.split()[1:])
-        if kv.get("kernel") == kname and kv.get("tuning") == sig:
-            return float(kv["insts"]), float(kv["int64"])
-    return None
+    if "insts_per_step" in kv:
+        return float(kv["insts_per_step"])
+    return float(kv["insts"]) * dispatches_per_step
 
 
 def precision_form(steps: int = 5000, interval: int = 10, ev: HipEvents | None = None) -> dict:
@@ -399,21 +401,67 @@ def correlation_form(steps: int, warmup: int) -> dict:
             "ms_per_step": ms, "value": 32 ** 3 / (ms * 1e-3)}
 
 
-def plain_per_L(steps: int, warmup: int, ev: HipEvents) -> dict:
-    """The batch run as 128 sequential plain drop-in runs (one state point
-    per handle, FP64, host volume move): each L's plain step time times its
-    32 points."""
-    from cvfbulkwater_amd import FP64, VOLUME_HOST, Params, Simulation
+def plain_sequential(steps: int, warmup: int, ev: HipEvents) -> dict:
+    """The headline batch's 128 state points run one after another in the
+    plain drop-in form (one point per handle, its own T, P and seed, FP64,
+    host volume move, default launch plan): `steps` timed steps each, HIP
+    events on each handle's stream; value = all cells x steps / summed time."""
+    from cvfbulkwater_amd import FP64, VOLUME_HOST, Simulation
 
-    per = {}
-    for L in (16, 24, 32, 48):
-        sim = Simulation(Params(L=L, T=0.815, P=0.45, seed=44648604, precision=FP64,
-                                volume_mode=VOLUME_HOST))
-        per[L] = time_steps(sim, steps, warmup, ev)
+    total_ms, cells, by_L = 0.0, 0, {}
+    for p in batch_params(fp32=False):
+        p.precision, p.volume_mode = FP64, VOLUME_HOST
+        sim = Simulation(p)
+        ms = time_steps(sim, steps, warmup, ev)
         sim.close()
-    cells = 32 * sum(L ** 3 for L in per)
-    total_ms = 32 * sum(per.values())
-    return {"ms_per_step_by_L": per, "value": cells / (total_ms * 1e-3)}
+        total_ms += ms * steps
+        cells += p.L ** 3 * steps
+        by_L.setdefault(p.L, []).append(ms)
+    return {"workload": f"the 128 batch state points as sequential plain runs (FP64, host volume move), "
+                        f"{steps} timed steps each", "value": cells / (total_ms * 1e-3),
+            "ms_per_step_by_L": {L: sum(v) / len(v) for L, v in by_L.items()}}
+
+
+DROPIN_INPUT = """Pressure 0.45
+Temperature 0.815
+Seed 44648604
+Metropolis_Steps 1
+Cluster_Steps {nc}
+Equilibrium_Steps 0
+Sampling_Steps {n}
+Sampling_Interval 10
+R_cutoff 6.0
+Flag_config 0
+Flag_distances 0
+Flag_correlation 0
+Flag_chessboard 1
+Device_ID 0
+"""
+
+
+def dropin_form(steps: int) -> dict:
+    """The drop-in executable end to end at the production shape (L = 32,
+    T = 0.815, P = 0.45, sampled every 10 steps with data.out and
+    cluster.out rows), Cluster_Steps 0 and 1: wall clock of a run of
+    `steps` + 1000 sampling steps minus a run of 1000, per step, so process
+    start, set-up and config.out are not counted."""
+    import tempfile
+
+    from cvfbulkwater_amd import EXE_PATH
+
+    res = {"workload": "cvf_water3d end to end, L=32, T=0.815, P=0.45, data.out + cluster.out row every "
+                       "10 steps, wall clock per step (difference of two run lengths)"}
+    for nc in (0, 1):
+        times = {}
+        for n in (1000, steps + 1000):
+            with tempfile.TemporaryDirectory() as d:
+                Path(d, "input_data").write_text(DROPIN_INPUT.format(nc=nc, n=n))
+                t0 = time.perf_counter()
+                subprocess.run([str(EXE_PATH)], cwd=d, check=True, capture_output=True, timeout=600)
+                times[n] = time.perf_counter() - t0
+        ms = (times[steps + 1000] - times[1000]) * 1e3 / steps
+        res[f"cluster_steps_{nc}"] = {"ms_per_step": ms, "value": 32 ** 3 / (ms * 1e-3)}
+    return res
 
 
 def reweight_samples(n_sim: int = 11, n: int = 100000, seed: int = 9):
@@ -537,8 +585,15 @@ def main() -> int:
             tuning = sim.autotune(args.tune_steps) if args.tune_steps else sim.tuning
         ms = time_steps(sim, args.steps, args.warmup, ev)
         prof = sim.profile_steps(args.profile_steps)
+        plan = sim.plan(0)
         sim.close()
-        if prof["chunk"][1]:
+        if prof["resident"][1]:
+            # resident kernel: every word staged once and written back once
+            # per launch; the profiled steps run in one launch
+            kname, k_ms = "k_resident", prof["resident"][0]
+            per_step = prof["resident"][1] / args.profile_steps
+            k_bytes = cells * CHUNK_BYTES_PER_CELL
+        elif prof["chunk"][1]:
             # fused slab kernel: one read and one write of every word per launch
             kname, k_ms = "k_chunk", prof["chunk"][0]
             per_step = prof["chunk"][1] / args.profile_steps
@@ -550,7 +605,7 @@ def main() -> int:
             k_bytes = (cells // 2) * SIGMA_BYTES_PER_ACTIVE_CELL * 12 / per_step
         achieved = k_bytes / (k_ms * 1e-3) / 1e9
         out.update(batch_ms=ms, batch_cells=cells, batch_rate=cells / (ms * 1e-3), prof=prof, tuning=tuning,
-                   kname=kname, sig_ms=k_ms, sig_bytes=k_bytes, achieved=achieved)
+                   kname=kname, sig_ms=k_ms, sig_bytes=k_bytes, achieved=achieved, per_step=per_step, plan=plan)
     if args.only in (None, "plain", "plain1"):
         sim = Simulation(Params(L=32, T=0.815, P=0.45, seed=44648604, precision=FP64,
                                 volume_mode=VOLUME_HOST))
@@ -559,7 +614,8 @@ def main() -> int:
         sim.close()
         out.update(plain_ms=pms, plain_rate=32 ** 3 / (pms * 1e-3), plain_tuning=ptuning)
     if args.only in (None, "plain"):
-        out.update(seq=plain_per_L(max(args.plain_steps // 4, 100), args.warmup, ev),
+        out.update(seq=plain_sequential(max(args.plain_steps // 10, 100), args.warmup, ev),
+                   dropin=dropin_form(20000),
                    cluster=cluster_form(max(args.plain_steps // 4, 100), args.warmup, ev),
                    sampled=plain_sampled(args.plain_steps, args.warmup, ev),
                    nonchess=nonchess_form(max(args.plain_steps // 4, 100), args.warmup, ev,
@@ -585,25 +641,26 @@ def main() -> int:
                                       "sweep + 1 volume move per step, FP32-compensated vdW, "
                                       "hipGraph step", "cells": out["batch_cells"],
                           "parallelism": "one GPU, state points batched",
-                          "tuning": out["tuning"]}
-        line["roofline"] = {"bound": "hbm", "kernel": out["kname"], "achieved": out["achieved"],
-                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": out["achieved"] / HBM_PEAK_GBS,
-                            "traffic": measured_traffic(out["kname"], out["tuning"]),
-                            "bytes_per_launch": out["sig_bytes"], "launch_ms": out["sig_ms"]}
+                          "tuning": out["tuning"], "plan_point0": out["plan"]}
+        hbm = {"achieved": out["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": out["achieved"] / HBM_PEAK_GBS, "traffic": measured_traffic(out["kname"], out["tuning"]),
+               "bytes_per_launch": out["sig_bytes"], "launch_ms": out["sig_ms"]}
         line["kernel_ms"] = {k: v[0] for k, v in out["prof"].items() if v[1]}
-        valu = measured_valu(out["kname"], out["tuning"])
+        valu = measured_valu_per_step(out["kname"], out["tuning"], out["per_step"])
         if valu is not None:
-            # the kernel is bound by integer VALU issue, not HBM: its wave64
-            # instructions per MC step against the chip's issue peak
-            # (256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction;
-            # on gfx950 v_mad_u64_u32 and v_mul_hi_u32 issue at that same rate)
-            per_step = out["prof"][{"k_chunk": "chunk", "k_sigma": "sigma"}[out["kname"]]][1] / args.profile_steps
+            # the kernel is bound by integer VALU issue, not HBM (its state is
+            # cache- or LDS-resident): its wave64 instructions per MC step
+            # (rocprofv3 SQ_INSTS_VALU of the same launch configuration)
+            # against the chip's issue peak, 256 CUs x 4 SIMDs x 2.4 GHz / 4
+            # cycles per wave64 instruction (v_mad_u64_u32 and v_mul_hi_u32
+            # issue at that rate on gfx950); HBM kept beside it
             peak = 256 * 4 * 2.4e9 / 4
-            rate = valu[0] * per_step / (out["batch_ms"] * 1e-3)
-            line["valu_issue"] = {"kernel": out["kname"], "insts_per_step": valu[0] * per_step,
-                                  "achieved": rate / 1e9, "peak": peak / 1e9, "unit": "G wave-instr/s",
-                                  "frac": rate / peak}
+            rate = valu / (out["batch_ms"] * 1e-3)
+            line["roofline"] = {"bound": "valu", "kernel": out["kname"], "achieved": rate / 1e9,
+                                "peak": peak / 1e9, "unit": "G wave-instr/s", "frac": rate / peak,
+                                "traffic": hbm["traffic"], "insts_per_step": valu, "hbm": hbm}
+        else:
+            line["roofline"] = {"bound": "hbm", "kernel": out["kname"], **hbm}
     if "plain_rate" in out:
         line["plain"] = {"workload": "plain drop-in form: L=32, T=0.815, P=0.45, FP64 vdW, host "
                                      "volume move", "value": out["plain_rate"],
@@ -614,9 +671,8 @@ def main() -> int:
         line["plain_sampled"] = out["sampled"]
         line["nonchess_form"] = out["nonchess"]
         line["correlation_form"] = out["correlation"]
-        line["plain_sequential_128"] = {
-            "workload": "the 128 batch state points as sequential plain runs (FP64, host volume "
-                        "move), from the per-L plain step times", **out["seq"]}
+        line["dropin_form"] = out["dropin"]
+        line["plain_sequential_128"] = out["seq"]
     if "plain_rate" in out:
         if "batch_rate" in out:
             line["batched_over_plain"] = out["batch_rate"] / out["plain_rate"]

@@ -171,7 +171,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cvf_thermo_derivative": (C.c_int, [DP, DP, DP, C.POINTER(C.c_int32), I32, I32, DP, DP, DP]),
         "cvf_thermo_kt": (C.c_int, [DP, DP, DP, DP, I32, C.c_double, DP, DP]),
         "cvf_set_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
-        "cvf_get_plan": (C.c_int, [P, I32, I32P, I32P, I32P]),
+        "cvf_get_plan": (C.c_int, [P, I32, I32P, I32P, I32P, I32P]),
         "cvf_vdw_energy_noalloc": (C.c_int, [I32, I32, F32P, I32P, I32, C.c_float, DP, I32, I32, DP]),
     }
     for name, (res, args) in sig.items():
@@ -310,12 +310,17 @@ class Simulation:
         t = _Tuning(*[int(cur[k]) for k in self.TUNING_FIELDS])
         _check(self._lib, self._lib.cvf_set_tuning(self._h, C.byref(t)), "cvf_set_tuning")
 
+    PLAN_KINDS = ("unfused", "fused", "resident")
+
     def plan(self, pt: int = 0) -> dict:
-        """Launch plan of point pt: stream group, planes per fused z-slab (0:
-        unfused kernels) and sigma half-sweeps per fused launch."""
-        g, t, k = C.c_int32(), C.c_int32(), C.c_int32()
-        _check(self._lib, self._lib.cvf_get_plan(self._h, pt, C.byref(g), C.byref(t), C.byref(k)), "cvf_get_plan")
-        return {"group": g.value, "slab_planes": t.value, "chunk_sweeps": k.value}
+        """Launch plan of point pt: stream group, planes per z-slab (0:
+        unfused kernels), sigma half-sweeps per fused launch and the kind of
+        Metropolis launch (unfused, fused, resident)."""
+        g, t, k, kind = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _check(self._lib, self._lib.cvf_get_plan(self._h, pt, C.byref(g), C.byref(t), C.byref(k), C.byref(kind)),
+               "cvf_get_plan")
+        return {"group": g.value, "slab_planes": t.value, "chunk_sweeps": k.value,
+                "kind": self.PLAN_KINDS[kind.value]}
 
     def autotune(self, steps_per_trial: int = 20) -> dict:
         """Time candidate launch configurations and keep the fastest (state unchanged)."""
@@ -323,7 +328,7 @@ class Simulation:
         _check(self._lib, self._lib.cvf_autotune(self._h, steps_per_trial, C.byref(t)), "cvf_autotune")
         return {k: getattr(t, k) for k in self.TUNING_FIELDS}
 
-    KCLASSES = ("begin", "eta", "sigma", "sw_local", "sw_union", "sw_flip", "chunk")
+    KCLASSES = ("begin", "eta", "sigma", "sw_local", "sw_union", "sw_flip", "chunk", "resident")
 
     def profile_steps(self, n: int) -> dict:
         """Instrumented steps: mean ms per launch and launch count per kernel class."""

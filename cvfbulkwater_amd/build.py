@@ -35,7 +35,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function",
 ]
 
-LIB_SOURCES = ["cvf_kernels.hip", "cvf_rw_kernels.hip", "cvf_noalloc.hip", "cvf_api.cpp", "cvf_host.cpp", "cvf_corr.cpp",
+LIB_SOURCES = ["cvf_kernels.hip", "cvf_resident.hip", "cvf_rw_kernels.hip", "cvf_noalloc.hip", "cvf_api.cpp", "cvf_host.cpp", "cvf_corr.cpp",
                "cvf_thermo.cpp", "cvf_reweight.cpp"]
 # executable -> sources (each linked against libcvf.so)
 EXES = {"cvf_water3d": ["cvf_water3d.cpp"], "cvf_thermo": ["cvf_thermo_main.cpp"],

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <queue>
 #include <memory>
 #include <string>
 #include <vector>
@@ -90,6 +91,14 @@ struct Group {
   uint32_t chunk_lds_eta = 0;   // the same for the eta sweep alone (fast schedule)
   bool any_nc = false;          // some point runs the non-chessboard rule
   size_t sigma_cells = 0;    // cells swept per fused launch
+  // resident Metropolis kernel (tuning fused = 2): slabs dealt to at most one
+  // workgroup per CU, their neighbour rings and versions
+  bool resident = false;
+  DevBuf<RSlab> rslabs;
+  DevBuf<RGroupEnt> rgroups;
+  DevBuf<uint32_t> rings;
+  DevBuf<uint64_t> rver;
+  uint32_t r_ngroups = 0, r_lds = 0;
   // captured step sequences, longest first: graph_steps, then the powers of
   // two below it, so any step count is a few graph launches
   std::vector<int> gsteps;
@@ -162,6 +171,8 @@ struct cvf_sim {
   DevBuf<unsigned long long> hist;     // sigma histogram of one point (6 counters)
   DevBuf<unsigned long long> hist_series;  // per-step histograms (6 counters per step)
   DevBuf<uint64_t> hist_base;              // step count at the first recorded step
+  DevBuf<uint32_t> rstatus;                // abort flag of the resident kernel
+  bool resident_used = false;              // some launch ran the resident kernel
   uint64_t hist_base_host = 0;
   int rec_pt = -1;                         // point recorded by the groups' rec graphs
   uint32_t nb_cell_all = 0;
@@ -170,6 +181,7 @@ struct cvf_sim {
 
   int64_t steps = 0;
   int64_t obs_at = -1;
+  bool sw_labels = false;  // the labels hold a Swendsen-Wang step's (all zero before the first)
   std::vector<unsigned long long> obs_counters;
 
   void drop_graphs() {
@@ -357,6 +369,18 @@ void init_words(const PointHost& P, std::vector<uint32_t>& w) {
   });
 }
 
+// The resident kernel's abort flag: a neighbour wait that timed out leaves
+// the lattice invalid.  Call after the handle's stream has been synchronised.
+int check_resident(cvf_sim* s) {
+  if (!s->resident_used) return CVF_OK;
+  uint32_t v = 0;
+  HIPC(hipMemcpy(&v, s->rstatus.p, sizeof v, hipMemcpyDeviceToHost));
+  if (v)
+    return fail(CVF_EDEVICE, "resident kernel: a workgroup's wait for its neighbour slab timed out; "
+                             "the lattice state is invalid");
+  return CVF_OK;
+}
+
 int upload_point_words(cvf_sim* s, int pt, const std::vector<uint32_t>& w) {
   HIPC(hipStreamSynchronize(s->stream));
   HIPC(hipMemcpyAsync(s->words.p + s->ph[pt].woff, w.data(), w.size() * 4, hipMemcpyHostToDevice,
@@ -369,7 +393,232 @@ int upload_point_words(cvf_sim* s, int pt, const std::vector<uint32_t>& w) {
 int download_point_words(cvf_sim* s, int pt, std::vector<uint32_t>& w) {
   w.resize(s->ph[pt].N);
   HIPC(hipStreamSynchronize(s->stream));
+  if (int rc = check_resident(s)) return rc;
   HIPC(hipMemcpy(w.data(), s->words.p + s->ph[pt].woff, w.size() * 4, hipMemcpyDeviceToHost));
+  return CVF_OK;
+}
+
+// ------------------------------------------------- resident kernel plan
+uint32_t window_words(uint32_t L, uint32_t T) { return (T == L ? T : T + 2) * L * L; }
+
+struct RDeal {
+  std::vector<RSlab> slabs;   // in workgroup order
+  std::vector<RGroupEnt> groups;
+  uint32_t lds_bytes = 0;
+  size_t ring_words = 0;
+  double makespan = 0;
+};
+
+// Deal the slabs of `points` (T planes each, T per lattice size in Tof) to
+// at most nwg workgroups of at most cap LDS words, longest first onto the
+// least loaded workgroup with room.  Cost of a slab: its cells plus half a
+// plane for the boundary work (crossing cubes, ring copies) when the
+// lattice is split.
+bool deal_slabs(const std::vector<PointHost>& ph, const std::vector<int>& points,
+                const std::vector<std::pair<uint32_t, uint32_t>>& Tof, int nwg, uint32_t cap, RDeal& out) {
+  struct Sl {
+    int point;
+    uint32_t z0, T, L;
+    double cost;
+    uint32_t words;
+  };
+  std::vector<Sl> sl;
+  for (int p : points) {
+    const uint32_t L = ph[p].L;
+    uint32_t T = 0;
+    for (auto& lt : Tof)
+      if (lt.first == L) T = lt.second;
+    if (!T) return false;
+    for (uint32_t z0 = 0; z0 < L; z0 += T)
+      sl.push_back(Sl{p, z0, T, L, (double)T * L * L + (T < L ? 0.5 * L * L : 0.0), window_words(L, T)});
+  }
+  std::stable_sort(sl.begin(), sl.end(), [](const Sl& a, const Sl& b) { return a.cost > b.cost; });
+  std::vector<double> cost(nwg, 0.0);
+  std::vector<uint32_t> words(nwg, 0), cnt(nwg, 0);
+  std::vector<int> wg(sl.size(), -1);
+  // least loaded first; ties to the lower index
+  using E = std::pair<double, int>;
+  std::priority_queue<E, std::vector<E>, std::greater<E>> heap;
+  for (int w = 0; w < nwg; ++w) heap.push(E(0.0, w));
+  std::vector<E> held;
+  for (size_t i = 0; i < sl.size(); ++i) {
+    int best = -1;
+    held.clear();
+    while (!heap.empty()) {
+      const E e = heap.top();
+      heap.pop();
+      if (words[e.second] + sl[i].words <= cap && cnt[e.second] < kMaxRSlabs) {
+        best = e.second;
+        break;
+      }
+      held.push_back(e);
+    }
+    for (const E& e : held) heap.push(e);
+    if (best < 0) return false;
+    wg[i] = best;
+    cost[best] += sl[i].cost;
+    words[best] += sl[i].words;
+    ++cnt[best];
+    heap.push(E(cost[best], best));
+  }
+  // slabs in workgroup order, windows packed in each workgroup's LDS
+  out = RDeal();
+  std::vector<std::vector<size_t>> members(nwg);
+  for (size_t i = 0; i < sl.size(); ++i) members[wg[i]].push_back(i);
+  size_t ring = 0;
+  for (int w = 0; w < nwg; ++w) {
+    if (members[w].empty()) continue;
+    RGroupEnt ge{(uint32_t)out.slabs.size(), (uint32_t)members[w].size()};
+    uint32_t lds = 0;
+    for (size_t i : members[w]) {
+      const Sl& x = sl[i];
+      const PointHost& P = ph[x.point];
+      RSlab r;
+      memset(&r, 0, sizeof r);
+      r.point = x.point;
+      r.L = x.L;
+      r.z0 = x.z0;
+      r.T = x.T;
+      r.ns = x.L / x.T;
+      r.woff = P.woff;
+      r.lds = lds;
+      lds += x.words;
+      r.seed_lo = (uint32_t)P.prm.seed;
+      r.seed_hi = (uint32_t)(P.prm.seed >> 32);
+      r.mH = make_fastdiv(x.L / 2).m;
+      r.mQ = make_fastdiv(x.L / 4).m;
+      r.mP8 = make_fastdiv(x.L * x.L / 8).m;
+      r.mP4 = make_fastdiv(x.L * x.L / 4).m;
+      r.mQQ = make_fastdiv((x.L / 4) * (x.L / 4)).m;
+      r.above = r.below = -1;
+      if (r.ns > 1) {
+        r.ring = (uint32_t)ring;
+        ring += 4 * (size_t)kRingHalf * x.L * x.L;
+      }
+      out.slabs.push_back(r);
+    }
+    out.lds_bytes = std::max(out.lds_bytes, lds * 4);
+    out.groups.push_back(ge);
+    out.makespan = std::max(out.makespan, cost[w]);
+  }
+  if (ring >= 0xFFFFFFFFull) return false;
+  out.ring_words = ring;
+  // neighbours: the slabs of the same point above and below
+  std::vector<std::vector<int32_t>> by_point(ph.size());
+  for (size_t j = 0; j < out.slabs.size(); ++j) by_point[out.slabs[j].point].push_back((int32_t)j);
+  for (auto& r : out.slabs) {
+    if (r.ns == 1) continue;
+    const uint32_t up = (r.z0 + r.T) % r.L, dn = (r.z0 + r.L - r.T) % r.L;
+    for (int32_t j : by_point[r.point]) {
+      if (out.slabs[j].z0 == up) r.above = j;
+      if (out.slabs[j].z0 == dn) r.below = j;
+    }
+  }
+  for (auto& r : out.slabs)
+    if (r.ns > 1) {
+      r.ring_hi = out.slabs[r.above].ring;
+      r.ring_lo = out.slabs[r.below].ring;
+    }
+  return true;
+}
+
+// The resident plan of a group on nwg workgroups of at most cap LDS words:
+// for each lattice size the slab thickness is taken among its divisors
+// whose slabs hold at most about an even share of the cells per workgroup
+// (the five thickest), every combination over the batch's lattice sizes is
+// dealt (a scan of fractions of the even share instead when there would be
+// more than 400), and the deal with the shortest longest workgroup is kept.
+// False when the group's points cannot run resident (fast schedule, several
+// Metropolis sweeps per step, the non-chessboard rule, or a window beyond
+// the LDS).
+bool plan_resident(const cvf_sim* s, const Group& g, int nwg, uint32_t cap, RDeal& best) {
+  if (s->sched != CVF_SCHED_REFERENCE || s->nm != 1) return false;
+  std::vector<uint32_t> Ls;
+  uint64_t total = 0;
+  for (int p : g.points) {
+    if (!s->ph[p].prm.flag_chessboard) return false;
+    Ls.push_back(s->ph[p].L);
+    total += s->ph[p].N;
+  }
+  std::sort(Ls.begin(), Ls.end());
+  Ls.erase(std::unique(Ls.begin(), Ls.end()), Ls.end());
+  const double target = (double)total / nwg;
+  std::vector<std::vector<uint32_t>> cand(Ls.size());
+  size_t combos = 1;
+  for (size_t i = 0; i < Ls.size(); ++i) {
+    const uint32_t L = Ls[i];
+    for (uint32_t t = L; t >= 1 && cand[i].size() < 5; --t)
+      if (L % t == 0 && window_words(L, t) <= cap && (double)t * L * L <= 1.05 * target) cand[i].push_back(t);
+    if (cand[i].empty() && window_words(L, 1) <= cap) cand[i].push_back(1);
+    if (cand[i].empty()) return false;
+    combos *= cand[i].size();
+  }
+  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> trials;
+  if (combos <= 400) {
+    for (size_t c = 0; c < combos; ++c) {
+      std::vector<std::pair<uint32_t, uint32_t>> Tof;
+      size_t r = c;
+      for (size_t i = 0; i < Ls.size(); ++i) {
+        Tof.emplace_back(Ls[i], cand[i][r % cand[i].size()]);
+        r /= cand[i].size();
+      }
+      trials.push_back(Tof);
+    }
+  } else {
+    for (double f : {1.0, 1.0 / 2, 1.0 / 3, 1.0 / 4, 1.0 / 6, 1.0 / 8, 1.0 / 12, 1.0 / 16}) {
+      std::vector<std::pair<uint32_t, uint32_t>> Tof;
+      for (size_t i = 0; i < Ls.size(); ++i) {
+        uint32_t T = cand[i].back();
+        for (uint32_t t : cand[i])
+          if ((double)t * Ls[i] * Ls[i] <= f * target) {
+            T = t;
+            break;
+          }
+        Tof.emplace_back(Ls[i], T);
+      }
+      trials.push_back(Tof);
+    }
+  }
+  bool found = false;
+  for (const auto& Tof : trials) {
+    RDeal d;
+    if (!deal_slabs(s->ph, g.points, Tof, nwg, cap, d)) continue;
+    if (!found || d.makespan < best.makespan) {
+      best = std::move(d);
+      found = true;
+    }
+  }
+  return found;
+}
+
+template <class T>
+int upload_vec(DevBuf<T>& buf, const std::vector<T>& v);
+
+// With fused = 2, `streams` is the number of resident workgroups per CU (1
+// or 2): every workgroup of the launch must be resident at once, so the
+// plan is checked against the occupancy the runtime reports for its
+// workgroup size and LDS, and falls back to fused slab launches otherwise.
+int build_resident(cvf_sim* s, Group& g) {
+  g.resident = false;
+  int ncu = 0;
+  HIPC(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device));
+  if (ncu <= 0) return CVF_OK;
+  const int per_cu = std::min(2, std::max(1, s->tune.streams));
+  const uint32_t cap = (resident_lds_limit() - (uint32_t)(per_cu - 1) * kResidentStatic) / (uint32_t)per_cu / 4u;
+  RDeal d;
+  if (!plan_resident(s, g, ncu * per_cu, cap, d)) return CVF_OK;
+  int occ = 0;
+  HIPC(resident_occupancy(&occ, (uint32_t)s->tune.block_chunk, d.lds_bytes));
+  if (occ < per_cu) return CVF_OK;
+  int rc;
+  if ((rc = upload_vec(g.rslabs, d.slabs)) || (rc = upload_vec(g.rgroups, d.groups))) return rc;
+  HIPC(g.rings.alloc(std::max<size_t>(d.ring_words, 4)));
+  HIPC(g.rver.alloc(2 * d.slabs.size()));
+  HIPC(hipMemset(g.rver.p, 0, 2 * d.slabs.size() * sizeof(uint64_t)));
+  g.r_ngroups = (uint32_t)d.groups.size();
+  g.r_lds = d.lds_bytes;
+  g.resident = true;
+  for (const auto& r : d.slabs) s->ph[r.point].slab_T = r.T;
   return CVF_OK;
 }
 
@@ -525,7 +774,9 @@ int build_groups(cvf_sim* s) {
   HIPC(hipDeviceSynchronize());
   const uint64_t slab_cells = auto_slab_cells(s);
   s->groups.clear();
-  const int ng = std::max(1, std::min(s->tune.streams, s->npts));
+  // the resident kernel takes every point in one group (its workgroups must
+  // all be resident at once, one per CU at most)
+  const int ng = s->tune.fused == 2 ? 1 : std::max(1, std::min(s->tune.streams, s->npts));
   for (int g = 0; g < ng; ++g) s->groups.emplace_back(new Group);
   std::vector<int> order(s->npts);
   for (int i = 0; i < s->npts; ++i) order[i] = i;
@@ -557,7 +808,7 @@ int build_groups(cvf_sim* s) {
     if ((rc = upload_vec(g->plist, pl))) return rc;
     // fused path when every point's slab fits in LDS: the reference
     // schedule's eta sweep and half-sweeps, or the fast schedule's eta sweep
-    g->fused = s->tune.fused != 0;
+    g->fused = s->tune.fused != 0;  // (fused = 2: the resident kernel where it applies, else fused slabs)
     std::vector<BlockEnt> ts;
     g->chunk_lds = 0;
     g->chunk_lds_long = 0;
@@ -607,6 +858,7 @@ int build_groups(cvf_sim* s) {
       if ((rc = upload_vec(g->tab_slab, ts))) return rc;
       g->nb_slab = (uint32_t)ts.size();
     }
+    if (s->tune.fused == 2 && (rc = build_resident(s, *g))) return rc;
   }
   HIPC(s->cur_steps.alloc(s->groups.size()));
   int rc0 = upload_group_steps(s);
@@ -644,7 +896,14 @@ void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Time
   const Launch lh{g.tab_half.p, g.nb_half, (uint32_t)s->tune.block_sigma, stepp, step_off};
   const Launch lc{g.tab_cube.p, g.nb_cube, (uint32_t)s->tune.block_eta, stepp, step_off};
   const Launch ll{g.tab_cell.p, g.nb_cell, (uint32_t)s->tune.block_cell, stepp, step_off};
-  if (s->sched == CVF_SCHED_FAST) {
+  if (g.resident) {
+    s->resident_used = true;
+    // one step of the resident kernel, in place on `words`
+    timed(CVF_KCLASS_RESIDENT, [&] {
+      launch_resident(st, g.rgroups.p, g.r_ngroups, (uint32_t)s->tune.block_chunk, g.r_lds, g.rslabs.p, s->pts.p,
+                      s->words.p, g.rings.p, g.rver.p, stepp, step_off, 1, s->rstatus.p);
+    });
+  } else if (s->sched == CVF_SCHED_FAST) {
     // eta cube sweep, then n_metropolis x the two colour passes (in place,
     // on the array holding the state)
     uint32_t* cur = par ? s->words2.p : s->words.p;
@@ -703,6 +962,8 @@ constexpr uint32_t kHistSpan = 4096;  // per-step histograms per copy back
 // state in `words`.
 int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps, bool rec = false) {
   Group& g = *s->groups[gi];
+  const int nsteps_all = nsteps;
+  if (g.resident) s->resident_used = true;
   const bool ok = g.fused && s->nm > 0;
   bool variant_ok = false;
   int flips = 0;  // word-array swaps of one variant step
@@ -714,6 +975,12 @@ int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps, bool rec = fal
     flips = 1;
   }
   int par = 0;
+  if (g.resident && s->nc == 0 && !rec) {
+    // every step of the sequence in one resident launch
+    launch_resident(st, g.rgroups.p, g.r_ngroups, (uint32_t)s->tune.block_chunk, g.r_lds, g.rslabs.p, s->pts.p,
+                    s->words.p, g.rings.p, g.rver.p, s->cur_steps.p + gi, 0, (uint32_t)nsteps, s->rstatus.p);
+    nsteps = 0;
+  }
   for (int i = 0; i < nsteps; ++i) {
     const bool variant = variant_ok && (i < nsteps - 1 || (par ^ flips) == 0);
     enqueue_lattice(s, gi, st, (uint32_t)i, Untimed{}, par, variant);
@@ -724,7 +991,7 @@ int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps, bool rec = fal
     }
   }
   launch_vol_chain(st, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, s->cur_steps.p + gi,
-                   (uint32_t)nsteps, s->dist.p, s->freq.p, s->precision, s->vol_device);
+                   (uint32_t)nsteps_all, s->dist.p, s->freq.p, s->precision, s->vol_device);
   return CVF_OK;
 }
 
@@ -817,7 +1084,7 @@ int check_tuning(const cvf_tuning& t) {
   if (!valid_block(t.block_chunk)) return fail(CVF_EINVAL, "block_chunk must be a power of 2 in [64, 1024]");
   if (t.slab_planes < 0 || t.slab_planes > kMaxSlab)
     return fail(CVF_EINVAL, "slab_planes must be in [0, %d]", kMaxSlab);
-  if (t.fused != 0 && t.fused != 1) return fail(CVF_EINVAL, "fused must be 0 or 1");
+  if (t.fused < 0 || t.fused > 2) return fail(CVF_EINVAL, "fused must be 0, 1 or 2");
   if (t.chunk_sweeps != 0 && t.chunk_sweeps != kChunk && t.chunk_sweeps != kMaxChunk)
     return fail(CVF_EINVAL, "chunk_sweeps must be 0, %d or %d", kChunk, kMaxChunk);
   return CVF_OK;
@@ -857,7 +1124,8 @@ int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
   int ndev = 0;
   HIPC(hipGetDeviceCount(&ndev));
   if (pts[0].device_id < 0 || pts[0].device_id >= ndev)
-    return fail(CVF_EINVAL, "Error: Tried to execute in GPU device ID = %d (total %d)",
+    // the reference's message, typo included (gpu_water3D.cu:461-465)
+    return fail(CVF_EINVAL, "Error: Tried to execute in GPU deive ID = %d\n\tTotal number of devices is %d",
                 pts[0].device_id, ndev);
   HIPC(hipSetDevice(pts[0].device_id));
   std::unique_ptr<cvf_sim> s(new cvf_sim);
@@ -923,6 +1191,8 @@ int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
   s->max_slots = 3 * maxL * maxL + 1;  // a cluster of >= 2L nodes: at most 6N/(2L)
   HIPC(s->cover.alloc((size_t)s->max_slots * 3 * ((maxL + 31) / 32)));
   HIPC(s->nslot_nperc.alloc(2));
+  HIPC(s->rstatus.alloc(4));
+  HIPC(hipMemset(s->rstatus.p, 0, 4 * sizeof(uint32_t)));
   int rc = upload_tables(s.get());
   if (rc) return rc;
   std::vector<PointState> st(n);
@@ -1052,6 +1322,7 @@ int cvf_get_config(cvf_sim* s, int32_t pt, uint8_t* sigma6N, uint8_t* eta6N) {
 int run_steps(cvf_sim* s, int64_t n_steps, int rec_gi) {
   int rc = capture_step_graphs(s);
   if (rc) return rc;
+  if (s->nc > 0 && n_steps > 0) s->sw_labels = true;
   if (rec_gi >= 0 && (rc = capture_rec_graphs(s, rec_gi))) return rc;
   if ((rc = fork_groups(s))) return rc;
   const std::vector<int>& sizes = s->groups[0]->gsteps;  // the same for every group
@@ -1086,7 +1357,7 @@ int cvf_sync(cvf_sim* s) {
   if (!s) return fail(CVF_EINVAL, "null handle");
   HIPC(hipSetDevice(s->device));
   HIPC(hipStreamSynchronize(s->stream));
-  return CVF_OK;
+  return check_resident(s);
 }
 
 int cvf_get_volume(cvf_sim* s, int32_t pt, double* r_cell, double* V, double* E_vdw) {
@@ -1119,6 +1390,7 @@ int cvf_observe(cvf_sim* s, int32_t pt, cvf_obs* o) {
     HIPC(hipMemcpyAsync(s->obs_counters.data(), s->counters.p, 16 * (size_t)s->npts,
                         hipMemcpyDeviceToHost, s->stream));
     HIPC(hipStreamSynchronize(s->stream));
+    if ((rc = check_resident(s))) return rc;
     s->obs_at = s->steps;
   }
   double r_cell, V, E_vdw;
@@ -1215,40 +1487,21 @@ int cvf_get_labels(cvf_sim* s, int32_t pt, int32_t* labels6N) {
   if (rc) return rc;
   HIPC(hipSetDevice(s->device));
   HIPC(hipStreamSynchronize(s->stream));
+  if ((rc = check_resident(s))) return rc;
   const PointHost& P = s->ph[pt];
   HIPC(hipMemcpy(labels6N, s->parent.p + P.loff, 6 * (size_t)P.N * 4, hipMemcpyDeviceToHost));
   return CVF_OK;
 }
 
-int cvf_cluster_stats(cvf_sim* s, int32_t pt, cvf_cstat* c) {
-  int rc = check_pt(s, pt);
-  if (rc) return rc;
-  if (!c) return fail(CVF_EINVAL, "null output");
-  HIPC(hipSetDevice(s->device));
-  const PointHost& P = s->ph[pt];
-  const uint32_t nn = 6 * P.N, L = P.L;
-  const uint32_t* label = s->parent.p + P.loff;
-  const int32_t minsize = 2 * (int32_t)L;
-  const uint32_t nw = (L + 31) / 32;
-  launch_zero_u32(s->stream, (uint32_t*)s->csize.p, nn);
-  launch_zero_u32(s->stream, s->nslot_nperc.p, 2);
-  launch_zero_u32(s->stream, s->cover.p, (size_t)s->max_slots * 3 * nw);
-  launch_cluster_sizes(s->stream, label, nn, s->csize.p);
-  launch_perc_slots(s->stream, label, nn, s->csize.p, minsize, s->slot.p, s->nslot_nperc.p,
-                    s->max_slots);
-  launch_perc_cover(s->stream, label, nn, L, s->csize.p, minsize, s->slot.p, s->cover.p, s->max_slots);
-  launch_perc_count(s->stream, s->cover.p, s->nslot_nperc.p, L, s->max_slots, s->nslot_nperc.p + 1);
-  std::vector<int32_t> size(nn);
-  uint32_t np[2];
-  HIPC(hipMemcpyAsync(size.data(), s->csize.p, (size_t)nn * 4, hipMemcpyDeviceToHost, s->stream));
-  HIPC(hipMemcpyAsync(np, s->nslot_nperc.p, 8, hipMemcpyDeviceToHost, s->stream));
-  HIPC(hipStreamSynchronize(s->stream));
-  if (np[0] > s->max_slots) return fail(CVF_EDEVICE, "percolation slot overflow");
-  // print_cluster_statistics (gpu_water3D.cu:2677-2721): float sums in
-  // ascending label order, int32 size*size with wrap-around.
+namespace {
+
+// print_cluster_statistics (gpu_water3D.cu:2677-2721) on the cluster sizes
+// indexed by label (0 where no cluster): float sums in ascending label
+// order, int32 size*size with wrap-around.
+void assemble_cluster_row(const int32_t* size, uint32_t n, int32_t n_perc, cvf_cstat* c) {
   int32_t largest = 0, second = 0, ncl = 0;
   float average = 0, average2 = 0;
-  for (uint32_t i = 0; i < nn; ++i) {
+  for (uint32_t i = 0; i < n; ++i) {
     const int32_t sz = size[i];
     if (sz == 0) continue;
     ++ncl;
@@ -1266,10 +1519,46 @@ int cvf_cluster_stats(cvf_sim* s, int32_t pt, cvf_cstat* c) {
   average2 -= average * average;
   c->largest = largest;
   c->second = second;
-  c->n_perc = (int32_t)np[1];
+  c->n_perc = n_perc;
   c->n_clusters = ncl;
   c->mean = average;
   c->var = average2;
+}
+
+}  // namespace
+
+int cvf_cluster_stats(cvf_sim* s, int32_t pt, cvf_cstat* c) {
+  int rc = check_pt(s, pt);
+  if (rc) return rc;
+  if (!c) return fail(CVF_EINVAL, "null output");
+  HIPC(hipSetDevice(s->device));
+  const PointHost& P = s->ph[pt];
+  const uint32_t nn = 6 * P.N, L = P.L;
+  if (!s->sw_labels) {
+    // no Swendsen-Wang step yet: every label is 0, one cluster of all 6N
+    // nodes that covers every x (the row SOURCE/cluster.out shows)
+    const int32_t all = (int32_t)nn;
+    assemble_cluster_row(&all, 1, nn >= 2 * L ? 1 : 0, c);
+    return CVF_OK;
+  }
+  const uint32_t* label = s->parent.p + P.loff;
+  const int32_t minsize = 2 * (int32_t)L;
+  const uint32_t nw = (L + 31) / 32;
+  launch_zero_u32(s->stream, (uint32_t*)s->csize.p, nn);
+  launch_zero_u32(s->stream, s->nslot_nperc.p, 2);
+  launch_zero_u32(s->stream, s->cover.p, (size_t)s->max_slots * 3 * nw);
+  launch_cluster_sizes(s->stream, label, nn, s->csize.p);
+  launch_perc_slots(s->stream, label, nn, s->csize.p, minsize, s->slot.p, s->nslot_nperc.p,
+                    s->max_slots);
+  launch_perc_cover(s->stream, label, nn, L, s->csize.p, minsize, s->slot.p, s->cover.p, s->max_slots);
+  launch_perc_count(s->stream, s->cover.p, s->nslot_nperc.p, L, s->max_slots, s->nslot_nperc.p + 1);
+  std::vector<int32_t> size(nn);
+  uint32_t np[2];
+  HIPC(hipMemcpyAsync(size.data(), s->csize.p, (size_t)nn * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipMemcpyAsync(np, s->nslot_nperc.p, 8, hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipStreamSynchronize(s->stream));
+  if (np[0] > s->max_slots) return fail(CVF_EDEVICE, "percolation slot overflow");
+  assemble_cluster_row(size.data(), nn, (int32_t)np[1], c);
   return CVF_OK;
 }
 
@@ -1351,9 +1640,27 @@ int cvf_profile_steps(cvf_sim* s, int64_t n_steps, double* kernel_ms, int64_t* l
   };
   int rc = fork_groups(s);
   if (rc) return rc;
+  if (s->nc > 0 && n_steps > 0) s->sw_labels = true;
+  // a resident group without SW steps runs all n_steps in one launch, as in
+  // its captured graphs (timed once, with one volume-chain launch)
+  for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+    Group& g = *s->groups[gi];
+    if (!g.resident || s->nc > 0 || n_steps == 0) continue;
+    s->resident_used = true;
+    timed(CVF_KCLASS_RESIDENT, [&] {
+      launch_resident(g.stream, g.rgroups.p, g.r_ngroups, (uint32_t)s->tune.block_chunk, g.r_lds, g.rslabs.p,
+                      s->pts.p, s->words.p, g.rings.p, g.rver.p, s->cur_steps.p + gi, 0, (uint32_t)n_steps,
+                      s->rstatus.p);
+    });
+    timed(CVF_KCLASS_BEGIN, [&] {
+      launch_vol_chain(g.stream, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, s->cur_steps.p + gi,
+                       (uint32_t)n_steps, s->dist.p, s->freq.p, s->precision, s->vol_device);
+    });
+  }
   for (int64_t i = 0; i < n_steps; ++i) {
     for (size_t gi = 0; gi < s->groups.size(); ++gi) {
       Group& g = *s->groups[gi];
+      if (g.resident && s->nc == 0) continue;
       hipStream_t st = g.stream;
       int par = 0;
       enqueue_lattice(s, (int)gi, st, 0, timed, par);
@@ -1388,12 +1695,19 @@ int cvf_profile_steps(cvf_sim* s, int64_t n_steps, double* kernel_ms, int64_t* l
   return CVF_OK;
 }
 
-int cvf_get_plan(cvf_sim* s, int32_t pt, int32_t* group, int32_t* slab_planes, int32_t* chunk_sweeps) {
+int cvf_get_plan(cvf_sim* s, int32_t pt, int32_t* group, int32_t* slab_planes, int32_t* chunk_sweeps,
+                 int32_t* kind) {
   int rc = check_pt(s, pt);
   if (rc) return rc;
   const PointHost& P = s->ph[pt];
   const Group& g = *s->groups[P.group];
   if (group) *group = P.group;
+  if (kind) *kind = g.resident ? 2 : P.slab_T ? 1 : 0;
+  if (g.resident) {
+    if (slab_planes) *slab_planes = (int32_t)P.slab_T;
+    if (chunk_sweeps) *chunk_sweeps = 0;
+    return CVF_OK;
+  }
   if (slab_planes) *slab_planes = (int32_t)P.slab_T;
   if (chunk_sweeps) {
     // half-sweeps per fused launch of the reference schedule: whole passes
@@ -1515,6 +1829,7 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
   std::vector<VolState> hv;
   for (auto& P : s->ph) hv.push_back(P.vol);
   const int64_t steps0 = s->steps;
+  const bool sw0 = s->sw_labels;
   // candidates
   std::vector<cvf_tuning> cand;
   const cvf_tuning base = default_tuning(s);
@@ -1548,6 +1863,18 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
           t.slab_planes = slab;
           t.block_chunk = bc;
           t.chunk_sweeps = len;
+          cand.push_back(t);
+        }
+    if (streams <= 2)
+      for (int bc : {256, 512, 1024})
+        for (int gs : {16, 64}) {
+          // resident kernel: one stream group, `streams` workgroups per CU
+          cvf_tuning t = base;
+          t.streams = streams;
+          t.fused = 2;
+          t.block_chunk = bc;
+          t.graph_steps = gs;
+          t.chunk_sweeps = kChunk;
           cand.push_back(t);
         }
     for (int bs : {64, 256}) {
@@ -1589,6 +1916,7 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
   HIPC(hipMemcpy(s->pst.p, p0.p, s->npts * sizeof(PointState), hipMemcpyDeviceToDevice));
   for (size_t i = 0; i < s->ph.size(); ++i) s->ph[i].vol = hv[i];
   s->steps = steps0;
+  s->sw_labels = sw0;
   s->obs_at = -1;
   if (rc) return rc;
   if ((rc = upload_group_steps(s))) return rc;

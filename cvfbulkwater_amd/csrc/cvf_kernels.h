@@ -111,5 +111,45 @@ void launch_perc_count(hipStream_t st, const uint32_t* cover, const uint32_t* ns
                        uint32_t max_slots, uint32_t* nperc);
 void launch_vdw_eval(hipStream_t st, const float* dist, const int32_t* freq, int nshell,
                      float r_cutoff, int N, const double* r, int n_r, int precision, double* out);
+// the events set by set_launch_timing (nullptr when launches are not timed)
+void get_launch_timing(hipEvent_t* start, hipEvent_t* stop);
+
+// ------------------------------------------------- resident Metropolis kernel
+// One z-slab of one state point held in a resident workgroup's LDS
+// (cvf_resident.hip).  Its window: the ghost of the plane below, its T
+// planes, the ghost of the plane above (no ghosts when ns = 1: the whole
+// lattice, periodic in LDS).  Its rings in `rings` (words): [ring, ring +
+// 2 kRingHalf L^2) its bottom plane as published for the slab below, then
+// its top plane for the slab above; versions ver[2 i] (bottom), ver[2 i + 1]
+// (top) for slab i.
+struct RSlab {
+  int32_t point;
+  uint32_t L, z0, T, ns;      // ns: slabs of the point
+  uint32_t woff, lds;         // the point's words; the window's first LDS word
+  uint32_t seed_lo, seed_hi;
+  uint32_t mH, mQ, mP8, mP4, mQQ;  // fast-division multipliers of L/2, L/4, L^2/8, L^2/4, (L/4)^2
+  int32_t above, below;       // neighbouring slabs (ns > 1), indices into the slab array
+  uint32_t ring;              // this slab's rings
+  uint32_t ring_hi, ring_lo;  // the rings of `above` and `below`
+  uint32_t pad;
+};
+static_assert(sizeof(RSlab) % 16 == 0, "RSlab is copied in words");
+// Workgroup g of a resident launch takes slabs [first, first + count).
+struct RGroupEnt {
+  uint32_t first, count;
+};
+constexpr uint32_t kMaxRSlabs = 8;   // slabs per resident workgroup
+constexpr uint32_t kRingHalf = 16;   // ring slots per step parity (15: end of step)
+constexpr uint32_t kResidentStatic = 6144;  // static LDS reserved for a resident workgroup
+// largest dynamic LDS of a resident workgroup (its static arrays take the
+// rest of the CU's 160 KB; each further workgroup on a CU takes another
+// kResidentStatic)
+uint32_t resident_lds_limit();
+// resident workgroups of `block` threads and `lds_bytes` of dynamic LDS that
+// fit on one CU at once
+hipError_t resident_occupancy(int* per_cu, uint32_t block, uint32_t lds_bytes);
+void launch_resident(hipStream_t st, const RGroupEnt* grp, uint32_t ngroups, uint32_t block, uint32_t lds_bytes,
+                     const RSlab* slabs, const PointDev* pts, uint32_t* words, uint32_t* rings, uint64_t* ver,
+                     const uint64_t* stepp, uint32_t step_off, uint32_t nsteps, uint32_t* status);
 
 }  // namespace cvf

@@ -13,6 +13,7 @@
 // scalar load.  Workgroup ids are remapped so that each XCD gets a
 // contiguous range of the table (each XCD has its own L2).
 #include "cvf_kernels.h"
+#include "cvf_device.h"
 
 #include <atomic>
 
@@ -184,50 +185,6 @@ __global__ __launch_bounds__(64) void k_vol_chain(const PointDev* __restrict__ p
 // thread updates its 8 corner words without atomics.  Cube states of both
 // families are independent given sigma, so running the families one after
 // the other equals the reference's simultaneous update.
-// Cube-move table: entry s holds the active-edge mask of solution s in bits
-// 0-11 and, in bits 12 + 6b .. 17 + 6b, the eta bits (arms 0..5) that the
-// mask gives corner b (b = bx + 2 by + 4 bz) on the arms it owns.
-__device__ __forceinline__ uint64_t eta_table_entry(uint32_t s) {
-  const uint32_t M = sol_mask(s);
-  uint64_t t = M;
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
-    const uint32_t bits = (((M >> corner_edge(b, 0)) & 1u) << bx) | (((M >> corner_edge(b, 1)) & 1u) << (2 + by)) |
-                          (((M >> corner_edge(b, 2)) & 1u) << (4 + bz));
-    t |= (uint64_t)bits << (12 + 6 * b);
-  }
-  return t;
-}
-
-// One cube move on its 8 corner words w (corner b = bx + 2 by + 4 bz):
-// proposal floor(9u), Delta N_HB over the 12 edges, accept with
-// min(1, exp(beta Jeff DeltaN)); store(b, word) writes a changed corner.
-template <class Store>
-__device__ __forceinline__ void eta_cube_move(const uint32_t (&w)[8], const uint64_t* thr, const uint64_t* etab,
-                                              uint32_t rw, Store&& store) {
-  uint32_t D = 0, E = 0;
-#pragma unroll
-  for (int jj = 0; jj < kEdges; ++jj) {
-    const int A = edge_start(jj), ax = edge_axis(jj), B = A | (1 << ax);
-    D |= (uint32_t)(sigma_of(w[A], 2 * ax) == sigma_of(w[B], 2 * ax + 1)) << jj;
-    E |= eta_of(w[A], 2 * ax) << jj;
-  }
-  uint32_t prop, acc;
-  split_word(rw, 9, prop, acc);
-  const uint64_t ent = etab[prop];
-  const uint32_t M = (uint32_t)ent & 0xFFFu;
-  const int dn = popc32(M & D) - popc32(E & D);
-  if ((uint64_t)acc < thr[dn + 4] && M != E) {
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint32_t own = (1u << (b & 1)) | (1u << (2 + ((b >> 1) & 1))) | (1u << (4 + ((b >> 2) & 1)));
-      const uint32_t bits = (uint32_t)(ent >> (12 + 6 * b)) & 0x3Fu;
-      store(b, (w[b] & ~(own << kEtaShift)) | (bits << kEtaShift));
-    }
-  }
-}
-
 // Four threads per quad of cubes of the family with origins
 // (ox + b L/2, oy + a L/2, oz), a,b in {0,1}: the quad's Philox block (each
 // of the four lanes forms it), word a + 2b for cube a,b, one cube per
@@ -280,41 +237,6 @@ __global__ void k_eta(const BlockEnt* __restrict__ tab, uint32_t nb, const uint6
 // (indices = 0).  A cell's new sigma_a depends on its own word and on the
 // facing arm of the neighbour across arm a; that neighbour has the other
 // parity along the axis, so no word read in a launch is written in it.
-// Sigma acceptance thresholds as 32-bit values min(thr, 2^32 - 1).  Exact:
-// the acceptance word of the q = 6 split is (6w) mod 2^32, always even, so
-// it is below 2^32 - 1 exactly when it is below 2^32.
-__device__ __forceinline__ uint32_t sigma_thr32(uint64_t t) {
-  return t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
-}
-
-// Acceptance-table index (h + 1) * 11 + d + 5 of a chessboard sigma move of
-// arm `arm` from sa to sp (sp != sa; any in-range index otherwise) on word w
-// with facing neighbour word wn, where h = eta ? delta(sp, snb) -
-// delta(sa, snb) : 0 and d = count_eq(w, sp) - count_eq(w, sa) + 1.
-// Branch-free: with sp != sa at most one of the deltas is 1, so (h + 1) * 11
-// is a two-way select; d + 5 = #(fields != sa) + #(fields == sp), each from
-// the zero-field test ((y & 011) + 011 | y) & 100 on every 3-bit field of
-// y = w ^ (s * 001) (no carry leaves a field), the counts chained through
-// the popcount's addend.
-// popcount(x) + acc in one v_bcnt_u32_b32 (written out so that the compiler
-// does not spread the table's x4 scaling over the terms of the sum)
-__device__ __forceinline__ uint32_t popc_add(uint32_t x, uint32_t acc) {
-  uint32_t r;
-  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
-  return r;
-}
-
-__device__ __forceinline__ uint32_t sigma_thr_index(uint32_t w, uint32_t wn, int arm, uint32_t sa, uint32_t sp) {
-  constexpr uint32_t kLo2 = 0x1B6DBu, kHi = 0x24924u;  // bits 0-1, bit 2 of each field
-  const uint32_t snb = sigma_of(wn, arm ^ 1);
-  const bool eta = (w >> (kEtaShift + arm)) & 1u;
-  const uint32_t h11 = eta && sp == snb ? 22u : eta && sa == snb ? 0u : 11u;
-  const uint32_t ya = w ^ (sa * kFieldLsb), yp = w ^ (sp * kFieldLsb);
-  const uint32_t ne_a = (((ya & kLo2) + kLo2) | ya) & kHi;
-  const uint32_t eq_p = ~(((yp & kLo2) + kLo2) | yp) & kHi;
-  return popc_add(ne_a, popc_add(eq_p, h11));
-}
-
 // The sigma rule for the four cells of a quad: words at W[cell[m]], facing
 // neighbours at W[nb[m]] (LDS or global memory), random words from draw().
 // The cells and their neighbours are distinct words (active cells against
@@ -475,35 +397,6 @@ __device__ __forceinline__ void eta_cube_plane(uint32_t* Wp, const uint64_t* thr
 #pragma unroll
   for (int b = 0; b < 8; ++b) w[b] = Wp[idx[b]];
   eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t v) { Wp[idx[b]] = v; });
-}
-
-// Cells and facing neighbours (window indices) of the in-plane (x or y arm)
-// quad with origin (x, y) on the plane starting at word `base`: cells (x, y),
-// (x, y + L/2), (x + L/2, y), (x + L/2, y + L/2) with x, y < L/2.  Along the
-// arm's axis only the coordinate at 0 (-1 step) or L-1 (+1 step) wraps, i.e.
-// the low cells when x (y) = 0 and the high cells when x (y) = L/2 - 1.
-__device__ __forceinline__ void quad_plane_cells(uint32_t base, uint32_t L, uint32_t x, uint32_t y, int arm,
-                                                 int ax, uint32_t* cell, uint32_t* nb) {
-  const int iL = (int)L, hL = iL / 2, cq = ax == 0 ? (int)x : (int)y, st = ax == 0 ? 1 : iL;
-  const int wrap = (iL - 1) * st;
-  int off_lo, off_hi;
-  if (arm & 1) {
-    off_lo = cq == 0 ? wrap : -st;
-    off_hi = -st;
-  } else {
-    off_lo = st;
-    off_hi = cq == hL - 1 ? -wrap : st;
-  }
-  const uint32_t b0 = base + x + L * y, dy = (uint32_t)hL * L, dx = (uint32_t)hL;
-  cell[0] = b0;
-  cell[1] = b0 + dy;
-  cell[2] = b0 + dx;
-  cell[3] = b0 + dx + dy;
-  // the cells on the high side of the arm's axis: 2, 3 for x, 1, 3 for y
-  nb[0] = cell[0] + off_lo;
-  nb[1] = cell[1] + (ax == 0 ? off_lo : off_hi);
-  nb[2] = cell[2] + (ax == 0 ? off_hi : off_lo);
-  nb[3] = cell[3] + off_hi;
 }
 
 // z-arm quad with origin (x, y) on the plane starting at `base`: the
@@ -1047,10 +940,27 @@ __global__ void k_sw_flip(const BlockEnt* __restrict__ tab, uint32_t nb,
 }
 
 // ------------------------------------------------- cluster statistics
-__global__ void k_cluster_sizes(const uint32_t* __restrict__ label, uint32_t nn,
-                                int32_t* __restrict__ size) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nn) atomicAdd(&size[label[i]], 1);
+// Nodes are visited in index order, 64 consecutive nodes per wavefront, so
+// the nodes of one cluster come in runs of lanes (consecutive cells of one
+// arm along x).  Each run is reduced inside the wavefront and its first lane
+// makes one global atomic for it, instead of one atomic per node on the same
+// address (a percolating cluster holds a large share of all 6N nodes).
+
+// End of the run of lanes that starts at or before `lane`: the first lane
+// above it that starts a run, or 64.  `heads` is the ballot of run starts.
+__device__ __forceinline__ uint32_t run_end(uint64_t heads, uint32_t lane) {
+  const uint64_t above = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1ull);
+  return above ? (uint32_t)__builtin_ctzll(above) : 64u;
+}
+
+__global__ __launch_bounds__(256) void k_cluster_sizes(const uint32_t* __restrict__ label, uint32_t nn,
+                                                       int32_t* __restrict__ size) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63u;
+  const uint32_t lab = i < nn ? label[i] : 0xFFFFFFFFu;  // no node has that label
+  const uint32_t prev = __shfl_up(lab, 1, 64);
+  const bool head = lane == 0 || lab != prev;
+  const uint64_t heads = __ballot(head);
+  if (head && i < nn) atomicAdd(&size[lab], (int32_t)(run_end(heads, lane) - lane));
 }
 
 __global__ void k_perc_slots(const uint32_t* __restrict__ label, uint32_t nn,
@@ -1064,24 +974,52 @@ __global__ void k_perc_slots(const uint32_t* __restrict__ label, uint32_t nn,
   }
 }
 
-__global__ void k_perc_cover(const uint32_t* __restrict__ label, uint32_t nn, uint32_t L,
-                             const int32_t* __restrict__ size, int32_t minsize,
-                             const int32_t* __restrict__ slot, uint32_t* __restrict__ cover,
-                             uint32_t max_slots) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nn) return;
-  const uint32_t r = label[i];
-  if (size[r] < minsize) return;
-  const int32_t s = slot[r];
-  if (s < 0 || (uint32_t)s >= max_slots) return;
-  const uint32_t N = L * L * L;
-  const uint32_t cell = i % N;
-  const uint32_t x = cell % L, y = (cell / L) % L, z = cell / (L * L);
+// Coverage bitsets of the percolation test (is_percolating_cluster,
+// gpu_water3D.cu:2723-2780): for each cluster of >= minsize nodes (its slot),
+// the x, y and z values its cells take.  Lanes of one run share the slot and
+// the bitset words their coordinates fall in; the run ORs its bits together
+// (segmented butterfly over the wavefront) and its first lane makes three
+// atomics.
+__global__ __launch_bounds__(256) void k_perc_cover(const uint32_t* __restrict__ label, uint32_t nn, uint32_t L,
+                                                    const int32_t* __restrict__ size, int32_t minsize,
+                                                    const int32_t* __restrict__ slot, uint32_t* __restrict__ cover,
+                                                    uint32_t max_slots) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63u;
+  uint32_t key = 0xFFFFFFFFu, bx = 0, by = 0, bz = 0, x = 0, y = 0, z = 0;
+  if (i < nn) {
+    const uint32_t r = label[i];
+    const int32_t s = size[r] >= minsize ? slot[r] : -1;
+    if (s >= 0 && (uint32_t)s < max_slots) {
+      const uint32_t N = L * L * L;
+      const uint32_t cell = i % N;
+      x = cell % L;
+      y = (cell / L) % L;
+      z = cell / (L * L);
+      // slot < 3 L^2 + 1 < 2^18, word indices < 8 (L <= 252)
+      key = ((uint32_t)s << 9) | ((x >> 5) << 6) | ((y >> 5) << 3) | (z >> 5);
+      bx = 1u << (x & 31u);
+      by = 1u << (y & 31u);
+      bz = 1u << (z & 31u);
+    }
+  }
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || key != prev;
+  const uint32_t end = run_end(__ballot(head), lane);
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t ox = __shfl_down(bx, d, 64), oy = __shfl_down(by, d, 64), oz = __shfl_down(bz, d, 64);
+    if (lane + d < end) {
+      bx |= ox;
+      by |= oy;
+      bz |= oz;
+    }
+  }
+  if (!head || key == 0xFFFFFFFFu) return;
   const uint32_t nw = (L + 31) / 32;
-  uint32_t* cv = cover + (size_t)s * 3 * nw;
-  atomicOr(&cv[x / 32], 1u << (x % 32));
-  atomicOr(&cv[nw + y / 32], 1u << (y % 32));
-  atomicOr(&cv[2 * nw + z / 32], 1u << (z % 32));
+  uint32_t* cv = cover + (size_t)(key >> 9) * 3 * nw;
+  atomicOr(&cv[x >> 5], bx);
+  atomicOr(&cv[nw + (y >> 5)], by);
+  atomicOr(&cv[2 * nw + (z >> 5)], bz);
 }
 
 __global__ void k_perc_count(const uint32_t* __restrict__ cover, const uint32_t* __restrict__ nslot,
@@ -1126,6 +1064,10 @@ static thread_local hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
 void set_launch_timing(hipEvent_t start, hipEvent_t stop) {
   g_t0 = start;
   g_t1 = stop;
+}
+void get_launch_timing(hipEvent_t* start, hipEvent_t* stop) {
+  *start = g_t0;
+  *stop = g_t1;
 }
 
 template <typename K, typename... A>

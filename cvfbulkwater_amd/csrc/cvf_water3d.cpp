@@ -230,7 +230,12 @@ int main(int argc, char** argv) {
     fprintf(stderr, "Error: can't build chessboard.\n\tUse NX muliply of 4.\n");
     exit(1);
   }
-  check(cvf_create(&p, &sim), "cvf_create");
+  if (cvf_create(&p, &sim) < 0) {
+    // set-up errors carry the reference's own message (e.g. a Device_ID
+    // beyond the device count, gpu_water3D.cu:461-465, which exits with 1)
+    fprintf(stderr, "%s\n", cvf_last_error());
+    exit(1);
+  }
   if (in.flag_chessboard == 1) fprintf(log, "Starting Chessboard\n");
 
   if (in.flag_config == 1) {

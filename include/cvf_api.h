@@ -145,13 +145,18 @@ void cvf_destroy(cvf_sim *s);
  * not on threads).  No reference counterpart: the reference's launch shape
  * is fixed at N/512 blocks of 512 threads (gpu_water3D.cu:68, 517-518). */
 typedef struct cvf_tuning {
-  int32_t streams;      /* groups of state points, each on its own stream (1..8) */
+  int32_t streams;      /* groups of state points, each on its own stream (1..8); with
+                           fused = 2, resident workgroups per CU (1 or 2) */
   int32_t block_sigma;  /* workgroup size of the sigma half-sweep (64..1024, power of 2) */
   int32_t block_eta;    /* workgroup size of the eta cube sweep */
   int32_t block_cell;   /* workgroup size of the per-cell kernels (SW, observables) */
   int32_t graph_steps;  /* MC steps per replayed hipGraph (1..64) */
   int32_t fused;        /* 1: eta sweep + 6 half-sweeps per kernel on LDS-staged z-slabs
-                           where the slab fits in LDS; 0: one kernel per half-sweep */
+                           where the slab fits in LDS; 0: one kernel per half-sweep;
+                           2: the resident kernel (every step of a graph in one launch, the
+                           lattice held in LDS, one stream group) where the points allow it
+                           (reference schedule, one Metropolis sweep per step, chessboard
+                           rule), else as 1 */
   int32_t slab_planes;  /* planes per slab of the fused kernel (<= 16); 0 = automatic */
   int32_t block_chunk;  /* workgroup size of the fused kernel */
   int32_t chunk_sweeps; /* half-sweeps per fused launch: 6, or 12 (a whole Metropolis pass per
@@ -163,10 +168,13 @@ int cvf_get_tuning(cvf_sim *s, cvf_tuning *t);
 
 /* Launch plan of state point pt under the current tuning (no reference
  * counterpart; the reference has one launch shape): its stream group, the
- * planes per z-slab of its fused launches (0 when its group runs the
- * unfused kernels) and the sigma half-sweeps per fused launch of the
- * reference schedule (6 or 12; 0 when none run fused). */
-int cvf_get_plan(cvf_sim *s, int32_t pt, int32_t *group, int32_t *slab_planes, int32_t *chunk_sweeps);
+ * planes per z-slab of its fused or resident launches (0 when its group
+ * runs the unfused kernels), the sigma half-sweeps per fused launch of the
+ * reference schedule (6 or 12; 0 when none run fused) and the kind of
+ * Metropolis launch: 0 unfused kernels, 1 fused slab launches, 2 the
+ * resident kernel. */
+int cvf_get_plan(cvf_sim *s, int32_t pt, int32_t *group, int32_t *slab_planes, int32_t *chunk_sweeps,
+                 int32_t *kind);
 /* Rebuilds the groups, block tables and graphs; synchronises the handle. */
 int cvf_set_tuning(cvf_sim *s, const cvf_tuning *t);
 
@@ -188,7 +196,9 @@ enum {
   CVF_KCLASS_SW_UNION = 4,
   CVF_KCLASS_SW_FLIP = 5,
   CVF_KCLASS_CHUNK = 6,   /* fused slab kernel (eta sweep + 6 sigma half-sweeps) */
-  CVF_KCLASS_COUNT = 7
+  CVF_KCLASS_RESIDENT = 7, /* resident kernel: all profiled steps in one launch (one step per
+                              launch with Swendsen-Wang steps) */
+  CVF_KCLASS_COUNT = 8
 };
 
 /* Instrumented cvf_step for measurement (no reference counterpart; the

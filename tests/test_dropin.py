@@ -71,8 +71,42 @@ def test_missing_input_data(tmp_path):
     assert r.returncode == 1 and "Can't open input file" in r.stderr
 
 
+@pytest.mark.parametrize("flag_config", [3, -1])
+def test_bad_flag_config_exits_with_reference_message(tmp_path, flag_config):
+    """gpu_water3D.cu:827-832: the message and the flag's three meanings,
+    exit status 1 (checked before any device work)."""
+    write_input(tmp_path, flag_config=flag_config)
+    r = run(tmp_path, check=False)
+    assert r.returncode == 1
+    assert f"Error: Wrong value for flag_config = {flag_config}" in r.stderr
+    assert "Flag_config=2  Initial spins completely ordered (T=0)" in r.stderr
+
+
+@pytest.mark.parametrize("L", [30, 6])
+def test_lattice_not_multiple_of_4_exits_with_reference_message(tmp_path, L):
+    """gpu_water3D.cu:655-658: the chessboard needs L % 4 == 0."""
+    write_input(tmp_path, flag_config=0)
+    r = run(tmp_path, "-L", str(L), check=False)
+    assert r.returncode == 1
+    assert "Error: can't build chessboard.\n\tUse NX muliply of 4." in r.stderr
+
+
 # ------------------------------------------------------------------ GPU runs
 DATA_RE = re.compile(r"^\d+( -?\d\.\d{7}e[+-]\d\d){6}$")
+
+
+@pytest.mark.gpu
+def test_device_id_beyond_device_count_exits_with_reference_message(tmp_path):
+    """gpu_water3D.cu:460-465: a Device_ID outside [0, devices) prints the
+    reference's message and exits with 1."""
+    write_input(tmp_path, flag_config=0)
+    lines = (tmp_path / "input_data").read_text().splitlines()
+    lines[13] = "Device_ID                   57"
+    (tmp_path / "input_data").write_text("\n".join(lines) + "\n")
+    r = run(tmp_path, check=False)
+    assert r.returncode == 1
+    assert "Error: Tried to execute in GPU deive ID = 57" in r.stderr
+    assert "\tTotal number of devices is " in r.stderr
 
 
 @pytest.mark.gpu

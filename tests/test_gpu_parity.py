@@ -93,15 +93,44 @@ def test_cluster_form_labels_every_step(L, T):
     assert_same_obs(eng, ora)
 
 
-def test_cluster_form_L32_first_steps():
+def test_config3_L32_labels_every_step():
+    """Config 3 shape: L=32, T=0.70, P=0.45, 1 Metropolis + 1 Swendsen-Wang
+    step per MC step (gpu_water3D.cu:1444-1603, labels 2440-2529): sigma,
+    eta and the 6N canonical labels identical to the oracle after each of
+    the first 1000 steps, the cluster.out row every 50 steps."""
     eng, ora = pair(32, T=0.70, nc=1)
-    for t in range(20):
+    for t in range(1000):
+        eng.step(1)
+        ora.step(1)
+        la, lb = eng.labels(), ora.labels()
+        if not np.array_equal(la, lb):
+            bad = np.flatnonzero(la != lb)
+            raise AssertionError(f"labels step {t}: {bad.size} nodes differ, first {bad[:8]}")
+        assert_same_state(eng, ora, f"step {t}")
+        if t % 50 == 49:
+            a, b = eng.cluster_stats(), ora.cluster_stats()
+            assert (a.largest, a.second, a.n_perc, a.n_clusters, a.mean, a.var) == tuple(b), t
+    assert_same_obs(eng, ora, "end")
+    assert eng.volume() == ora.volume()
+
+
+def test_config5_L96_labels_first_steps():
+    """Config 5 shape: L=96, T=0.65, P=0.30, 1 Metropolis + 1 Swendsen-Wang
+    + 1 volume move, FP32-compensated vdW: the lattice (sigma, eta) and the
+    5.3 M labels identical to the oracle after each of the first 3 steps
+    (the volume chain does not enter the lattice rules)."""
+    kw = dict(L=96, T=0.65, P=0.30, seed=44648604, n_cluster=1)
+    eng = Simulation(Params(precision=FP32_COMPENSATED, volume_mode=VOLUME_DEVICE, **kw))
+    ora = O.OracleSim(**kw)
+    for t in range(3):
         eng.step(1)
         ora.step(1)
         assert np.array_equal(eng.labels(), ora.labels()), f"labels step {t}"
         assert_same_state(eng, ora, f"step {t}")
     a, b = eng.cluster_stats(), ora.cluster_stats()
     assert (a.largest, a.second, a.n_perc, a.n_clusters, a.mean, a.var) == tuple(b)
+    a, b = eng.observe(), ora.measure()
+    assert (a.nhb2, a.n_sigma) == (b.nhb2, b.n_sigma)
 
 
 def test_device_volume_chain_equals_host_chain():
