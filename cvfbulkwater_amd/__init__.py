@@ -23,7 +23,7 @@ import numpy as np
 
 __all__ = [
     "CVFError", "Params", "Obs", "ClusterStats", "Simulation", "load_library",
-    "FP64", "FP32_COMPENSATED", "VOLUME_HOST", "VOLUME_DEVICE", "vdw_energy",
+    "FP64", "FP32_COMPENSATED", "VOLUME_HOST", "VOLUME_DEVICE", "vdw_energy", "vdw_pair_energy",
     "LIB_PATH", "EXE_PATH", "THERMO_EXE_PATH", "Correlator", "KT_NOT_VALUE", "thermo_averages",
     "thermo_derivative", "thermo_kt", "REWEIGHT_EXE_PATH", "SCHED_REFERENCE", "SCHED_FAST",
 ]
@@ -173,6 +173,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cvf_set_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
         "cvf_get_plan": (C.c_int, [P, I32, I32P, I32P, I32P, I32P]),
         "cvf_vdw_energy_noalloc": (C.c_int, [I32, I32, F32P, I32P, I32, C.c_float, DP, I32, I32, DP]),
+        "cvf_vdw_pair_energy": (C.c_int, [I32, I32, C.c_float, DP, I32, I32, DP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -378,6 +379,17 @@ def vdw_energy(L: int, dist: np.ndarray, freq: np.ndarray, r_cutoff: float, r: n
                                    f.ctypes.data_as(C.POINTER(C.c_int32)), d.size, r_cutoff,
                                    rr.ctypes.data_as(C.POINTER(C.c_double)), rr.size, precision,
                                    out.ctypes.data_as(C.POINTER(C.c_double))), name)
+    return out
+
+
+def vdw_pair_energy(L: int, r_cutoff: float, r: np.ndarray, precision: int = FP64, device_id: int = 0) -> np.ndarray:
+    """E_vdW(r) as the direct minimum-image pair sum over the L^3 lattice on
+    the GPU (no shell table): FP64 or FP32-compensated."""
+    lib = load_library()
+    rr = np.ascontiguousarray(r, np.float64)
+    out = np.empty_like(rr)
+    _check(lib, lib.cvf_vdw_pair_energy(device_id, L, r_cutoff, rr.ctypes.data_as(C.POINTER(C.c_double)), rr.size,
+                                        precision, out.ctypes.data_as(C.POINTER(C.c_double))), "cvf_vdw_pair_energy")
     return out
 
 

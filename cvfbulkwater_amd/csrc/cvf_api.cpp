@@ -160,7 +160,8 @@ struct cvf_sim {
 
   DevBuf<uint32_t> words, words2, parent, cover, nslot_nperc;  // words2: ping-pong partner
   DevBuf<uint8_t> cov;
-  DevBuf<int32_t> csize, slot;
+  DevBuf<int32_t> csize, slot, cclusters;  // cluster sizes by label, percolation slots, compacted sizes
+  DevBuf<uint32_t> cblock;                  // per-workgroup cluster counts of the compaction, then the total
   DevBuf<PointDev> pts;
   DevBuf<PointState> pst;
   DevBuf<float> dist;
@@ -1188,6 +1189,8 @@ int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
   const uint32_t maxN = maxL * maxL * maxL;
   HIPC(s->csize.alloc(6 * (size_t)maxN));
   HIPC(s->slot.alloc(6 * (size_t)maxN));
+  HIPC(s->cclusters.alloc(6 * (size_t)maxN));
+  HIPC(s->cblock.alloc(compact_blocks(6 * maxN) + 1));
   s->max_slots = 3 * maxL * maxL + 1;  // a cluster of >= 2L nodes: at most 6N/(2L)
   HIPC(s->cover.alloc((size_t)s->max_slots * 3 * ((maxL + 31) / 32)));
   HIPC(s->nslot_nperc.alloc(2));
@@ -1552,13 +1555,18 @@ int cvf_cluster_stats(cvf_sim* s, int32_t pt, cvf_cstat* c) {
                     s->max_slots);
   launch_perc_cover(s->stream, label, nn, L, s->csize.p, minsize, s->slot.p, s->cover.p, s->max_slots);
   launch_perc_count(s->stream, s->cover.p, s->nslot_nperc.p, L, s->max_slots, s->nslot_nperc.p + 1);
-  std::vector<int32_t> size(nn);
-  uint32_t np[2];
-  HIPC(hipMemcpyAsync(size.data(), s->csize.p, (size_t)nn * 4, hipMemcpyDeviceToHost, s->stream));
+  // the clusters' sizes only, still in label order
+  const uint32_t nb = compact_blocks(nn);
+  launch_compact_sizes(s->stream, s->csize.p, nn, s->cblock.p, s->cblock.p + nb, s->cclusters.p);
+  uint32_t np[2], ncl = 0;
   HIPC(hipMemcpyAsync(np, s->nslot_nperc.p, 8, hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipMemcpyAsync(&ncl, s->cblock.p + nb, 4, hipMemcpyDeviceToHost, s->stream));
   HIPC(hipStreamSynchronize(s->stream));
   if (np[0] > s->max_slots) return fail(CVF_EDEVICE, "percolation slot overflow");
-  assemble_cluster_row(size.data(), nn, (int32_t)np[1], c);
+  if (ncl > nn) return fail(CVF_EDEVICE, "cluster compaction overflow");
+  std::vector<int32_t> size(ncl);
+  HIPC(hipMemcpy(size.data(), s->cclusters.p, (size_t)ncl * 4, hipMemcpyDeviceToHost));
+  assemble_cluster_row(size.data(), ncl, (int32_t)np[1], c);
   return CVF_OK;
 }
 
@@ -1948,6 +1956,29 @@ int cvf_vdw_energy(int32_t device_id, int32_t L, const float* dist, const int32_
   launch_vdw_eval(nullptr, d.p, f.p, nshell, r_cutoff, N, dr.p, n_r, precision, de.p);
   HIPC(hipGetLastError());
   HIPC(hipMemcpy(energy, de.p, n_r * 8, hipMemcpyDeviceToHost));
+  return CVF_OK;
+}
+
+int cvf_vdw_pair_energy(int32_t device_id, int32_t L, float r_cutoff, const double* r, int32_t n_r,
+                        int32_t precision, double* energy) {
+  if (L < 2 || L > kMaxL || n_r <= 0 || !r || !energy || !(r_cutoff > 0.f))
+    return fail(CVF_EINVAL, "bad arguments");
+  if (precision != CVF_FP64 && precision != CVF_FP32_COMPENSATED) return fail(CVF_EINVAL, "unknown precision");
+  HIPC(hipSetDevice(device_id));
+  const uint32_t nb = vdw_pair_blocks(L);
+  DevBuf<double> dr, part;
+  HIPC(dr.alloc(n_r));
+  HIPC(part.alloc((size_t)nb * n_r));
+  HIPC(hipMemcpy(dr.p, r, n_r * 8, hipMemcpyHostToDevice));
+  launch_vdw_pairs(nullptr, L, r_cutoff, dr.p, n_r, precision, part.p);
+  HIPC(hipGetLastError());
+  std::vector<double> h((size_t)nb * n_r);
+  HIPC(hipMemcpy(h.data(), part.p, h.size() * 8, hipMemcpyDeviceToHost));
+  for (int32_t k = 0; k < n_r; ++k) {
+    double t = 0;
+    for (uint32_t b = 0; b < nb; ++b) t += h[(size_t)k * nb + b];
+    energy[k] = 0.5 * t;  // every pair was counted from both ends
+  }
   return CVF_OK;
 }
 

@@ -102,6 +102,12 @@ void launch_sigma_hist_slot(hipStream_t st, const uint32_t* W, uint32_t N, unsig
                             const uint64_t* gstep, uint32_t step_off, const uint64_t* base, uint32_t span);
 // cluster statistics of one point (labels at `label`, 6N nodes)
 void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, int32_t* size);
+// the non-zero cluster sizes of size[0..nn) in label order into out[0..*total)
+// (bcount: compact_blocks(nn) scratch words)
+constexpr uint32_t kCompactSpan = 1024;
+uint32_t compact_blocks(uint32_t nn);
+void launch_compact_sizes(hipStream_t st, const int32_t* size, uint32_t nn, uint32_t* bcount, uint32_t* total,
+                          int32_t* out);
 void launch_perc_slots(hipStream_t st, const uint32_t* label, uint32_t nn, const int32_t* size,
                        int32_t minsize, int32_t* slot, uint32_t* nslot, uint32_t max_slots);
 void launch_perc_cover(hipStream_t st, const uint32_t* label, uint32_t nn, uint32_t L,
@@ -111,6 +117,10 @@ void launch_perc_count(hipStream_t st, const uint32_t* cover, const uint32_t* ns
                        uint32_t max_slots, uint32_t* nperc);
 void launch_vdw_eval(hipStream_t st, const float* dist, const int32_t* freq, int nshell,
                      float r_cutoff, int N, const double* r, int n_r, int precision, double* out);
+// vdW pair form (validation of the shell table): per-workgroup partial sums
+// over ordered pairs, vdw_pair_blocks(L) per radius, radius-major in `part`
+uint32_t vdw_pair_blocks(int L);
+void launch_vdw_pairs(hipStream_t st, int L, float r_cutoff, const double* r, int n_r, int precision, double* part);
 // the events set by set_launch_timing (nullptr when launches are not timed)
 void get_launch_timing(hipEvent_t* start, hipEvent_t* stop);
 

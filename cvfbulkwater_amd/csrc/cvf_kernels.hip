@@ -73,6 +73,39 @@ __device__ __forceinline__ void two_sum_pair(float& s, float& c, float s2, float
   c = c + c2 + err;
 }
 
+// One Lennard-Jones term g = (r d)^-12 - (r d)^-6 - shift in FP32 as an
+// unevaluated pair g_hi + g_lo, for r = r_hi + r_lo and shift = sh_hi +
+// sh_lo, built with fma-based error-free transformations: x = r d, x^6,
+// u = x^-6 and u (u - 1) - shift from their FP32 heads plus first-order
+// FP32 corrections.
+__device__ __forceinline__ void lj_term_fp32(float r_hi, float r_lo, float d, float sh_hi, float sh_lo, float& g_hi,
+                                             float& g_lo) {
+#pragma clang fp contract(off)
+  // x = r d as x_hi + x_lo
+  const float x_hi = r_hi * d;
+  const float x_lo = fmaf(r_hi, d, -x_hi) + r_lo * d;
+  // x^2, x^3, x^6 with first-order tails
+  const float x2 = x_hi * x_hi;
+  const float x2e = fmaf(x_hi, x_hi, -x2) + 2.f * x_hi * x_lo;
+  const float x3 = x2 * x_hi;
+  const float x3e = fmaf(x2, x_hi, -x3) + x2e * x_hi + x2 * x_lo;
+  const float x6 = x3 * x3;
+  const float x6e = fmaf(x3, x3, -x6) + 2.f * x3 * x3e;
+  // u = 1/x^6 = u0 + u1
+  const float u0 = 1.0f / x6;
+  const float u1 = (fmaf(-x6, u0, 1.0f) - x6e * u0) * u0;
+  // g = u (u - 1) - shift
+  const float v0 = u0 - 1.0f;
+  const float bp = v0 - u0;
+  const float ev = (u0 - (v0 - bp)) + (-1.0f - bp);
+  const float p0 = u0 * v0;
+  const float p0e = fmaf(u0, v0, -p0);
+  g_hi = p0 - sh_hi;
+  const float gb = g_hi - p0;
+  const float ge = (p0 - (g_hi - gb)) + (-sh_hi - gb);
+  g_lo = ge + p0e + u0 * ev + u1 * (u0 + v0) - sh_lo;
+}
+
 // Per-molecule vdW sum with FP32 arithmetic.  Rounding r*d_k to a float
 // alone would cost 6 x 6e-8 relative in (r d)^-6, so each term is carried as
 // an unevaluated float pair (hi + lo) built with fma-based error-free
@@ -93,29 +126,8 @@ __device__ double vdw_fp32_wave(double r, const float* dist, const int32_t* freq
   for (int k = lane; k < nshell; k += 64) {
     const float d = dist[k];
     if (r * (double)d < (double)r_cutoff) {
-      // x = r d as x_hi + x_lo
-      const float x_hi = r_hi * d;
-      const float x_lo = fmaf(r_hi, d, -x_hi) + r_lo * d;
-      // x^2, x^3, x^6 with first-order tails
-      const float x2 = x_hi * x_hi;
-      const float x2e = fmaf(x_hi, x_hi, -x2) + 2.f * x_hi * x_lo;
-      const float x3 = x2 * x_hi;
-      const float x3e = fmaf(x2, x_hi, -x3) + x2e * x_hi + x2 * x_lo;
-      const float x6 = x3 * x3;
-      const float x6e = fmaf(x3, x3, -x6) + 2.f * x3 * x3e;
-      // u = 1/x^6 = u0 + u1
-      const float u0 = 1.0f / x6;
-      const float u1 = (fmaf(-x6, u0, 1.0f) - x6e * u0) * u0;
-      // g = u (u - 1) - shift
-      const float v0 = u0 - 1.0f;
-      const float bp = v0 - u0;
-      const float ev = (u0 - (v0 - bp)) + (-1.0f - bp);
-      const float p0 = u0 * v0;
-      const float p0e = fmaf(u0, v0, -p0);
-      const float g_hi = p0 - sh_hi;
-      const float gb = g_hi - p0;
-      const float ge = (p0 - (g_hi - gb)) + (-sh_hi - gb);
-      const float g_lo = ge + p0e + u0 * ev + u1 * (u0 + v0) - sh_lo;
+      float g_hi, g_lo;
+      lj_term_fp32(r_hi, r_lo, d, sh_hi, sh_lo, g_hi, g_lo);
       // weight f_k / N = count_k / 2
       const float w = (float)(freq[k] / N);
       const float t_hi = w * g_hi;
@@ -963,6 +975,78 @@ __global__ __launch_bounds__(256) void k_cluster_sizes(const uint32_t* __restric
   if (head && i < nn) atomicAdd(&size[lab], (int32_t)(run_end(heads, lane) - lane));
 }
 
+// Order-preserving compaction of the cluster sizes (non-zero entries of
+// size[], in label order): per workgroup of kCompactSpan labels the count
+// of clusters, one workgroup's exclusive scan of those counts, then every
+// workgroup writes its clusters at its offset.  The host then assembles the
+// cluster.out row from the clusters alone, in the same order.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* sh, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) sh[w] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    before += k < w ? sh[k] : 0u;
+    all += sh[k];
+  }
+  *total = all;
+  return before + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_compact_count(const int32_t* __restrict__ size, uint32_t nn,
+                                                       uint32_t* __restrict__ bcount) {
+  __shared__ uint32_t sh[4];
+  const uint32_t base = blockIdx.x * kCompactSpan + threadIdx.x * 4;
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) c += base + k < nn && size[base + k] != 0;
+  uint32_t total;
+  block_excl_scan256(c, sh, &total);
+  if (threadIdx.x == 0) bcount[blockIdx.x] = total;
+}
+
+// one workgroup: bcount[0..nb) -> exclusive offsets, the sum into *total
+__global__ __launch_bounds__(256) void k_compact_scan(uint32_t* __restrict__ bcount, uint32_t nb,
+                                                      uint32_t* __restrict__ total) {
+  __shared__ uint32_t sh[4];
+  uint32_t run = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
+    const uint32_t i = b0 + threadIdx.x;
+    const uint32_t v = i < nb ? bcount[i] : 0u;
+    uint32_t sum;
+    const uint32_t ex = block_excl_scan256(v, sh, &sum);
+    if (i < nb) bcount[i] = run + ex;
+    run += sum;
+    __syncthreads();  // sh is reused by the next round
+  }
+  if (threadIdx.x == 0) *total = run;
+}
+
+__global__ __launch_bounds__(256) void k_compact_write(const int32_t* __restrict__ size, uint32_t nn,
+                                                       const uint32_t* __restrict__ boff, int32_t* __restrict__ out) {
+  __shared__ uint32_t sh[4];
+  const uint32_t base = blockIdx.x * kCompactSpan + threadIdx.x * 4;
+  int32_t v[4];
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    v[k] = base + k < nn ? size[base + k] : 0;
+    c += v[k] != 0;
+  }
+  uint32_t total;
+  uint32_t pos = boff[blockIdx.x] + block_excl_scan256(c, sh, &total);
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k)
+    if (v[k]) out[pos++] = v[k];
+}
+
 __global__ void k_perc_slots(const uint32_t* __restrict__ label, uint32_t nn,
                              const int32_t* __restrict__ size, int32_t minsize,
                              int32_t* __restrict__ slot, uint32_t* __restrict__ nslot,
@@ -1053,6 +1137,88 @@ __global__ __launch_bounds__(64) void k_vdw_eval(const float* __restrict__ dist,
   else
     e = vdw_energy_fp64(rr, r_cutoff, dist, freq, nshell);
   if (threadIdx.x == 0) out[blockIdx.x] = e;
+}
+
+// ------------------------------------------------ vdW pair-form validation
+// The pair form of calculate_energy_vdW (gpu_water3D.cu:1784-1805): every
+// molecule i against every other molecule j whose minimum-image distance
+// (distance_PBC, 1323-1337: float of the root of the folded integer
+// displacement) is below R_cutoff (the shell table's filter, 886) and
+// scaled by r below R_cutoff (the energy's filter).  No shell table is
+// used: the sum runs over pairs from the lattice coordinates.  One thread
+// per molecule i over the offsets of the window [-R, R]^3 (R = floor
+// R_cutoff, clipped to the lattice's periodic range so that every j
+// appears once); each workgroup writes its partial sum (FP64, or the FP32
+// compensated pair hi, lo) for the radius of grid row blockIdx.y.
+__device__ __forceinline__ int fold_pbc(int a, int L) {
+  a = a < 0 ? -a : a;
+  return a > L / 2 ? L - a : a;
+}
+
+__global__ __launch_bounds__(256) void k_vdw_pairs(int L, int R, float r_cutoff, const double* __restrict__ rr,
+                                                   int precision, double* __restrict__ part) {
+#pragma clang fp contract(off)
+  const int N = L * L * L;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const double r = rr[blockIdx.y];
+  const double rc6 = 1.0 / ((double)r_cutoff * r_cutoff * r_cutoff * r_cutoff * r_cutoff * r_cutoff);
+  double tmp = 1.0 / (double)r_cutoff;
+  tmp = tmp * tmp * tmp;
+  const double att0 = tmp * tmp;
+  const double shift = att0 * att0 - att0;
+  const double shd = rc6 * rc6 - rc6;
+  const float sh_hi = (float)shd, sh_lo = (float)(shd - (double)sh_hi);
+  const float r_hi = (float)r, r_lo = (float)(r - (double)r_hi);
+  double sd = 0;
+  float s = 0.f, c = 0.f;
+  if (i < N) {
+    const int x1 = i % L, y1 = (i / L) % L, z1 = i / (L * L);
+    // offsets cover each j once: (-L/2, L/2] along an axis of the lattice
+    const int lo = -min(R, L / 2 - 1), hi = min(R, L / 2);
+    for (int oz = lo; oz <= hi; ++oz)
+      for (int oy = lo; oy <= hi; ++oy)
+        for (int ox = lo; ox <= hi; ++ox) {
+          if (!(ox | oy | oz)) continue;
+          const int x2 = (x1 + ox + L) % L, y2 = (y1 + oy + L) % L, z2 = (z1 + oz + L) % L;
+          const int dx = fold_pbc(x1 - x2, L), dy = fold_pbc(y1 - y2, L), dz = fold_pbc(z1 - z2, L);
+          const float d = (float)sqrt((double)(dx * dx + dy * dy + dz * dz));
+          if (!(d < r_cutoff) || !(r * (double)d < (double)r_cutoff)) continue;
+          if (precision == 1) {
+            float g_hi, g_lo;
+            lj_term_fp32(r_hi, r_lo, d, sh_hi, sh_lo, g_hi, g_lo);
+            two_sum_acc(s, c, g_hi);
+            c += g_lo;
+          } else {
+            double t = 1.0 / (r * d);
+            t = t * t * t;
+            const double att = t * t;
+            sd += att * att - att - shift;
+          }
+        }
+  }
+  __shared__ double ps[4];
+  __shared__ float pf[4][2];
+  if (precision == 1) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      const float s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(c, m, 64);
+      two_sum_pair(s, c, s2, c2);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      pf[threadIdx.x >> 6][0] = s;
+      pf[threadIdx.x >> 6][1] = c;
+    }
+  } else {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) sd += __shfl_xor(sd, m, 64);
+    if ((threadIdx.x & 63) == 0) ps[threadIdx.x >> 6] = sd;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < 4; ++w) t += precision == 1 ? (double)pf[w][0] + (double)pf[w][1] : ps[w];
+    part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+  }
 }
 
 // ------------------------------------------------------------ launchers
@@ -1162,6 +1328,14 @@ void launch_sigma_hist_slot(hipStream_t st, const uint32_t* W, uint32_t N, unsig
 void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, int32_t* size) {
   launch(k_cluster_sizes, grid1(nn, 256), dim3(256), st, label, nn, size);
 }
+uint32_t compact_blocks(uint32_t nn) { return (nn + kCompactSpan - 1) / kCompactSpan; }
+void launch_compact_sizes(hipStream_t st, const int32_t* size, uint32_t nn, uint32_t* bcount, uint32_t* total,
+                          int32_t* out) {
+  const uint32_t nb = compact_blocks(nn);
+  launch(k_compact_count, dim3(nb), dim3(256), st, size, nn, bcount);
+  launch(k_compact_scan, dim3(1), dim3(256), st, bcount, nb, total);
+  launch(k_compact_write, dim3(nb), dim3(256), st, size, nn, (const uint32_t*)bcount, out);
+}
 void launch_perc_slots(hipStream_t st, const uint32_t* label, uint32_t nn, const int32_t* size,
                        int32_t minsize, int32_t* slot, uint32_t* nslot, uint32_t max_slots) {
   launch(k_perc_slots, grid1(nn, 256), dim3(256), st, label, nn, size, minsize, slot, nslot, max_slots);
@@ -1179,6 +1353,10 @@ void launch_perc_count(hipStream_t st, const uint32_t* cover, const uint32_t* ns
 void launch_vdw_eval(hipStream_t st, const float* dist, const int32_t* freq, int nshell,
                      float r_cutoff, int N, const double* r, int n_r, int precision, double* out) {
   launch(k_vdw_eval, dim3(n_r), dim3(64), st, dist, freq, nshell, r_cutoff, N, r, precision, out);
+}
+uint32_t vdw_pair_blocks(int L) { return (uint32_t)(((size_t)L * L * L + 255) / 256); }
+void launch_vdw_pairs(hipStream_t st, int L, float r_cutoff, const double* r, int n_r, int precision, double* part) {
+  launch(k_vdw_pairs, dim3(vdw_pair_blocks(L), n_r), dim3(256), st, L, (int)r_cutoff, r_cutoff, r, precision, part);
 }
 
 }  // namespace cvf

@@ -239,6 +239,17 @@ int cvf_vdw_energy(int32_t device_id, int32_t L, const float *dist, const int32_
                    int32_t nshell, float r_cutoff, const double *r, int32_t n_r,
                    int32_t precision, double *energy);
 
+/* Pair form of E_vdW, for validating the shell table (calculate_energy_vdW,
+ * gpu_water3D.cu:1784-1805, with distance_PBC, 1323-1337): the sum over all
+ * minimum-image pairs of the L^3 lattice whose float distance d is below
+ * R_cutoff and r*d below R_cutoff, evaluated on the device from the lattice
+ * coordinates (no shell table): FP64, or FP32 terms with compensated sums.
+ * Equal to the shell form up to rounding where every distance class has an
+ * even count (L >= 12); at L = 8 the shell form drops half a pair of each
+ * odd class (gpu_water3D.cu:912-915). */
+int cvf_vdw_pair_energy(int32_t device_id, int32_t L, float r_cutoff, const double *r, int32_t n_r,
+                        int32_t precision, double *energy);
+
 /* Allocation-free form of cvf_vdw_energy: the same kernel and results, with
  * the shell table, radii and energies staged in a fixed device array that
  * each GPU holds until the process ends instead of memory allocated per

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from cvfbulkwater_amd import (FP32_COMPENSATED, FP64, VOLUME_DEVICE, VOLUME_HOST, Params, Simulation,
-                              vdw_energy)
+                              vdw_energy, vdw_pair_energy)
 from oracle import py as O
 
 pytestmark = pytest.mark.gpu
@@ -123,6 +123,53 @@ def test_fp32_compensated_vdw_within_1e6(L):
     assert np.array_equal(e64, ref)  # device FP64 == host FP64, bit for bit
     rel = np.abs(e32 - e64) / np.abs(e64)
     assert rel.max() <= 1e-6, rel.max()
+
+
+@pytest.mark.parametrize("L", [12, 32, 96])
+def test_vdw_pair_form_equals_shell_form(L):
+    """The GPU pair form (every minimum-image pair of the lattice, no shell
+    table; calculate_energy_vdW + distance_PBC, gpu_water3D.cu:1784-1805,
+    1323-1337) equals the shell form: FP64 within 1e-12, FP32-compensated
+    within 1e-6 relative, for r below 1 (where the table's d < R_cutoff
+    filter binds) and above."""
+    d, f = O.shell_table(L, 6.0)
+    r = np.array([0.90, 0.95, 0.999, 1.0001, 1.05, 1.12, 1.2, 1.3])
+    shell = np.array([O.vdw_energy(x, 6.0, d, f) for x in r])
+    p64 = vdw_pair_energy(L, 6.0, r, FP64)
+    p32 = vdw_pair_energy(L, 6.0, r, FP32_COMPENSATED)
+    rel64 = np.abs(p64 - shell) / np.abs(shell)
+    rel32 = np.abs(p32 - shell) / np.abs(shell)
+    print(L, rel64.max(), rel32.max())
+    assert rel64.max() <= 1e-12, rel64
+    assert rel32.max() <= 1e-6, rel32
+
+
+def test_vdw_pair_form_L8_keeps_the_odd_classes():
+    """At L=8 the shell table halves odd class counts with integer division
+    (gpu_water3D.cu:912-915): the pair form exceeds it by N/2 times the
+    terms of the odd classes (126,976 pairs in the table, 127,488 pairs)."""
+    L, N, rc = 8, 512, 6.0
+    d, f = O.shell_table(L, rc)
+    o = np.arange(-3, 5)
+    X, Y, Z = np.meshgrid(o, o, o, indexing="ij")
+    dist = np.sqrt((X ** 2 + Y ** 2 + Z ** 2).astype(np.float64)).astype(np.float32).ravel()
+    dist = dist[(dist > 0) & (dist < rc)]
+    cls, cnt = np.unique(dist, return_counts=True)
+    assert (f // N).sum() * N == 126976 and cnt.sum() * N // 2 == 127488
+    r = np.array([0.95, 1.0001, 1.1])
+    shell = np.array([O.vdw_energy(x, rc, d, f) for x in r])
+    pair = vdw_pair_energy(L, rc, r, FP64)
+
+    def g(x):
+        t = (1.0 / x) ** 3
+        a = t * t
+        t0 = (1.0 / rc) ** 3
+        return a * a - a - (t0 ** 4 - t0 ** 2)
+
+    for k, x in enumerate(r):
+        keep = (cnt % 2 == 1) & (x * cls.astype(np.float64) < rc)
+        want = 0.5 * N * g(x * cls[keep].astype(np.float64)).sum()
+        assert abs((pair[k] - shell[k]) - want) <= 1e-9 * abs(shell[k]), (x, pair[k] - shell[k], want)
 
 
 def test_precision_form_keeps_lattice_trajectory():

@@ -276,3 +276,103 @@ def test_order_parameter_axis_is_the_sine_stretch_of_a_uniform_grid():
     # the plain (unstretched) map does not fit
     lin = np.polyfit(i, z, 1)
     assert np.max(np.abs(np.polyval(lin, i) - z)) > 1e-3
+
+
+# ------------------------------------------------------------ cut-offs, closed forms
+def two_sim_grid(hist_cells, nb=4, T=250.0, P=1.0):
+    """Two simulations at the same (T, P) holding the same normalized cell
+    values; reweighted at that (T, P) every cell's arguments reduce to
+    arg_j = log_n_j - ctes_j (f90:1045), the same for all cells."""
+    datos = np.zeros((2, nb, nb))
+    for (i, w), v in hist_cells.items():
+        datos[:, i, w] = v
+    return rw.OraGrid.from_datos(8, datos, -1.0, 950.0, 0.01, 2.0, [T, T], [P, P], np.zeros(2))
+
+
+@pytest.mark.parametrize("ctes,xx", [
+    # |arg| <= 10 everywhere: the plain sum over both simulations
+    ((0.0, 10.0), lambda: np.log1p(np.exp(-10.0))),
+    ((-5.0, 4.0), lambda: np.log(np.exp(5.0) + np.exp(-4.0))),
+    # some |arg| > 10: terms within 10 of the largest kept, the rest dropped
+    ((0.0, 15.0), lambda: 0.0),
+    ((0.0, 10.0 + 1e-9), lambda: 0.0),
+    ((-12.0, -3.0), lambda: 12.0 + np.log1p(np.exp(-9.0))),
+    ((-12.0, -1.5), lambda: 12.0),
+])
+def test_calcula_switch_to_max_shifted_sum(ctes, xx):
+    """f90:1047-1078: with any |arg_j| > 10 the cell sum keeps only the terms
+    within 10 of the largest (strict <), otherwise it sums all terms; log_P =
+    log(hist) - xx.  Dropping the switch or the cut, or making either test
+    non-strict, changes log_P in one of these cells."""
+    hist = {(0, 0): 0.3, (1, 2): 0.7, (3, 3): 2.0}
+    g = two_sim_grid(hist)
+    lp, memo, _ = g.calcula(np.array(ctes), 250.0, 1.0)
+    for (i, w), v in hist.items():
+        assert memo[i, w] == 0
+        assert lp[i, w] == pytest.approx(np.log(2 * v) - xx(), rel=0, abs=1e-14), (ctes, i, w)
+    assert memo.sum() == 16 - len(hist)
+
+
+@pytest.mark.parametrize("depths,kept", [
+    ((9.0, 11.0), (9.0,)),          # a cell 11 below the maximum is dropped
+    ((9.999, 10.0 + 1e-6), (9.999,)),  # strict < 10
+    ((3.0, 7.0), (3.0, 7.0)),
+])
+def test_log_z_drops_cells_ten_below_the_maximum(depths, kept):
+    """f90:1106-1118: log_Z sums exp(log_P - max) over the non-empty cells
+    less than 10 below the maximum, then adds the maximum."""
+    hist = {(0, 1): 1.0}
+    for k, d in enumerate(depths):
+        hist[(2, k)] = float(np.exp(-d))
+    g = two_sim_grid(hist)
+    lp, memo, lz = g.calcula(np.zeros(2), 250.0, 1.0)
+    top = lp[0, 1]
+    assert all(lp[2, k] - top == pytest.approx(-d, abs=1e-12) for k, d in enumerate(depths))
+    want = top + np.log1p(sum(np.exp(-d) for d in kept))
+    assert lz == pytest.approx(want, rel=0, abs=1e-14)
+    every = top + np.log1p(sum(np.exp(-d) for d in depths))
+    if kept != depths:
+        assert abs(lz - every) > 1e-7
+
+
+def test_error_bars_of_a_two_point_distribution():
+    """Err_Calcula_Histograma_Rotado (f90:1622-2039) for one simulation whose
+    samples sit at two grid-cell centres, reweighted to its own point, at
+    alpha = 0 (x = rho).  Both histograms (the reweighted grid projected on
+    n_bin_mixed = 7 bins, the samples on n_bin = 49 bins, each over its range
+    widened by two empty bins a side) are two equal spikes; standardized
+    (mean 0, norm and variance 1) they sit at x = -1 and +1 with height
+    (k2 - k1) / 4, where k1 = int(2 n / (n + 4)) and k2 = int(n (n + 2) /
+    (n + 4)) are the spikes' bins.  Each reweighted bin i is matched to the
+    nearest sample bin: error_X = that distance, error_hist = the height
+    difference."""
+    nb, N = 49, 8
+    Emin, Dmin, dE, dD = -1.0, 950.0, 0.01, 2.0
+    datos = np.zeros((1, nb, nb))
+    cells = [(10, 5), (10, 30)]
+    for i, w in cells:
+        datos[0, i, w] = 1.0
+    g = rw.OraGrid.from_datos(N, datos, Emin, Dmin, dE, dD, [250.0], [1.0], np.zeros(1))
+    grid = g.distribution([0.0], 250.0, 1.0)
+    assert np.count_nonzero(grid) == 2
+    centre = lambda i, w: (Emin + (i + 0.5) * dE, Dmin + (w + 0.5) * dD)  # noqa: E731
+    samples = np.array([centre(i, w) for i, w in cells for _ in range(500)])
+    eh, ex, rc = g.errors(grid, 0.0, samples.ravel())
+    assert rc == 0
+
+    def spikes(n):
+        k1, k2 = int(2 * n / (n + 4)), int(n * (n + 2) / (n + 4))
+        t = 2.0 / (k2 - k1)  # standardized bin width
+        x = -1.0 + (np.arange(n) - k1) * t
+        h = np.zeros(n)
+        h[[k1, k2]] = (k2 - k1) / 4.0
+        return x, h
+
+    xa, ha = spikes(7)
+    xb, hb = spikes(nb)
+    j = np.array([np.argmin(np.abs(xb - v)) for v in xa])
+    np.testing.assert_allclose(ex, np.abs(xb[j] - xa), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(eh, np.abs(hb[j] - ha), rtol=0, atol=1e-12)
+    # the spikes: exact positions, heights 46/4 against 4/4
+    assert eh[1] == pytest.approx(10.5, abs=1e-12) and eh[5] == pytest.approx(10.5, abs=1e-12)
+    assert ex[1] == pytest.approx(0.0, abs=1e-12) and ex[3] == pytest.approx(0.0, abs=1e-12)
