@@ -400,120 +400,208 @@ int download_point_words(cvf_sim* s, int pt, std::vector<uint32_t>& w) {
 }
 
 // ------------------------------------------------- resident kernel plan
-uint32_t window_words(uint32_t L, uint32_t T) { return (T == L ? T : T + 2) * L * L; }
+// LDS words of a slab's window: T planes plus a ghost plane on each side when
+// the point is split, the whole periodic lattice otherwise.
+uint32_t window_words(uint32_t L, uint32_t T, bool whole) { return (whole ? T : T + 2) * L * L; }
 
 struct RDeal {
   std::vector<RSlab> slabs;   // in workgroup order
   std::vector<RGroupEnt> groups;
   uint32_t lds_bytes = 0;
   size_t ring_words = 0;
-  double makespan = 0;
+  double makespan = 0;  // largest cells per wave
 };
 
-// Deal the slabs of `points` (T planes each, T per lattice size in Tof) to
-// at most nwg workgroups of at most cap LDS words, longest first onto the
-// least loaded workgroup with room.  Cost of a slab: its cells plus half a
-// plane for the boundary work (crossing cubes, ring copies) when the
-// lattice is split.
-bool deal_slabs(const std::vector<PointHost>& ph, const std::vector<int>& points,
-                const std::vector<std::pair<uint32_t, uint32_t>>& Tof, int nwg, uint32_t cap, RDeal& out) {
-  struct Sl {
-    int point;
-    uint32_t z0, T, L;
-    double cost;
-    uint32_t words;
-  };
-  std::vector<Sl> sl;
-  for (int p : points) {
-    const uint32_t L = ph[p].L;
-    uint32_t T = 0;
-    for (auto& lt : Tof)
-      if (lt.first == L) T = lt.second;
-    if (!T) return false;
-    for (uint32_t z0 = 0; z0 < L; z0 += T)
-      sl.push_back(Sl{p, z0, T, L, (double)T * L * L + (T < L ? 0.5 * L * L : 0.0), window_words(L, T)});
-  }
-  std::stable_sort(sl.begin(), sl.end(), [](const Sl& a, const Sl& b) { return a.cost > b.cost; });
-  std::vector<double> cost(nwg, 0.0);
-  std::vector<uint32_t> words(nwg, 0), cnt(nwg, 0);
-  std::vector<int> wg(sl.size(), -1);
-  // least loaded first; ties to the lower index
-  using E = std::pair<double, int>;
-  std::priority_queue<E, std::vector<E>, std::greater<E>> heap;
-  for (int w = 0; w < nwg; ++w) heap.push(E(0.0, w));
-  std::vector<E> held;
-  for (size_t i = 0; i < sl.size(); ++i) {
-    int best = -1;
-    held.clear();
-    while (!heap.empty()) {
-      const E e = heap.top();
-      heap.pop();
-      if (words[e.second] + sl[i].words <= cap && cnt[e.second] < kMaxRSlabs) {
-        best = e.second;
-        break;
+// Waves of a workgroup for slabs of the given cells: one each, the rest by
+// largest remainder in proportion to the cells.  False if there are more
+// slabs than waves.
+bool split_waves(const std::vector<double>& cells, uint32_t wpg, std::vector<uint32_t>& nw) {
+  const size_t n = cells.size();
+  if (n == 0 || n > wpg) return false;
+  double sum = 0;
+  for (double c : cells) sum += c;
+  nw.assign(n, 1);
+  for (uint32_t left = wpg - (uint32_t)n; left > 0; --left) {
+    size_t best = 0;
+    double bd = -1e300;
+    for (size_t j = 0; j < n; ++j) {
+      const double d = (double)wpg * cells[j] / sum - (double)nw[j];
+      if (d > bd) {
+        bd = d;
+        best = j;
       }
-      held.push_back(e);
     }
-    for (const E& e : held) heap.push(e);
-    if (best < 0) return false;
-    wg[i] = best;
-    cost[best] += sl[i].cost;
-    words[best] += sl[i].words;
-    ++cnt[best];
-    heap.push(E(cost[best], best));
+    ++nw[best];
   }
-  // slabs in workgroup order, windows packed in each workgroup's LDS
+  return true;
+}
+
+struct Piece {
+  int point;
+  uint32_t z0, T, wg;
+};
+
+// Fill workgroups in order with z-pieces of the points (in `order`): a
+// workgroup takes the thickest piece that keeps its largest cells per wave
+// within W, its windows within cap words and its slabs within max_slabs;
+// pieces are at least min_T planes, and so is what a cut leaves of the
+// point.  False if the points do not fit in nwg workgroups.
+bool fill_workgroups(const cvf_sim* s, const std::vector<int>& order, double W, int nwg, uint32_t wpg, uint32_t cap,
+                     uint32_t min_T, uint32_t max_T, std::vector<Piece>& pc) {
+  pc.clear();
+  const uint32_t max_slabs = std::min<uint32_t>(kMaxRSlabs, wpg);
+  uint32_t wg = 0, nsl = 0, used = 0;
+  std::vector<double> cells;
+  std::vector<uint32_t> nw;
+  for (int p : order) {
+    const uint32_t L = s->ph[p].L, LL = L * L, tcap = max_T ? std::min(max_T, L) : L;
+    uint32_t z = 0;
+    while (z < L) {
+      const uint32_t rest = L - z;
+      uint32_t take = 0;
+      if (nsl < max_slabs)
+        for (uint32_t t = std::min(rest, tcap); t >= 1; --t) {
+          if (t < rest && (t < min_T || rest - t < min_T)) continue;
+          const uint32_t need = window_words(L, t, t == L);
+          if (used + need > cap) continue;
+          cells.push_back((double)t * LL);
+          const bool ok = split_waves(cells, wpg, nw);
+          double worst = 0;
+          for (size_t j = 0; ok && j < cells.size(); ++j) worst = std::max(worst, cells[j] / nw[j]);
+          cells.pop_back();
+          if (ok && worst <= W) {
+            take = t;
+            break;
+          }
+        }
+      if (take == 0) {
+        // the next workgroup; an empty one that takes nothing cannot help
+        if (nsl == 0 || wg + 1 == (uint32_t)nwg) return false;
+        ++wg;
+        nsl = used = 0;
+        cells.clear();
+        continue;
+      }
+      pc.push_back(Piece{p, z, take, wg});
+      cells.push_back((double)take * LL);
+      used += window_words(L, take, take == L);
+      ++nsl;
+      z += take;
+    }
+  }
+  return true;
+}
+
+// The resident plan of a group on at most nwg workgroups of wpg waves and
+// cap LDS words each: the points, largest lattice first, cut into z-pieces
+// that fill the workgroups in order (fill_workgroups), with the smallest
+// bound on any slab's cells per wave (its cells over its waves, which set
+// the length of every phase) for which they fit, found by bisection.  A
+// point in one piece is a whole-lattice window; the pieces of a split point
+// are slabs of at least two planes (one with max_T = 1) and at most max_T
+// planes when max_T > 0.  Each workgroup's waves go to its slabs in
+// proportion to their cells, at least one each.  False when the group's
+// points cannot run resident (fast schedule, several Metropolis sweeps per
+// step, the non-chessboard rule) or do not fit.
+bool plan_resident(const cvf_sim* s, const Group& g, int nwg, uint32_t wpg, uint32_t cap, uint32_t max_T,
+                   RDeal& out) {
+  if (s->sched != CVF_SCHED_REFERENCE || s->nm != 1 || nwg <= 0 || wpg == 0) return false;
+  std::vector<int> order = g.points;
+  double total = 0, biggest = 0;
+  for (int p : order) {
+    if (!s->ph[p].prm.flag_chessboard) return false;
+    total += (double)s->ph[p].N;
+    biggest = std::max(biggest, (double)s->ph[p].N);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return s->ph[a].L > s->ph[b].L; });
+  const uint32_t min_T = max_T == 1 ? 1u : 2u;
+  std::vector<Piece> pc, trial;
+  double lo = 0.99 * total / ((double)nwg * wpg), hi = biggest;
+  if (!fill_workgroups(s, order, hi, nwg, wpg, cap, min_T, max_T, pc)) return false;
+  for (int it = 0; it < 30 && hi - lo > 0.002 * hi; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (fill_workgroups(s, order, mid, nwg, wpg, cap, min_T, max_T, trial)) {
+      hi = mid;
+      pc.swap(trial);
+    } else {
+      lo = mid;
+    }
+  }
+  const uint32_t wg = pc.empty() ? 0 : pc.back().wg;
   out = RDeal();
-  std::vector<std::vector<size_t>> members(nwg);
-  for (size_t i = 0; i < sl.size(); ++i) members[wg[i]].push_back(i);
+  std::vector<std::vector<size_t>> members(wg + 1);
+  for (size_t i = 0; i < pc.size(); ++i) members[pc[i].wg].push_back(i);
+  std::vector<uint32_t> pieces_of(s->ph.size(), 0);
+  for (const Piece& x : pc) ++pieces_of[x.point];
   size_t ring = 0;
-  for (int w = 0; w < nwg; ++w) {
-    if (members[w].empty()) continue;
-    RGroupEnt ge{(uint32_t)out.slabs.size(), (uint32_t)members[w].size()};
-    uint32_t lds = 0;
-    for (size_t i : members[w]) {
-      const Sl& x = sl[i];
-      const PointHost& P = ph[x.point];
+  for (uint32_t w = 0; w <= wg; ++w) {
+    const auto& m = members[w];
+    if (m.empty()) continue;
+    // waves in proportion to cells (largest remainders), at least one each
+    std::vector<double> cw;
+    for (size_t i : m) cw.push_back((double)pc[i].T * s->ph[pc[i].point].L * s->ph[pc[i].point].L);
+    std::vector<uint32_t> nw;
+    if (!split_waves(cw, wpg, nw)) return false;
+    RGroupEnt ge{(uint32_t)out.slabs.size(), (uint32_t)m.size()};
+    uint32_t lds = 0, w0 = 0;
+    for (size_t j = 0; j < m.size(); ++j) {
+      const Piece& x = pc[m[j]];
+      const PointHost& P = s->ph[x.point];
+      const uint32_t L = P.L;
+      const bool whole = pieces_of[x.point] == 1;
       RSlab r;
       memset(&r, 0, sizeof r);
       r.point = x.point;
-      r.L = x.L;
+      r.L = L;
       r.z0 = x.z0;
       r.T = x.T;
-      r.ns = x.L / x.T;
+      r.ns = pieces_of[x.point];
       r.woff = P.woff;
       r.lds = lds;
-      lds += x.words;
+      lds += window_words(L, x.T, whole);
       r.seed_lo = (uint32_t)P.prm.seed;
       r.seed_hi = (uint32_t)(P.prm.seed >> 32);
-      r.mH = make_fastdiv(x.L / 2).m;
-      r.mQ = make_fastdiv(x.L / 4).m;
-      r.mP8 = make_fastdiv(x.L * x.L / 8).m;
-      r.mP4 = make_fastdiv(x.L * x.L / 4).m;
-      r.mQQ = make_fastdiv((x.L / 4) * (x.L / 4)).m;
+      r.mH = make_fastdiv(L / 2).m;
+      r.mQ = make_fastdiv(L / 4).m;
+      r.mP8 = make_fastdiv(L * L / 8).m;
+      r.mP4 = make_fastdiv(L * L / 4).m;
+      r.mQQ = make_fastdiv((L / 4) * (L / 4)).m;
       r.above = r.below = -1;
-      if (r.ns > 1) {
+      r.w0 = w0;
+      r.nw = nw[j];
+      w0 += nw[j];
+      const uint32_t stride = 64 * r.nw, Q8 = L * L / 8, Q4 = L * L / 4, Qc = (L / 4) * (L / 4);
+      r.dp8 = stride / Q8;
+      r.du8 = stride % Q8;
+      r.dp4 = stride / Q4;
+      r.du4 = stride % Q4;
+      r.dpc = stride / Qc;
+      r.duc = stride % Qc;
+      if (!whole) {
         r.ring = (uint32_t)ring;
-        ring += 4 * (size_t)kRingHalf * x.L * x.L;
+        ring += 4 * (size_t)kRingHalf * L * L;
       }
       out.slabs.push_back(r);
+      out.makespan = std::max(out.makespan, (double)x.T * L * L / nw[j]);
     }
+    if (lds > cap) return false;
     out.lds_bytes = std::max(out.lds_bytes, lds * 4);
     out.groups.push_back(ge);
-    out.makespan = std::max(out.makespan, cost[w]);
   }
-  if (ring >= 0xFFFFFFFFull) return false;
+  if (ring >= 0xFFFFFFFFull || out.groups.size() > (size_t)nwg) return false;
   out.ring_words = ring;
   // neighbours: the slabs of the same point above and below
-  std::vector<std::vector<int32_t>> by_point(ph.size());
+  std::vector<std::vector<int32_t>> by_point(s->ph.size());
   for (size_t j = 0; j < out.slabs.size(); ++j) by_point[out.slabs[j].point].push_back((int32_t)j);
   for (auto& r : out.slabs) {
     if (r.ns == 1) continue;
-    const uint32_t up = (r.z0 + r.T) % r.L, dn = (r.z0 + r.L - r.T) % r.L;
+    const uint32_t up = (r.z0 + r.T) % r.L;
     for (int32_t j : by_point[r.point]) {
-      if (out.slabs[j].z0 == up) r.above = j;
-      if (out.slabs[j].z0 == dn) r.below = j;
+      const RSlab& o = out.slabs[j];
+      if (o.z0 == up) r.above = j;
+      if ((o.z0 + o.T) % o.L == r.z0) r.below = j;
     }
+    if (r.above < 0 || r.below < 0) return false;
   }
   for (auto& r : out.slabs)
     if (r.ns > 1) {
@@ -523,82 +611,66 @@ bool deal_slabs(const std::vector<PointHost>& ph, const std::vector<int>& points
   return true;
 }
 
-// The resident plan of a group on nwg workgroups of at most cap LDS words:
-// for each lattice size the slab thickness is taken among its divisors
-// whose slabs hold at most about an even share of the cells per workgroup
-// (the five thickest), every combination over the batch's lattice sizes is
-// dealt (a scan of fractions of the even share instead when there would be
-// more than 400), and the deal with the shortest longest workgroup is kept.
-// False when the group's points cannot run resident (fast schedule, several
-// Metropolis sweeps per step, the non-chessboard rule, or a window beyond
-// the LDS).
-bool plan_resident(const cvf_sim* s, const Group& g, int nwg, uint32_t cap, RDeal& best) {
-  if (s->sched != CVF_SCHED_REFERENCE || s->nm != 1) return false;
-  std::vector<uint32_t> Ls;
-  uint64_t total = 0;
-  for (int p : g.points) {
-    if (!s->ph[p].prm.flag_chessboard) return false;
-    Ls.push_back(s->ph[p].L);
-    total += s->ph[p].N;
+// Consistency of a resident plan, checked before it is used: every point of
+// the group covered by its slabs exactly once, each slab's waves a
+// non-empty range of its workgroup's wpg waves, disjoint from the others',
+// windows packed in order within cap words, neighbours above and below of
+// the same point adjacent in z, rings inside the ring buffer.
+bool resident_plan_valid(const cvf_sim* s, const Group& g, const RDeal& d, uint32_t wpg, uint32_t cap) {
+  std::vector<std::vector<uint8_t>> cov(s->ph.size());
+  std::vector<uint32_t> nslab(s->ph.size(), 0);
+  for (int p : g.points) cov[p].assign(s->ph[p].L, 0);
+  for (const RSlab& r : d.slabs) {
+    if (r.point < 0 || (size_t)r.point >= s->ph.size() || cov[r.point].empty()) return false;
+    ++nslab[r.point];
   }
-  std::sort(Ls.begin(), Ls.end());
-  Ls.erase(std::unique(Ls.begin(), Ls.end()), Ls.end());
-  const double target = (double)total / nwg;
-  std::vector<std::vector<uint32_t>> cand(Ls.size());
-  size_t combos = 1;
-  for (size_t i = 0; i < Ls.size(); ++i) {
-    const uint32_t L = Ls[i];
-    for (uint32_t t = L; t >= 1 && cand[i].size() < 5; --t)
-      if (L % t == 0 && window_words(L, t) <= cap && (double)t * L * L <= 1.05 * target) cand[i].push_back(t);
-    if (cand[i].empty() && window_words(L, 1) <= cap) cand[i].push_back(1);
-    if (cand[i].empty()) return false;
-    combos *= cand[i].size();
-  }
-  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> trials;
-  if (combos <= 400) {
-    for (size_t c = 0; c < combos; ++c) {
-      std::vector<std::pair<uint32_t, uint32_t>> Tof;
-      size_t r = c;
-      for (size_t i = 0; i < Ls.size(); ++i) {
-        Tof.emplace_back(Ls[i], cand[i][r % cand[i].size()]);
-        r /= cand[i].size();
+  for (const RGroupEnt& ge : d.groups) {
+    if (ge.count == 0 || ge.count > kMaxRSlabs || (size_t)ge.first + ge.count > d.slabs.size()) return false;
+    uint64_t waves = 0;
+    uint32_t lds = 0;
+    for (uint32_t j = ge.first; j < ge.first + ge.count; ++j) {
+      const RSlab& r = d.slabs[j];
+      const uint32_t L = s->ph[r.point].L, LL = L * L;
+      if (r.L != L || r.T == 0 || r.z0 + r.T > L || r.ns != nslab[r.point]) return false;
+      for (uint32_t z = r.z0; z < r.z0 + r.T; ++z)
+        if (cov[r.point][z]++) return false;
+      if (r.nw == 0 || r.w0 + r.nw > wpg || r.nw > 16) return false;
+      const uint64_t mine = ((1ull << r.nw) - 1) << r.w0;
+      if (waves & mine) return false;
+      waves |= mine;
+      if (r.lds != lds) return false;
+      lds += window_words(L, r.T, r.ns == 1);
+      const uint32_t stride = 64 * r.nw;
+      if (r.dp8 * (LL / 8) + r.du8 != stride || r.du8 >= LL / 8 || r.dp4 * (LL / 4) + r.du4 != stride ||
+          r.du4 >= LL / 4 || r.dpc * (L / 4) * (L / 4) + r.duc != stride || r.duc >= (L / 4) * (L / 4))
+        return false;
+      if (r.ns > 1) {
+        if (r.above < 0 || r.below < 0 || (size_t)r.above >= d.slabs.size() || (size_t)r.below >= d.slabs.size())
+          return false;
+        const RSlab& a = d.slabs[r.above];
+        const RSlab& b = d.slabs[r.below];
+        if (a.point != r.point || b.point != r.point || a.z0 != (r.z0 + r.T) % L || (b.z0 + b.T) % L != r.z0)
+          return false;
+        if ((size_t)r.ring + 4 * (size_t)kRingHalf * LL > d.ring_words || r.ring_hi != a.ring || r.ring_lo != b.ring)
+          return false;
       }
-      trials.push_back(Tof);
     }
-  } else {
-    for (double f : {1.0, 1.0 / 2, 1.0 / 3, 1.0 / 4, 1.0 / 6, 1.0 / 8, 1.0 / 12, 1.0 / 16}) {
-      std::vector<std::pair<uint32_t, uint32_t>> Tof;
-      for (size_t i = 0; i < Ls.size(); ++i) {
-        uint32_t T = cand[i].back();
-        for (uint32_t t : cand[i])
-          if ((double)t * Ls[i] * Ls[i] <= f * target) {
-            T = t;
-            break;
-          }
-        Tof.emplace_back(Ls[i], T);
-      }
-      trials.push_back(Tof);
-    }
+    if (lds > cap || lds * 4 > d.lds_bytes) return false;
   }
-  bool found = false;
-  for (const auto& Tof : trials) {
-    RDeal d;
-    if (!deal_slabs(s->ph, g.points, Tof, nwg, cap, d)) continue;
-    if (!found || d.makespan < best.makespan) {
-      best = std::move(d);
-      found = true;
-    }
-  }
-  return found;
+  for (int p : g.points)
+    for (uint8_t c : cov[p])
+      if (c != 1) return false;
+  return true;
 }
 
 template <class T>
 int upload_vec(DevBuf<T>& buf, const std::vector<T>& v);
 
 // With fused = 2, `streams` is the number of resident workgroups per CU (1
-// or 2): every workgroup of the launch must be resident at once, so the
-// plan is checked against the occupancy the runtime reports for its
-// workgroup size and LDS, and falls back to fused slab launches otherwise.
+// or 2) and slab_planes caps the slab thickness (0: automatic): every
+// workgroup of the launch must be resident at once, so the plan is checked
+// against the occupancy the runtime reports for its workgroup size and LDS,
+// and falls back to fused slab launches otherwise.
 int build_resident(cvf_sim* s, Group& g) {
   g.resident = false;
   int ncu = 0;
@@ -606,8 +678,10 @@ int build_resident(cvf_sim* s, Group& g) {
   if (ncu <= 0) return CVF_OK;
   const int per_cu = std::min(2, std::max(1, s->tune.streams));
   const uint32_t cap = (resident_lds_limit() - (uint32_t)(per_cu - 1) * kResidentStatic) / (uint32_t)per_cu / 4u;
+  const uint32_t wpg = (uint32_t)s->tune.block_chunk / 64u;
   RDeal d;
-  if (!plan_resident(s, g, ncu * per_cu, cap, d)) return CVF_OK;
+  if (!plan_resident(s, g, ncu * per_cu, wpg, cap, (uint32_t)std::max(0, s->tune.slab_planes), d)) return CVF_OK;
+  if (!resident_plan_valid(s, g, d, wpg, cap)) return fail(CVF_EINVAL, "internal: the resident plan is inconsistent");
   int occ = 0;
   HIPC(resident_occupancy(&occ, (uint32_t)s->tune.block_chunk, d.lds_bytes));
   if (occ < per_cu) return CVF_OK;
@@ -619,7 +693,9 @@ int build_resident(cvf_sim* s, Group& g) {
   g.r_ngroups = (uint32_t)d.groups.size();
   g.r_lds = d.lds_bytes;
   g.resident = true;
-  for (const auto& r : d.slabs) s->ph[r.point].slab_T = r.T;
+  for (int p : g.points) s->ph[p].slab_T = 0;
+  // the plan query reports a point's thickest slab
+  for (const auto& r : d.slabs) s->ph[r.point].slab_T = std::max(s->ph[r.point].slab_T, r.T);
   return CVF_OK;
 }
 

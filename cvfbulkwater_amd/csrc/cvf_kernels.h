@@ -126,9 +126,9 @@ void get_launch_timing(hipEvent_t* start, hipEvent_t* stop);
 
 // ------------------------------------------------- resident Metropolis kernel
 // One z-slab of one state point held in a resident workgroup's LDS
-// (cvf_resident.hip).  Its window: the ghost of the plane below, its T
-// planes, the ghost of the plane above (no ghosts when ns = 1: the whole
-// lattice, periodic in LDS).  Its rings in `rings` (words): [ring, ring +
+// (cvf_resident.hip), worked on by its own waves of the workgroup.  Its
+// window: the ghost of the plane below, its T planes, the ghost of the
+// plane above (no ghosts when ns = 1: the whole lattice, periodic in LDS).  Its rings in `rings` (words): [ring, ring +
 // 2 kRingHalf L^2) its bottom plane as published for the slab below, then
 // its top plane for the slab above; versions ver[2 i] (bottom), ver[2 i + 1]
 // (top) for slab i.
@@ -141,6 +141,11 @@ struct RSlab {
   int32_t above, below;       // neighbouring slabs (ns > 1), indices into the slab array
   uint32_t ring;              // this slab's rings
   uint32_t ring_hi, ring_lo;  // the rings of `above` and `below`
+  uint32_t w0, nw;            // its waves [w0, w0 + nw) of the workgroup
+  // a lane's walk over a phase's items advances by 64 nw items: dp planes
+  // and du items of L^2/8 (in-plane quads), L^2/4 (z quads) or (L/4)^2
+  // (four-cube items) per plane
+  uint32_t dp8, du8, dp4, du4, dpc, duc;
   uint32_t pad;
 };
 static_assert(sizeof(RSlab) % 16 == 0, "RSlab is copied in words");

@@ -2,13 +2,18 @@
 // of MC steps (the reference schedule: eta cube sweep + 12 sigma
 // half-sweeps per step, gpu_water3D.cu:1346-1417, 1890-2008, 3374-3466).
 //
-// Each state point is cut into ns z-slabs of T planes (ns = 1: the whole
-// lattice in one window).  The host deals the slabs to at most one
-// workgroup per CU, balancing cells, so every workgroup of the launch is
-// resident at once.  A workgroup stages its slabs once, runs `nsteps` MC
-// steps on them in LDS and writes them back once.  A slab's window holds
-// its T planes plus a ghost copy of the plane below (window plane 0) and of
-// the plane above (window plane T + 1), owned by the neighbouring slabs.
+// Each state point is cut into ns z-slabs (ns = 1: the whole lattice in one
+// window, periodic in LDS); slabs of one point may differ in thickness.
+// The host fills the workgroups with slabs, about the same number of cells
+// each, at most one or two workgroups per CU so that every workgroup of the
+// launch is resident at once, and gives every slab of a workgroup its own
+// range of waves, in proportion to its cells.  A wave works on its slab
+// only: the slab's fields sit in scalar registers for the whole launch, and
+// the items of every phase are walked without a division.  A workgroup
+// stages its slabs once, runs `nsteps` MC steps on them in LDS and writes
+// them back once.  A split slab's window holds its T planes plus a ghost
+// copy of the plane below (window plane 0) and of the plane above (window
+// plane T + 1), owned by the neighbouring slabs.
 //
 // Only z half-sweeps and the cube sweep read across a slab boundary:
 //   * a +z half-sweep (arm 4) of the top plane reads sigma_5 of the plane
@@ -93,6 +98,7 @@ __device__ __forceinline__ void ring_stores_done() {
 }
 
 __device__ __forceinline__ uint32_t parity_of(int z, uint32_t L) { return (uint32_t)(z + (int)L) & 1u; }
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Cube move on the cube with origin (ox, oy) of bottom plane B and top
 // plane U (L x L words each, periodic in x and y).
@@ -113,7 +119,7 @@ __device__ __forceinline__ void eta_cube_planes(uint32_t* B, uint32_t* U, const 
 }
 
 // Chessboard sigma rule on four words held in registers (facing neighbour
-// words wn, one Philox block), as sigma_quad.
+// words wn, one Philox block): the cells' new words.
 __device__ __forceinline__ void sigma_quad_regs(uint32_t (&w)[4], const uint32_t (&wn)[4], const uint32_t* thr,
                                                 int arm, const U4& q) {
 #pragma unroll
@@ -128,381 +134,329 @@ __device__ __forceinline__ void sigma_quad_regs(uint32_t (&w)[4], const uint32_t
   }
 }
 
-// Items of one slab in one phase of a step: planes first, first + step,
-// ... (np of them), Q items per plane.
-struct Seg {
-  int32_t first;  // first slab-relative plane
-  uint32_t np;    // planes
-  uint32_t Q;     // items per plane
-  uint32_t mQ;    // fast-division multiplier of Q
-  uint32_t step;  // plane stride (1 or 2)
+// A lane's walk over the items of one phase of its slab: item it = pi * Q +
+// u, starting at (wave - w0) * 64 + lane and advancing by the slab's
+// 64 nw lanes (dp planes and du items), without a division.
+struct Walk {
+  uint32_t pi, u;
 };
-
-constexpr int kPhases = 14;  // per step: two cube families, then the 12 half-sweeps
-
-__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// Wave-uniform copy of the fields of the slab a wave's items belong to.
-struct Cur {
-  uint32_t g, L, hL, qL, LL, z0, T, ns, lds, mH, mQ;
-  uint64_t seed;
-  Seg sg;
-};
-
-// The slab of 64-item chunk ch of phase ph (pre: the slabs' first chunks)
-// and its fields, as scalars: every chunk lies within one slab.
-__device__ __forceinline__ Cur chunk_slab(const uint32_t* pre, const Seg* seg, const RSlab* S, uint32_t ch) {
-  Cur c;
-  uint32_t g = 0;
-  while (ch >= pre[g + 1]) ++g;
-  c.g = uni(g);
-  const RSlab& s = S[c.g];
-  c.L = uni(s.L);
-  c.hL = c.L / 2;
-  c.qL = c.L / 4;
-  c.LL = c.L * c.L;
-  c.z0 = uni(s.z0);
-  c.T = uni(s.T);
-  c.ns = uni(s.ns);
-  c.lds = uni(s.lds);
-  c.mH = uni(s.mH);
-  c.mQ = uni(s.mQ);
-  c.seed = ((uint64_t)uni(s.seed_hi) << 32) | uni(s.seed_lo);
-  const Seg& sg = seg[c.g];
-  c.sg = Seg{(int32_t)uni((uint32_t)sg.first), uni(sg.np), uni(sg.Q), uni(sg.mQ), uni(sg.step)};
-  return c;
+__device__ __forceinline__ void advance(Walk& w, uint32_t Q, uint32_t dp, uint32_t du) {
+  w.u += du;
+  w.pi += dp;
+  if (w.u >= Q) {
+    w.u -= Q;
+    ++w.pi;
+  }
+}
+__device__ __forceinline__ Walk walk_start(uint32_t it, uint32_t Q, uint32_t mQ) {
+  const uint32_t pi = fdiv(it, FastDiv{Q, mQ});
+  return Walk{pi, it - pi * Q};
 }
 
 }  // namespace
 
 // grid = one workgroup per RGroupEnt; dynamic LDS = the largest sum of
-// window sizes of one workgroup.
+// window sizes of one workgroup.  The slab records sit in LDS and each
+// phase reads the fields it uses (wave-uniform), so that they are not all
+// held in scalar registers through the step loop.
 __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__ grp, const RSlab* __restrict__ slabs,
                                                     const PointDev* __restrict__ pts, uint32_t* __restrict__ words,
                                                     uint32_t* __restrict__ rings, uint64_t* __restrict__ ver,
                                                     const uint64_t* __restrict__ stepp, uint32_t step_off,
                                                     uint32_t nsteps, uint32_t* __restrict__ status) {
   extern __shared__ __align__(16) uint32_t lds[];
-  __shared__ RSlab S[kMaxRSlabs];
+  __shared__ RSlab Ssh[kMaxRSlabs];
   __shared__ uint32_t thr_s[kMaxRSlabs][33];
   __shared__ uint64_t thr_e[kMaxRSlabs][9];
   __shared__ uint64_t etab[9];
-  __shared__ uint32_t sch[kMaxRSlabs][12];
-  // per slab and half-sweep: ring slot j + 1 to read above (+z, top plane)
-  // or below (-z, bottom plane), 0 = the window's ghost; slot j + 1 to
-  // publish into, 0 = none
-  __shared__ uint8_t rd[kMaxRSlabs][12], wr[kMaxRSlabs][12];
-  // items of each slab in each phase of the step, and their prefix sums
-  __shared__ Seg seg[kPhases][kMaxRSlabs];
-  __shared__ uint32_t pre[kPhases][kMaxRSlabs + 1];
   __shared__ uint32_t abort_s;
   const RGroupEnt G = grp[blockIdx.x];
-  const uint32_t nsl = G.count, tid = threadIdx.x, nt = blockDim.x;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = uni(tid >> 6);
   {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(slabs + G.first);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(S);
-    for (uint32_t i = tid; i < nsl * (uint32_t)(sizeof(RSlab) / 4); i += nt) dst[i] = src[i];
+    uint32_t* dst = reinterpret_cast<uint32_t*>(Ssh);
+    for (uint32_t i = tid; i < G.count * (uint32_t)(sizeof(RSlab) / 4); i += blockDim.x) dst[i] = src[i];
   }
   if (tid == 0) abort_s = 0;
   if (tid < 9) etab[tid] = eta_table_entry(tid);
   __syncthreads();
-  for (uint32_t i = tid; i < nsl * 33; i += nt) {
-    const uint32_t g = i / 33, k = i - g * 33;
-    thr_s[g][k] = sigma_thr32(pts[S[g].point].thr_sigma[k]);
+  // this wave's slab: the one whose wave range holds it (none: the wave
+  // only keeps the workgroup's barriers)
+  uint32_t gs = kMaxRSlabs;
+  for (uint32_t g = 0; g < G.count; ++g)
+    if (wave >= Ssh[g].w0 && wave < Ssh[g].w0 + Ssh[g].nw) gs = g;
+  gs = uni(gs);
+  const bool has = gs < kMaxRSlabs;
+  const RSlab& S = Ssh[has ? gs : 0];
+#define F(f) uni((uint32_t)S.f)
+  const uint32_t li = has ? (wave - F(w0)) * 64u + lane : 0u;  // the lane's first item
+  if (has && wave == F(w0)) {
+    const PointDev& P = pts[F(point)];
+    if (lane < 33) thr_s[gs][lane] = sigma_thr32(P.thr_sigma[lane]);
+    if (lane < 9) thr_e[gs][lane] = P.thr_eta[lane];
   }
-  for (uint32_t i = tid; i < nsl * 9; i += nt) {
-    const uint32_t g = i / 9, k = i - g * 9;
-    thr_e[g][k] = pts[S[g].point].thr_eta[k];
-  }
-  // stage every window in 16-byte pieces (L^2 is a multiple of 16 words)
-  for (uint32_t g = 0; g < nsl; ++g) {
-    const RSlab& s = S[g];
-    const uint32_t LL = s.L * s.L, Q4 = LL / 4, np = s.ns > 1 ? s.T + 2 : s.T;
-    const uint4* W4 = reinterpret_cast<const uint4*>(words + s.woff);
-    uint4* D4 = reinterpret_cast<uint4*>(lds + s.lds);
-    for (uint32_t i = tid; i < np * Q4; i += nt) {
-      const uint32_t p = i / Q4, r = i - p * Q4;
-      const int zg = s.ns > 1 ? (int)s.z0 - 1 + (int)p : (int)p;
-      const uint32_t z = (uint32_t)(zg < 0 ? zg + (int)s.L : zg >= (int)s.L ? zg - (int)s.L : zg);
+  if (has) {
+    // the slab's window, staged by its waves in 16-byte pieces (L^2 is a
+    // multiple of 16 words)
+    const uint32_t L = F(L), LL = L * L, Q4 = LL / 4, gofs = F(ns) > 1 ? 1u : 0u, np = F(T) + 2 * gofs;
+    const uint4* W4 = reinterpret_cast<const uint4*>(words + F(woff));
+    uint4* D4 = reinterpret_cast<uint4*>(lds + F(lds));
+    const FastDiv dq{Q4, F(mP4)};
+    const int zs = (int)F(z0) - (int)gofs;
+    for (uint32_t i = li; i < np * Q4; i += 64 * F(nw)) {
+      const uint32_t p = fdiv(i, dq), r = i - p * Q4;
+      const int zg = zs + (int)p;
+      const uint32_t z = (uint32_t)(zg < 0 ? zg + (int)L : zg >= (int)L ? zg - (int)L : zg);
       D4[i] = W4[z * Q4 + r];
     }
   }
   __syncthreads();
+  // the lane's first item in the three item shapes: in-plane quads (L^2/8
+  // per plane), z quads (L^2/4), four-cube items ((L/4)^2)
+  Walk s8{0, 0}, s4{0, 0}, sc{0, 0};
+  if (has) {
+    const uint32_t L = F(L), qL = L / 4;
+    s8 = walk_start(li, L * L / 8, F(mP8));
+    s4 = walk_start(li, L * L / 4, F(mP4));
+    sc = walk_start(li, qL * qL, F(mQQ));
+  }
   const uint64_t step0 = *stepp + step_off;
   for (uint32_t si = 0; si < nsteps; ++si) {
     const uint64_t step = step0 + si;
     const uint32_t half = (uint32_t)(step & 1u) * kRingHalf;
-    if (tid < nsl * 12) {
-      const uint32_t g = tid / 12, k = tid - g * 12;
-      const uint64_t seed = ((uint64_t)S[g].seed_hi << 32) | S[g].seed_lo;
-      const U4 r = draw(seed, kStSched, 0, step, k);
-      sch[g][k] = uniform_int(r.x, 6) | ((r.y >> 31) << 3);
-    }
-    if (si > 0) {
-      // the ghosts: the neighbours' boundary planes at the end of the last step
-      if (tid == 0) {
+    // the step's 12 half-sweeps (lane k: half-sweep k) and their ring slots:
+    // rd = 1 + the neighbour's publish to read (0: the ghost is current),
+    // wr = 1 + this slab's publish
+    uint32_t schv = 0, rdv = 0, wrv = 0;
+    if (has) {
+      const uint32_t L = F(L), z0 = F(z0), T = F(T);
+      const uint64_t seed = ((uint64_t)F(seed_hi) << 32) | F(seed_lo);
+      if (lane < 12) {
+        const U4 r = draw(seed, kStSched, 0, step, lane);
+        schv = uniform_int(r.x, 6) | ((r.y >> 31) << 3);
+      }
+      const uint32_t arm = schv & 7u, q = (schv >> 3) ? 0u : 1u;
+      const bool in = lane < 12 && F(ns) > 1;
+      const uint32_t p_bot = parity_of((int)z0, L), p_top = parity_of((int)(z0 + T - 1), L);
+      const uint32_t p_abv = parity_of((int)(z0 + T), L), p_blw = parity_of((int)z0 - 1, L);
+      const uint64_t before = (1ull << lane) - 1;  // lanes before this one
+      const uint64_t up = __ballot(in && arm == 4 && q == p_top), dn = __ballot(in && arm == 5 && q == p_bot);
+      const uint64_t ab = __ballot(in && arm == 5 && q == p_abv), bl = __ballot(in && arm == 4 && q == p_blw);
+      if ((up >> lane) & 1) {
+        rdv = __popcll(ab & before);
+        wrv = __popcll(up & before) + 1;
+      } else if ((dn >> lane) & 1) {
+        rdv = __popcll(bl & before);
+        wrv = __popcll(dn & before) + 1;
+      }
+      if (si > 0 && F(ns) > 1) {
+        // the ghosts: the neighbours' boundary planes at the end of the last step
+        const uint32_t LL = L * L;
         const uint64_t want = ring_version(step - 1, kRingHalf - 1);
-        for (uint32_t g = 0; g < nsl && !abort_s; ++g)
-          if (S[g].ns > 1 && (!wait_version(ver + 2 * S[g].above, want, status) ||
-                              !wait_version(ver + 2 * S[g].below + 1, want, status)))
-            abort_s = 1;
-      }
-      __syncthreads();
-      const uint32_t prev = (uint32_t)((step - 1) & 1u) * kRingHalf + kRingHalf - 1;
-      for (uint32_t g = 0; g < nsl && !abort_s; ++g) {
-        const RSlab& s = S[g];
-        if (s.ns == 1) continue;
-        const uint32_t LL = s.L * s.L;
-        const uint32_t* hi = rings + s.ring_hi + prev * LL;
-        const uint32_t* lo = rings + s.ring_lo + (2 * kRingHalf + prev) * LL;
-        uint32_t* D = lds + s.lds;
-        for (uint32_t i = tid; i < 2 * LL; i += nt)
-          D[i < LL ? (s.T + 1) * LL + i : i - LL] = ring_load(i < LL ? hi + i : lo + (i - LL));
-      }
-    }
-    __syncthreads();
-    // per slab: ring slots of this step's half-sweeps and the items of every
-    // phase
-    if (tid < nsl) {
-      const RSlab& s = S[tid];
-      uint32_t n_dn = 0, n_up = 0, n_hi = 0, n_lo = 0;  // publishes so far: mine (bottom, top), above's, below's
-      const uint32_t p_bot = parity_of((int)s.z0, s.L), p_top = parity_of((int)(s.z0 + s.T - 1), s.L);
-      const uint32_t p_abv = parity_of((int)(s.z0 + s.T), s.L), p_blw = parity_of((int)s.z0 - 1, s.L);
-      const uint32_t LL = s.L * s.L, qL = s.L / 4;
-      for (uint32_t fam = 0; fam < 2; ++fam) {
-        // cube origins: planes [-1, T-1] (ghost below included) of the
-        // family's parity; [0, T-1] for a whole-lattice window
-        const int zlo = s.ns > 1 ? -1 : 0;
-        const int first = zlo + (parity_of((int)s.z0 + zlo, s.L) == fam ? 0 : 1);
-        seg[fam][tid] = Seg{first, first <= (int)s.T - 1 ? (uint32_t)((int)s.T - 1 - first) / 2 + 1 : 0u,
-                            qL * qL, s.mQQ, 2};
-      }
-      for (int k = 0; k < 12; ++k) {
-        const uint32_t arm = sch[tid][k] & 7u, q = (sch[tid][k] >> 3) ? 0u : 1u;
-        uint8_t r = 0, w = 0;
-        if (s.ns > 1 && arm == 4) {
-          if (q == p_top) {
-            r = (uint8_t)n_hi;  // 0: nothing published above in this step yet
-            w = (uint8_t)(++n_up);
-          }
-        } else if (s.ns > 1 && arm == 5) {
-          if (q == p_bot) {
-            r = (uint8_t)n_lo;
-            w = (uint8_t)(++n_dn);
-          }
-        }
-        // the neighbours publish on the same schedule (one state point)
-        if (s.ns > 1 && arm == 5 && q == p_abv) ++n_hi;
-        if (s.ns > 1 && arm == 4 && q == p_blw) ++n_lo;
-        rd[tid][k] = r;
-        wr[tid][k] = w;
-        if (arm < 4) {
-          seg[2 + k][tid] = Seg{0, s.T, LL / 8, s.mP8, 1};
+        bool ok = true;
+        if (lane == 0) ok = wait_version(ver + 2 * F(above), want, status) && wait_version(ver + 2 * F(below) + 1, want, status);
+        if (!uni(ok ? 1u : 0u)) {
+          abort_s = 1;
         } else {
-          const int first = parity_of((int)s.z0, s.L) == q ? 0 : 1;
-          seg[2 + k][tid] = Seg{first, first < (int)s.T ? (s.T - (uint32_t)first + 1) / 2 : 0u, LL / 4, s.mP4, 2};
+          const uint32_t prev = (uint32_t)((step - 1) & 1u) * kRingHalf + kRingHalf - 1;
+          const uint32_t* hi = rings + F(ring_hi) + prev * LL;
+          const uint32_t* lo = rings + F(ring_lo) + (2 * kRingHalf + prev) * LL;
+          uint32_t* D = lds + F(lds);
+          for (uint32_t i = li; i < 2 * LL; i += 64 * F(nw))
+            D[i < LL ? (T + 1) * LL + i : i - LL] = ring_load(i < LL ? hi + i : lo + (i - LL));
         }
       }
     }
     __syncthreads();
-    if (tid < kPhases) {
-      // items are handed out in chunks of 64 (one per wave), each slab's
-      // items padded to whole chunks, so that a wave's slab is uniform
-      uint32_t a = 0;
-      for (uint32_t g = 0; g < nsl; ++g) {
-        pre[tid][g] = a;
-        a += (seg[tid][g].np * seg[tid][g].Q + 63) / 64;
-      }
-      pre[tid][nsl] = a;
-    }
-    __syncthreads();
-    const uint32_t lane = tid & 63u, wave = uni(tid >> 6), nwave = nt >> 6;
-    // The waves take the phase's 64-item chunks in turn (wave w: chunks w,
-    // w + nwave, ...: consecutive chunks on different SIMDs).  A wave keeps
-    // the scalar fields of its chunk's slab until a chunk lies past that
-    // slab.  body(c, pi, u) handles the lane's item: plane pi of the slab's
-    // phase segment, in-plane item u.
-    auto walk = [&](uint32_t ph, auto&& body) {
-      const uint32_t C = uni(pre[ph][nsl]);
-      Cur c;
-      uint32_t sbeg = 0, send = 0, cnt = 0;
-      for (uint32_t ch = wave; ch < C && !abort_s; ch += nwave) {
-        if (ch >= send) {
-          c = chunk_slab(pre[ph], seg[ph], S, ch);
-          sbeg = uni(pre[ph][c.g]);
-          send = uni(pre[ph][c.g + 1]);
-          cnt = c.sg.np * c.sg.Q;
-        }
-        const uint32_t loc = (ch - sbeg) * 64 + lane;
-        if (loc < cnt) {
-          const uint32_t pi = fdiv(loc, FastDiv{c.sg.Q, c.sg.mQ});
-          body(c, pi, loc - pi * c.sg.Q);
-        }
-      }
-    };
+    const bool run = has && !abort_s;
     // ---------------------------------------------------------- cube sweep
     for (uint32_t fam = 0; fam < 2; ++fam) {
-      walk(fam, [&](const Cur& c, uint32_t pi, uint32_t u) {
-        const uint32_t g = c.g;
-        const uint32_t jj = fdiv(u, FastDiv{c.qL, c.mQ}), ii = u - jj * c.qL;
-        const int zr = c.sg.first + 2 * (int)pi;
-        const uint32_t ox = 2 * ii + fam, oy = 2 * jj + fam;
-        const int zg = (int)c.z0 + zr;
-        const uint32_t zgw = (uint32_t)(zg < 0 ? zg + (int)c.L : zg);
-        // cube index x/2 + (L/2)(y/2) + (L^2/4) z (gpu_water3D.cu:3405)
-        const uint32_t cube_ref = ii + c.hL * jj + (c.LL / 4) * zgw;
-        const U4 r = draw(c.seed, kStEta, cube_ref, step, 0);
-        const uint32_t wb = c.ns > 1 ? (uint32_t)(zr + 1) : (uint32_t)zr;  // window plane of the origin
-        const uint32_t wt = c.ns > 1 ? wb + 1 : (wb + 1 == c.T ? 0u : wb + 1);
-        uint32_t* B = lds + c.lds + wb * c.LL;
-        uint32_t* U = lds + c.lds + wt * c.LL;
-        eta_cube_planes(B, U, thr_e[g], etab, c.L, ox, oy, r.x);
-        eta_cube_planes(B, U, thr_e[g], etab, c.L, ox, oy + c.hL, r.y);
-        eta_cube_planes(B, U, thr_e[g], etab, c.L, ox + c.hL, oy, r.z);
-        eta_cube_planes(B, U, thr_e[g], etab, c.L, ox + c.hL, oy + c.hL, r.w);
-      });
+      if (run) {
+        const uint32_t L = F(L), z0 = F(z0), T = F(T), ns = F(ns), hL = L / 2, qL = L / 4, LL = L * L, Qc = qL * qL;
+        const uint64_t seed = ((uint64_t)F(seed_hi) << 32) | F(seed_lo);
+        const uint32_t lbase = F(lds), gofs = ns > 1 ? 1u : 0u, dpc = F(dpc), duc = F(duc);
+        const FastDiv dQ{qL, F(mQ)};
+        const uint64_t* thre = thr_e[gs];
+        // origin planes of the family's parity among [-1, T-1] (the ghost
+        // below included) or [0, T-1] for a whole-lattice window
+        const int zlo = ns > 1 ? -1 : 0;
+        const int first = zlo + (parity_of((int)z0 + zlo, L) == fam ? 0 : 1);
+        const uint32_t npl = first <= (int)T - 1 ? (uint32_t)((int)T - 1 - first) / 2 + 1 : 0u;
+        for (Walk it = sc; it.pi < npl; advance(it, Qc, dpc, duc)) {
+          const uint32_t jj = fdiv(it.u, dQ), ii = it.u - jj * qL;
+          const int zr = first + 2 * (int)it.pi;
+          const uint32_t ox = 2 * ii + fam, oy = 2 * jj + fam;
+          const int zg = (int)z0 + zr;
+          const uint32_t zgw = (uint32_t)(zg < 0 ? zg + (int)L : zg);
+          // cube index x/2 + (L/2)(y/2) + (L^2/4) z (gpu_water3D.cu:3405)
+          const uint32_t cube_ref = ii + hL * jj + (LL / 4) * zgw;
+          const U4 r = draw(seed, kStEta, cube_ref, step, 0);
+          const uint32_t wb = (uint32_t)(zr + (int)gofs);
+          const uint32_t wt = ns > 1 ? wb + 1 : (wb + 1 == T ? 0u : wb + 1);
+          uint32_t* B = lds + lbase + wb * LL;
+          uint32_t* U = lds + lbase + wt * LL;
+          eta_cube_planes(B, U, thre, etab, L, ox, oy, r.x);
+          eta_cube_planes(B, U, thre, etab, L, ox, oy + hL, r.y);
+          eta_cube_planes(B, U, thre, etab, L, ox + hL, oy, r.z);
+          eta_cube_planes(B, U, thre, etab, L, ox + hL, oy + hL, r.w);
+        }
+      }
       __syncthreads();
     }
     // ----------------------------------------------------- 12 half-sweeps
     for (uint32_t k = 0; k < 12; ++k) {
-      const uint32_t ph = 2 + k;
-      bool wrote_ring = false;
-      walk(ph, [&](const Cur& c, uint32_t pi, uint32_t u) {
-        const uint32_t g = c.g;
-        const uint32_t L = c.L, hL = c.hL, qL = c.qL, LL = c.LL;
-        const uint32_t sc = uni(sch[g][k]);
-        const int arm = (int)(sc & 7u);
-        const uint32_t q = (sc >> 3) ? 0u : 1u;
-        const int ax = arm >> 1;
-        const int zr = c.sg.first + (int)(c.sg.step * pi);
-        const uint32_t zg = c.z0 + (uint32_t)zr;
-        const uint32_t wz = c.ns > 1 ? (uint32_t)(zr + 1) : (uint32_t)zr;  // window plane
-        const uint32_t base = c.lds + wz * LL;
-        uint32_t x, y, cell[4], nb[4];
-        const uint32_t* RG = nullptr;  // the facing neighbours' ring plane, or none (LDS)
+      const uint32_t sk = __builtin_amdgcn_readlane(schv, k);
+      const uint32_t wr = __builtin_amdgcn_readlane(wrv, k);
+      const int arm = (int)(sk & 7u);
+      const uint32_t q = (sk >> 3) ? 0u : 1u;
+      const int ax = arm >> 1;
+      if (run) {
+        const uint32_t L = F(L), z0 = F(z0), T = F(T), ns = F(ns), hL = L / 2, qL = L / 4, LL = L * L;
+        const uint64_t seed = ((uint64_t)F(seed_hi) << 32) | F(seed_lo);
+        const uint32_t lbase = F(lds), gofs = ns > 1 ? 1u : 0u;
+        const FastDiv dH{hL, F(mH)}, dQ{qL, F(mQ)};
+        const uint32_t* thr = thr_s[gs];
         if (ax < 2) {
-          if (ax == 0) {  // u = hx + qL y
-            y = fdiv(u, FastDiv{qL, c.mQ});
-            x = 2 * (u - y * qL) + q;
-          } else {  // u = x + hL hy
-            const uint32_t hy = fdiv(u, FastDiv{hL, c.mH});
-            x = u - hy * hL;
-            y = 2 * hy + q;
-          }
-          quad_plane_cells(base, L, x, y, arm, ax, cell, nb);
-        } else {
-          y = fdiv(u, FastDiv{hL, c.mH});
-          x = u - y * hL;
-          const uint32_t b0 = base + x + L * y, dy = hL * L;
-          cell[0] = b0;
-          cell[1] = b0 + dy;
-          cell[2] = b0 + hL;
-          cell[3] = b0 + hL + dy;
-          const bool edge = arm == 4 ? zr == (int)c.T - 1 : zr == 0;
-          const uint32_t r = edge ? uni(rd[g][k]) : 0u;
-          if (r) {
-            // the neighbour's plane as published after its r-th z half-sweep
-            // of this step, read from its ring
-            const RSlab& s = S[g];
-            const uint32_t nslab = arm == 4 ? (uint32_t)s.above : (uint32_t)s.below;
-            if (!wait_version(ver + 2 * nslab + (arm == 4 ? 0 : 1), ring_version(step, r - 1), status)) {
-              abort_s = 1;
-              return;
+          const uint32_t Q8 = LL / 8, dp8 = F(dp8), du8 = F(du8);
+          for (Walk it = s8; it.pi < T; advance(it, Q8, dp8, du8)) {
+            uint32_t x, y;
+            if (ax == 0) {  // u = hx + qL y
+              y = fdiv(it.u, dQ);
+              x = 2 * (it.u - y * qL) + q;
+            } else {  // u = x + hL hy
+              const uint32_t hy = fdiv(it.u, dH);
+              x = it.u - hy * hL;
+              y = 2 * hy + q;
             }
-            RG = arm == 4 ? rings + s.ring_hi + (half + r - 1) * LL
-                          : rings + s.ring_lo + (2 * kRingHalf + half + r - 1) * LL;
-            const uint32_t o = x + L * y;
-            nb[0] = o;
-            nb[1] = o + dy;
-            nb[2] = o + hL;
-            nb[3] = o + hL + dy;
-          } else {
-            // neighbour plane in the window (a whole-lattice window wraps)
-            const uint32_t nz = arm == 4 ? (c.ns == 1 && wz + 1 == c.T ? 0u : wz + 1)
-                                         : (c.ns == 1 && wz == 0 ? c.T - 1 : wz - 1);
-            const uint32_t nbase = c.lds + nz * LL;
+            uint32_t cell[4], nb[4], w[4], wn[4];
+            quad_plane_cells(lbase + (it.pi + gofs) * LL, L, x, y, arm, ax, cell, nb);
 #pragma unroll
-            for (int m = 0; m < 4; ++m) nb[m] = nbase + (cell[m] - base);
-          }
-        }
-        uint32_t w[4], wn[4];
-        if (RG) {
+            for (int m = 0; m < 4; ++m) {
+              w[m] = lds[cell[m]];
+              wn[m] = lds[nb[m]];
+            }
+            const U4 rq = draw(seed, kStSigma, x + L * y + LL * (z0 + it.pi), step, k);
+            sigma_quad_regs(w, wn, thr, arm, rq);
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            w[m] = lds[cell[m]];
-            wn[m] = ring_load(RG + nb[m]);
+            for (int m = 0; m < 4; ++m) lds[cell[m]] = w[m];
           }
         } else {
+          // planes of the kept parity; the boundary plane whose neighbour
+          // lies across the slab edge (top for +z, bottom for -z) is walked
+          // last, as plane nint of the same walk
+          const uint32_t Q4 = LL / 4, dp4 = F(dp4), du4 = F(du4);
+          const uint32_t first = parity_of((int)z0, L) == q ? 0u : 1u;
+          const uint32_t npl = first < T ? (T - first + 1) / 2 : 0u;
+          const uint32_t zb = arm == 4 ? T - 1 : 0u;
+          const bool bactive = npl > 0 && ((zb - first) & 1u) == 0;
+          const int dz = arm == 4 ? (int)LL : -(int)LL;
+          const uint32_t nint = bactive ? npl - 1 : npl;
+          const uint32_t zi0 = first + (bactive && zb == 0 ? 2u : 0u);
+          Walk it = s4;
+          for (; it.pi < nint; advance(it, Q4, dp4, du4)) {
+            const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
+            const uint32_t zr = zi0 + 2 * it.pi;
+            const uint32_t b0 = lbase + (zr + gofs) * LL + x + L * y, dy = hL * L;
+            const uint32_t cell[4] = {b0, b0 + dy, b0 + hL, b0 + hL + dy};
+            uint32_t w[4], wn[4];
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            w[m] = lds[cell[m]];
-            wn[m] = lds[nb[m]];
+            for (int m = 0; m < 4; ++m) {
+              w[m] = lds[cell[m]];
+              wn[m] = lds[cell[m] + dz];
+            }
+            const U4 rq = draw(seed, kStSigma, x + L * y + LL * (z0 + zr), step, k);
+            sigma_quad_regs(w, wn, thr, arm, rq);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) lds[cell[m]] = w[m];
+          }
+          if (bactive) {
+            // the neighbour's plane as published after its rd-th z
+            // half-sweep of this step (rd = 0: the ghost, or the wrapped
+            // plane of a whole-lattice window); the new words also go to
+            // this slab's ring slot wr
+            const uint32_t rd = __builtin_amdgcn_readlane(rdv, k);
+            const int dzb = ns > 1 ? dz : (arm == 4 ? -(int)((T - 1) * LL) : (int)((T - 1) * LL));
+            const bool mine = __ballot(it.pi == nint) != 0;
+            const uint32_t* RG = nullptr;
+            bool ok = true;
+            if (rd && mine) {
+              if (lane == 0)
+                ok = wait_version(ver + 2 * (arm == 4 ? F(above) : F(below)) + (arm == 4 ? 0 : 1),
+                                  ring_version(step, rd - 1), status);
+              if (!uni(ok ? 1u : 0u)) {
+                ok = false;
+                abort_s = 1;
+              } else {
+                RG = arm == 4 ? rings + F(ring_hi) + (half + rd - 1) * LL
+                              : rings + F(ring_lo) + (2 * kRingHalf + half + rd - 1) * LL;
+              }
+            }
+            uint32_t* RW = wr ? rings + F(ring) + ((arm == 4 ? 2 * kRingHalf : 0u) + half + wr - 1) * LL : nullptr;
+            const uint32_t pl = lbase + (zb + gofs) * LL, dy = hL * L;
+            if (ok)
+              for (; it.pi == nint; advance(it, Q4, dp4, du4)) {
+                const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
+                const uint32_t o2 = x + L * y;
+                const uint32_t off[4] = {o2, o2 + dy, o2 + hL, o2 + hL + dy};
+                uint32_t w[4], wn[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                  w[m] = lds[pl + off[m]];
+                  wn[m] = RG ? ring_load(RG + off[m]) : lds[pl + off[m] + dzb];
+                }
+                const U4 rq = draw(seed, kStSigma, o2 + LL * (z0 + zb), step, k);
+                sigma_quad_regs(w, wn, thr, arm, rq);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                  lds[pl + off[m]] = w[m];
+                  if (RW) ring_store(RW + off[m], w[m]);
+                }
+              }
+            if (RW) ring_stores_done();
           }
         }
-        const U4 rq = draw(c.seed, kStSigma, x + L * y + LL * zg, step, k);
-        sigma_quad_regs(w, wn, thr_s[g], arm, rq);
-#pragma unroll
-        for (int m = 0; m < 4; ++m) lds[cell[m]] = w[m];
-        const uint32_t wslot = uni(wr[g][k]);
-        if (ax == 2 && (arm == 4 ? zr == (int)c.T - 1 : zr == 0) && wslot) {
-          // a boundary plane all of whose cells move in this half-sweep: the
-          // new words go to this slab's ring as well
-          uint32_t* R = rings + uni(S[g].ring) + ((arm == 4 ? 2 * kRingHalf : 0u) + half + wslot - 1) * LL;
-#pragma unroll
-          for (int m = 0; m < 4; ++m) ring_store(R + (cell[m] - base), w[m]);
-          wrote_ring = true;
-        }
-      });
-      if (wrote_ring) ring_stores_done();
-      __syncthreads();
-      if (tid < nsl && wr[tid][k] && !abort_s) {
-        const uint32_t arm = sch[tid][k] & 7u;
-        publish(ver + 2 * (G.first + tid) + (arm == 4 ? 1 : 0), ring_version(step, wr[tid][k] - 1));
       }
+      __syncthreads();
+      if (has && wave == F(w0) && lane == 0 && wr && !abort_s)
+        publish(ver + 2 * (G.first + gs) + (arm == 4 ? 1 : 0), ring_version(step, wr - 1));
     }
     // ------------------------------------- boundary planes at step end
-    bool copied = false;
-    for (uint32_t g = 0; g < nsl && !abort_s; ++g) {
-      const RSlab& s = S[g];
-      if (s.ns == 1) continue;
-      const uint32_t LL = s.L * s.L;
-      uint32_t* dn = rings + s.ring + (half + kRingHalf - 1) * LL;
-      uint32_t* up = rings + s.ring + (2 * kRingHalf + half + kRingHalf - 1) * LL;
-      const uint32_t* bot = lds + s.lds + LL;
-      const uint32_t* top = lds + s.lds + s.T * LL;
-      for (uint32_t i = tid; i < 2 * LL; i += nt) {
+    if (has && F(ns) > 1 && !abort_s) {
+      const uint32_t L = F(L), LL = L * L, T = F(T);
+      uint32_t* dn = rings + F(ring) + (half + kRingHalf - 1) * LL;
+      uint32_t* up = rings + F(ring) + (2 * kRingHalf + half + kRingHalf - 1) * LL;
+      const uint32_t* bot = lds + F(lds) + LL;
+      const uint32_t* top = lds + F(lds) + T * LL;
+      for (uint32_t i = li; i < 2 * LL; i += 64 * F(nw)) {
         if (i < LL)
           ring_store(dn + i, bot[i]);
         else
           ring_store(up + (i - LL), top[i - LL]);
-        copied = true;
       }
+      ring_stores_done();
     }
-    if (copied) ring_stores_done();
     __syncthreads();
-    if (tid < nsl && S[tid].ns > 1 && !abort_s) {
-      publish(ver + 2 * (G.first + tid), ring_version(step, kRingHalf - 1));
-      publish(ver + 2 * (G.first + tid) + 1, ring_version(step, kRingHalf - 1));
+    if (has && F(ns) > 1 && wave == F(w0) && lane == 0 && !abort_s) {
+      publish(ver + 2 * (G.first + gs), ring_version(step, kRingHalf - 1));
+      publish(ver + 2 * (G.first + gs) + 1, ring_version(step, kRingHalf - 1));
     }
   }
   // the neighbours have read their windows (they published after staging)
   // before this workgroup overwrites its planes in `words`
-  if (tid == 0 && nsteps > 0) {
+  if (has && F(ns) > 1 && nsteps > 0 && lane == 0 && wave == F(w0)) {
     const uint64_t want = ring_version(step0 + nsteps - 1, kRingHalf - 1);
-    for (uint32_t g = 0; g < nsl && !abort_s; ++g)
-      if (S[g].ns > 1 && (!wait_version(ver + 2 * S[g].above, want, status) ||
-                          !wait_version(ver + 2 * S[g].below + 1, want, status)))
-        abort_s = 1;
+    if (!wait_version(ver + 2 * F(above), want, status) || !wait_version(ver + 2 * F(below) + 1, want, status))
+      abort_s = 1;
   }
   __syncthreads();
-  if (abort_s) return;  // after the last barrier
-  for (uint32_t g = 0; g < nsl; ++g) {
-    const RSlab& s = S[g];
-    const uint32_t LL = s.L * s.L, Q4 = LL / 4;
-    uint4* W4 = reinterpret_cast<uint4*>(words + s.woff + s.z0 * LL);
-    const uint4* Wl = reinterpret_cast<const uint4*>(lds + s.lds + (s.ns > 1 ? LL : 0u));
-    for (uint32_t i = tid; i < s.T * Q4; i += nt) W4[i] = Wl[i];
+  if (abort_s || !has) return;  // after the last barrier
+  {
+    const uint32_t L = F(L), LL = L * L, gofs = F(ns) > 1 ? 1u : 0u;
+    uint4* W4 = reinterpret_cast<uint4*>(words + F(woff) + F(z0) * LL);
+    const uint4* Wl = reinterpret_cast<const uint4*>(lds + F(lds) + gofs * LL);
+    for (uint32_t i = li; i < F(T) * (LL / 4); i += 64 * F(nw)) W4[i] = Wl[i];
   }
+#undef F
 }
 
 uint32_t resident_lds_limit() { return 160 * 1024 - kResidentStatic; }

@@ -157,7 +157,8 @@ typedef struct cvf_tuning {
                            lattice held in LDS, one stream group) where the points allow it
                            (reference schedule, one Metropolis sweep per step, chessboard
                            rule), else as 1 */
-  int32_t slab_planes;  /* planes per slab of the fused kernel (<= 16); 0 = automatic */
+  int32_t slab_planes;  /* planes per slab of the fused kernel (<= 16), or the thickest slab
+                           of the resident kernel; 0 = automatic */
   int32_t block_chunk;  /* workgroup size of the fused kernel */
   int32_t chunk_sweeps; /* half-sweeps per fused launch: 6, or 12 (a whole Metropolis pass per
                            launch where the wider window still fits in LDS); 0 = 6.  The default
