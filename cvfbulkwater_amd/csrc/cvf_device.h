@@ -50,6 +50,38 @@ __device__ __forceinline__ void eta_cube_move(const uint32_t (&w)[8], const uint
   }
 }
 
+// Cube moves on the four cubes of a cube item: origins (ox + a L/2, oy +
+// b L/2) on bottom plane B and top plane U (L x L words each, periodic in
+// x and y), random words r.x .. r.w in the order (0,0), (0,1), (1,0),
+// (1,1).  The four cubes are disjoint, so all 32 corner words are loaded
+// before the first move.
+__device__ __forceinline__ void eta_cube_item(uint32_t* lds, uint32_t B, uint32_t U, const uint64_t* thr,
+                                              const uint64_t* etab, uint32_t L, uint32_t ox, uint32_t oy,
+                                              const U4& r) {
+  const uint32_t hL = L / 2;
+  uint32_t idx[4][8], w[4][8];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t cx = ox + ((c & 2) ? hL : 0u), cy = oy + ((c & 1) ? hL : 0u);
+    const uint32_t x1 = cx + 1 == L ? 0 : cx + 1;
+    const uint32_t y1 = cy + 1 == L ? 0 : cy + 1;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint32_t x = (b & 1) ? x1 : cx, y = (b & 2) ? y1 : cy;
+      idx[c][b] = ((b & 4) ? U : B) + x + L * y;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) w[c][b] = lds[idx[c][b]];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t rw = c == 0 ? r.x : c == 1 ? r.y : c == 2 ? r.z : r.w;
+    eta_cube_move(w[c], thr, etab, rw, [&](int b, uint32_t v) { lds[idx[c][b]] = v; });
+  }
+}
+
 // Sigma acceptance thresholds as 32-bit values min(thr, 2^32 - 1).  Exact:
 // the acceptance word of the q = 6 split is (6w) mod 2^32, always even, so
 // it is below 2^32 - 1 exactly when it is below 2^32.

@@ -394,23 +394,6 @@ __global__ void k_sigma(const BlockEnt* __restrict__ tab, uint32_t nb, const uin
   sigma_quad(W, c, nbc, thr, arm, [&] { return draw(seed, kStSigma, c0, step, sub); });
 }
 
-// Cube update on LDS planes: Wp is the cube's origin plane (L x L words,
-// periodic in x and y), the top face is the next plane.
-__device__ __forceinline__ void eta_cube_plane(uint32_t* Wp, const uint64_t* thr, const uint64_t* etab,
-                                               uint32_t L, uint32_t LL, uint32_t ox, uint32_t oy, uint32_t rw) {
-  const uint32_t x1 = ox + 1 == L ? 0 : ox + 1;
-  const uint32_t y1 = oy + 1 == L ? 0 : oy + 1;
-  uint32_t idx[8], w[8];
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const uint32_t x = (b & 1) ? x1 : ox, y = (b & 2) ? y1 : oy;
-    idx[b] = ((b & 4) ? LL : 0u) + x + L * y;
-  }
-#pragma unroll
-  for (int b = 0; b < 8; ++b) w[b] = Wp[idx[b]];
-  eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t v) { Wp[idx[b]] = v; });
-}
-
 // z-arm quad with origin (x, y) on the plane starting at `base`: the
 // neighbour is on the plane above (arm 4) or below (arm 5)
 __device__ __forceinline__ void quad_z_cells(uint32_t base, uint32_t L, int LL, uint32_t x, uint32_t y, int arm,
@@ -549,11 +532,8 @@ __global__ void k_chunk(const BlockEnt* __restrict__ tab, uint32_t /* entries, l
         // cube index x/2 + (L/2)(y/2) + (L^2/4) z (gpu_water3D.cu:3405)
         const uint32_t cube_ref = ii + hL * jj + (gofs[wz] >> 2);
         const U4 r = draw(seed, kStEta, cube_ref, step, 0);
-        uint32_t* Wp = win + wz * LL;  // cube origin plane; the cube spans wz, wz+1
-        eta_cube_plane(Wp, thr_e, etab, (uint32_t)L, (uint32_t)LL, ox, oy, r.x);
-        eta_cube_plane(Wp, thr_e, etab, (uint32_t)L, (uint32_t)LL, ox, oy + hL, r.y);
-        eta_cube_plane(Wp, thr_e, etab, (uint32_t)L, (uint32_t)LL, ox + hL, oy, r.z);
-        eta_cube_plane(Wp, thr_e, etab, (uint32_t)L, (uint32_t)LL, ox + hL, oy + hL, r.w);
+        // cube origin plane wz; the cubes span wz, wz+1
+        eta_cube_item(win, (uint32_t)wz * LL, (uint32_t)(wz + 1) * LL, thr_e, etab, (uint32_t)L, ox, oy, r);
       }
       __syncthreads();
     }

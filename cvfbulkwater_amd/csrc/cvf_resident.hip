@@ -100,38 +100,37 @@ __device__ __forceinline__ void ring_stores_done() {
 __device__ __forceinline__ uint32_t parity_of(int z, uint32_t L) { return (uint32_t)(z + (int)L) & 1u; }
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Cube move on the cube with origin (ox, oy) of bottom plane B and top
-// plane U (L x L words each, periodic in x and y).
-__device__ __forceinline__ void eta_cube_planes(uint32_t* B, uint32_t* U, const uint64_t* thr, const uint64_t* etab,
-                                                uint32_t L, uint32_t ox, uint32_t oy, uint32_t rw) {
-  const uint32_t x1 = ox + 1 == L ? 0 : ox + 1;
-  const uint32_t y1 = oy + 1 == L ? 0 : oy + 1;
-  uint32_t* P[8];
-  uint32_t w[8];
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const uint32_t x = (b & 1) ? x1 : ox, y = (b & 2) ? y1 : oy;
-    P[b] = ((b & 4) ? U : B) + x + L * y;
-  }
-#pragma unroll
-  for (int b = 0; b < 8; ++b) w[b] = *P[b];
-  eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t v) { *P[b] = v; });
-}
-
 // Chessboard sigma rule on four words held in registers (facing neighbour
-// words wn, one Philox block): the cells' new words.
+// words wn, one Philox block): the cells' new words.  The four threshold
+// loads are issued together (the index is in range also when the proposal
+// equals the current value), then the four decisions are made.
 __device__ __forceinline__ void sigma_quad_regs(uint32_t (&w)[4], const uint32_t (&wn)[4], const uint32_t* thr,
                                                 int arm, const U4& q) {
+  uint32_t sp[4], ra[4], t[4];
+  bool act[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const uint32_t rw = m == 0 ? q.x : m == 1 ? q.y : m == 2 ? q.z : q.w;
-    uint32_t sp, ra;
-    split_word(rw, 6, sp, ra);
+    split_word(rw, 6, sp[m], ra[m]);
     const uint32_t sa = sigma_of(w[m], arm);
-    const uint32_t t = thr[sigma_thr_index(w[m], wn[m], arm, sa, sp)];
-    const uint32_t nw = (w[m] & ~(7u << (3 * arm))) | (sp << (3 * arm));
-    w[m] = sp != sa && ra < t ? nw : w[m];
+    act[m] = sp[m] != sa;
+    t[m] = thr[sigma_thr_index(w[m], wn[m], arm, sa, sp[m])];
   }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const uint32_t nw = (w[m] & ~(7u << (3 * arm))) | (sp[m] << (3 * arm));
+    w[m] = act[m] && ra[m] < t[m] ? nw : w[m];
+  }
+}
+
+// draw() with the key passed through an empty asm statement, so that the
+// compiler rebuilds the Philox round keys with scalar adds in each
+// iteration instead of holding all twenty in scalar registers across the
+// loop (which spills the phase's other scalars).
+__device__ __forceinline__ U4 draw_k(uint64_t seed, uint32_t stream, uint32_t site, uint64_t step, uint32_t sub) {
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  asm volatile("" : "+s"(k0), "+s"(k1));
+  return philox(site, (uint32_t)step, (stream << 24) | sub, (uint32_t)(step >> 32), k0, k1);
 }
 
 // A lane's walk over the items of one phase of its slab: item it = pi * Q +
@@ -153,6 +152,38 @@ __device__ __forceinline__ Walk walk_start(uint32_t it, uint32_t Q, uint32_t mQ)
   return Walk{pi, it - pi * Q};
 }
 
+// One in-plane half-sweep (arm along axis AX: 0 x, 1 y) of a slab's T
+// planes: the lane's quads from item li on, 64 nw items apart.
+template <int AX>
+__device__ __forceinline__ void sweep_inplane(uint32_t* lds, uint32_t li, uint32_t L, uint32_t T, uint32_t z0,
+                                              uint32_t pbase, uint64_t seed, uint64_t step, uint32_t k, int arm,
+                                              uint32_t q, FastDiv dH, FastDiv dQ, uint32_t mP8, uint32_t dp8,
+                                              uint32_t du8, const uint32_t* thr) {
+  const uint32_t hL = L / 2, qL = L / 4, LL = L * L, Q8 = LL / 8;
+  for (Walk it = walk_start(li, Q8, mP8); it.pi < T; advance(it, Q8, dp8, du8)) {
+    uint32_t x, y;
+    if (AX == 0) {  // u = hx + qL y
+      y = fdiv(it.u, dQ);
+      x = 2 * (it.u - y * qL) + q;
+    } else {  // u = x + hL hy
+      const uint32_t hy = fdiv(it.u, dH);
+      x = it.u - hy * hL;
+      y = 2 * hy + q;
+    }
+    uint32_t cell[4], nb[4], w[4], wn[4];
+    quad_plane_cells(pbase + it.pi * LL, L, x, y, arm, AX, cell, nb);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      w[m] = lds[cell[m]];
+      wn[m] = lds[nb[m]];
+    }
+    const U4 rq = draw_k(seed, kStSigma, x + L * y + LL * (z0 + it.pi), step, k);
+    sigma_quad_regs(w, wn, thr, arm, rq);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) lds[cell[m]] = w[m];
+  }
+}
+
 }  // namespace
 
 // grid = one workgroup per RGroupEnt; dynamic LDS = the largest sum of
@@ -169,6 +200,9 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
   __shared__ uint32_t thr_s[kMaxRSlabs][33];
   __shared__ uint64_t thr_e[kMaxRSlabs][9];
   __shared__ uint64_t etab[9];
+  // the step's half-sweeps of each slab: arm | parity << 3 | rd << 4 |
+  // wr << 12 (rd, wr: ring slots, see below)
+  __shared__ uint32_t sch_s[kMaxRSlabs][12];
   __shared__ uint32_t abort_s;
   const RGroupEnt G = grp[blockIdx.x];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = uni(tid >> 6);
@@ -211,15 +245,6 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     }
   }
   __syncthreads();
-  // the lane's first item in the three item shapes: in-plane quads (L^2/8
-  // per plane), z quads (L^2/4), four-cube items ((L/4)^2)
-  Walk s8{0, 0}, s4{0, 0}, sc{0, 0};
-  if (has) {
-    const uint32_t L = F(L), qL = L / 4;
-    s8 = walk_start(li, L * L / 8, F(mP8));
-    s4 = walk_start(li, L * L / 4, F(mP4));
-    sc = walk_start(li, qL * qL, F(mQQ));
-  }
   const uint64_t step0 = *stepp + step_off;
   for (uint32_t si = 0; si < nsteps; ++si) {
     const uint64_t step = step0 + si;
@@ -227,10 +252,10 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     // the step's 12 half-sweeps (lane k: half-sweep k) and their ring slots:
     // rd = 1 + the neighbour's publish to read (0: the ghost is current),
     // wr = 1 + this slab's publish
-    uint32_t schv = 0, rdv = 0, wrv = 0;
     if (has) {
       const uint32_t L = F(L), z0 = F(z0), T = F(T);
       const uint64_t seed = ((uint64_t)F(seed_hi) << 32) | F(seed_lo);
+      uint32_t schv = 0, rdv = 0, wrv = 0;
       if (lane < 12) {
         const U4 r = draw(seed, kStSched, 0, step, lane);
         schv = uniform_int(r.x, 6) | ((r.y >> 31) << 3);
@@ -249,6 +274,8 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
         rdv = __popcll(bl & before);
         wrv = __popcll(dn & before) + 1;
       }
+      // the slab's first wave keeps them for every wave of the slab
+      if (wave == F(w0) && lane < 12) sch_s[gs][lane] = schv | (rdv << 4) | (wrv << 12);
       if (si > 0 && F(ns) > 1) {
         // the ghosts: the neighbours' boundary planes at the end of the last step
         const uint32_t LL = L * L;
@@ -282,7 +309,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
         const int zlo = ns > 1 ? -1 : 0;
         const int first = zlo + (parity_of((int)z0 + zlo, L) == fam ? 0 : 1);
         const uint32_t npl = first <= (int)T - 1 ? (uint32_t)((int)T - 1 - first) / 2 + 1 : 0u;
-        for (Walk it = sc; it.pi < npl; advance(it, Qc, dpc, duc)) {
+        for (Walk it = walk_start(li, Qc, F(mQQ)); it.pi < npl; advance(it, Qc, dpc, duc)) {
           const uint32_t jj = fdiv(it.u, dQ), ii = it.u - jj * qL;
           const int zr = first + 2 * (int)it.pi;
           const uint32_t ox = 2 * ii + fam, oy = 2 * jj + fam;
@@ -290,25 +317,20 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
           const uint32_t zgw = (uint32_t)(zg < 0 ? zg + (int)L : zg);
           // cube index x/2 + (L/2)(y/2) + (L^2/4) z (gpu_water3D.cu:3405)
           const uint32_t cube_ref = ii + hL * jj + (LL / 4) * zgw;
-          const U4 r = draw(seed, kStEta, cube_ref, step, 0);
+          const U4 r = draw_k(seed, kStEta, cube_ref, step, 0);
           const uint32_t wb = (uint32_t)(zr + (int)gofs);
           const uint32_t wt = ns > 1 ? wb + 1 : (wb + 1 == T ? 0u : wb + 1);
-          uint32_t* B = lds + lbase + wb * LL;
-          uint32_t* U = lds + lbase + wt * LL;
-          eta_cube_planes(B, U, thre, etab, L, ox, oy, r.x);
-          eta_cube_planes(B, U, thre, etab, L, ox, oy + hL, r.y);
-          eta_cube_planes(B, U, thre, etab, L, ox + hL, oy, r.z);
-          eta_cube_planes(B, U, thre, etab, L, ox + hL, oy + hL, r.w);
+          eta_cube_item(lds, lbase + wb * LL, lbase + wt * LL, thre, etab, L, ox, oy, r);
         }
       }
       __syncthreads();
     }
     // ----------------------------------------------------- 12 half-sweeps
     for (uint32_t k = 0; k < 12; ++k) {
-      const uint32_t sk = __builtin_amdgcn_readlane(schv, k);
-      const uint32_t wr = __builtin_amdgcn_readlane(wrv, k);
+      const uint32_t sk = has ? uni(sch_s[gs][k]) : 0u;
+      const uint32_t wr = (sk >> 12) & 0xFFu;
       const int arm = (int)(sk & 7u);
-      const uint32_t q = (sk >> 3) ? 0u : 1u;
+      const uint32_t q = ((sk >> 3) & 1u) ? 0u : 1u;
       const int ax = arm >> 1;
       if (run) {
         const uint32_t L = F(L), z0 = F(z0), T = F(T), ns = F(ns), hL = L / 2, qL = L / 4, LL = L * L;
@@ -316,30 +338,12 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
         const uint32_t lbase = F(lds), gofs = ns > 1 ? 1u : 0u;
         const FastDiv dH{hL, F(mH)}, dQ{qL, F(mQ)};
         const uint32_t* thr = thr_s[gs];
-        if (ax < 2) {
-          const uint32_t Q8 = LL / 8, dp8 = F(dp8), du8 = F(du8);
-          for (Walk it = s8; it.pi < T; advance(it, Q8, dp8, du8)) {
-            uint32_t x, y;
-            if (ax == 0) {  // u = hx + qL y
-              y = fdiv(it.u, dQ);
-              x = 2 * (it.u - y * qL) + q;
-            } else {  // u = x + hL hy
-              const uint32_t hy = fdiv(it.u, dH);
-              x = it.u - hy * hL;
-              y = 2 * hy + q;
-            }
-            uint32_t cell[4], nb[4], w[4], wn[4];
-            quad_plane_cells(lbase + (it.pi + gofs) * LL, L, x, y, arm, ax, cell, nb);
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-              w[m] = lds[cell[m]];
-              wn[m] = lds[nb[m]];
-            }
-            const U4 rq = draw(seed, kStSigma, x + L * y + LL * (z0 + it.pi), step, k);
-            sigma_quad_regs(w, wn, thr, arm, rq);
-#pragma unroll
-            for (int m = 0; m < 4; ++m) lds[cell[m]] = w[m];
-          }
+        if (ax == 0) {
+          sweep_inplane<0>(lds, li, L, T, z0, lbase + gofs * LL, seed, step, k, arm, q, dH, dQ, F(mP8), F(dp8),
+                           F(du8), thr);
+        } else if (ax == 1) {
+          sweep_inplane<1>(lds, li, L, T, z0, lbase + gofs * LL, seed, step, k, arm, q, dH, dQ, F(mP8), F(dp8),
+                           F(du8), thr);
         } else {
           // planes of the kept parity; the boundary plane whose neighbour
           // lies across the slab edge (top for +z, bottom for -z) is walked
@@ -352,7 +356,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
           const int dz = arm == 4 ? (int)LL : -(int)LL;
           const uint32_t nint = bactive ? npl - 1 : npl;
           const uint32_t zi0 = first + (bactive && zb == 0 ? 2u : 0u);
-          Walk it = s4;
+          Walk it = walk_start(li, Q4, F(mP4));
           for (; it.pi < nint; advance(it, Q4, dp4, du4)) {
             const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
             const uint32_t zr = zi0 + 2 * it.pi;
@@ -364,7 +368,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
               w[m] = lds[cell[m]];
               wn[m] = lds[cell[m] + dz];
             }
-            const U4 rq = draw(seed, kStSigma, x + L * y + LL * (z0 + zr), step, k);
+            const U4 rq = draw_k(seed, kStSigma, x + L * y + LL * (z0 + zr), step, k);
             sigma_quad_regs(w, wn, thr, arm, rq);
 #pragma unroll
             for (int m = 0; m < 4; ++m) lds[cell[m]] = w[m];
@@ -374,7 +378,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
             // half-sweep of this step (rd = 0: the ghost, or the wrapped
             // plane of a whole-lattice window); the new words also go to
             // this slab's ring slot wr
-            const uint32_t rd = __builtin_amdgcn_readlane(rdv, k);
+            const uint32_t rd = (sk >> 4) & 0xFFu;
             const int dzb = ns > 1 ? dz : (arm == 4 ? -(int)((T - 1) * LL) : (int)((T - 1) * LL));
             const bool mine = __ballot(it.pi == nint) != 0;
             const uint32_t* RG = nullptr;
@@ -404,7 +408,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
                   w[m] = lds[pl + off[m]];
                   wn[m] = RG ? ring_load(RG + off[m]) : lds[pl + off[m] + dzb];
                 }
-                const U4 rq = draw(seed, kStSigma, o2 + LL * (z0 + zb), step, k);
+                const U4 rq = draw_k(seed, kStSigma, o2 + LL * (z0 + zb), step, k);
                 sigma_quad_regs(w, wn, thr, arm, rq);
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
