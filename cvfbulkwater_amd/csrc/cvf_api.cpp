@@ -441,17 +441,71 @@ struct Piece {
   uint32_t z0, T, wg;
 };
 
+// Modeled length of one MC step of a workgroup whose slabs (lattice L,
+// first plane z0, T planes, split or whole) have nw waves each, in
+// consecutive wave ranges: per phase, the busiest SIMD's wave iterations
+// (waves w run on SIMD w % 4; a wave of a slab with n items and v waves
+// makes ceil((n - 64 j) / (64 v)) iterations, j its rank in the slab)
+// times the phase's VALU instructions per iteration, summed over the two
+// cube families, eight in-plane and four z half-sweeps (two of each
+// parity): the expected phases of a step.
+struct SlabShape {
+  uint32_t L, z0, T;
+  bool split;
+};
+double workgroup_step_model(const std::vector<SlabShape>& sl, const std::vector<uint32_t>& nw) {
+  constexpr double kInPlane = 190, kZ = 176, kCube = 614;  // VALU per iteration
+  double total = 0;
+  for (int ph = 0; ph < 5; ++ph) {
+    // 0, 1: cube families; 2: in-plane (x8); 3, 4: z of parity 0, 1 (x2)
+    double simd[4] = {0, 0, 0, 0};
+    uint32_t w0 = 0;
+    for (size_t j = 0; j < sl.size(); ++j) {
+      const SlabShape& x = sl[j];
+      const uint32_t LL = x.L * x.L;
+      uint64_t n = 0;
+      double cost = 0;
+      if (ph < 2) {
+        const int zlo = x.split ? -1 : 0;
+        const int first = zlo + ((((int)x.z0 + zlo + (int)x.L) & 1) == ph ? 0 : 1);
+        const uint32_t npl = first <= (int)x.T - 1 ? (uint32_t)((int)x.T - 1 - first) / 2 + 1 : 0u;
+        n = (uint64_t)npl * (x.L / 4) * (x.L / 4);
+        cost = kCube;
+      } else if (ph == 2) {
+        n = (uint64_t)x.T * LL / 8;
+        cost = 8 * kInPlane;
+      } else {
+        const uint32_t first = (x.z0 & 1u) == (uint32_t)(ph - 3) ? 0u : 1u;
+        const uint32_t npl = first < x.T ? (x.T - first + 1) / 2 : 0u;
+        n = (uint64_t)npl * LL / 4;
+        cost = 2 * kZ;
+      }
+      const uint64_t span = 64ull * nw[j];
+      for (uint32_t i = 0; i < nw[j]; ++i) {
+        const uint64_t start = 64ull * i;
+        const uint64_t it = n > start ? (n - start + span - 1) / span : 0;
+        simd[(w0 + i) & 3] += (double)it * cost;
+      }
+      w0 += nw[j];
+    }
+    total += std::max(std::max(simd[0], simd[1]), std::max(simd[2], simd[3]));
+  }
+  return total;
+}
+
 // Fill workgroups in order with z-pieces of the points (in `order`): a
-// workgroup takes the thickest piece that keeps its largest cells per wave
-// within W, its windows within cap words and its slabs within max_slabs;
-// pieces are at least min_T planes, and so is what a cut leaves of the
-// point.  False if the points do not fit in nwg workgroups.
+// workgroup takes the thickest piece that keeps its modeled step length
+// (workgroup_step_model) within W, its windows within cap words and its
+// slabs within max_slabs; pieces are at least min_T planes, and so is what
+// a cut leaves of the point.  False if the points do not fit in nwg
+// workgroups.
 bool fill_workgroups(const cvf_sim* s, const std::vector<int>& order, double W, int nwg, uint32_t wpg, uint32_t cap,
                      uint32_t min_T, uint32_t max_T, std::vector<Piece>& pc) {
   pc.clear();
   const uint32_t max_slabs = std::min<uint32_t>(kMaxRSlabs, wpg);
-  uint32_t wg = 0, nsl = 0, used = 0;
+  uint32_t wg = 0, used = 0;
   std::vector<double> cells;
+  std::vector<SlabShape> shapes;
   std::vector<uint32_t> nw;
   for (int p : order) {
     const uint32_t L = s->ph[p].L, LL = L * L, tcap = max_T ? std::min(max_T, L) : L;
@@ -459,33 +513,34 @@ bool fill_workgroups(const cvf_sim* s, const std::vector<int>& order, double W, 
     while (z < L) {
       const uint32_t rest = L - z;
       uint32_t take = 0;
-      if (nsl < max_slabs)
+      if (shapes.size() < max_slabs)
         for (uint32_t t = std::min(rest, tcap); t >= 1; --t) {
           if (t < rest && (t < min_T || rest - t < min_T)) continue;
           const uint32_t need = window_words(L, t, t == L);
           if (used + need > cap) continue;
           cells.push_back((double)t * LL);
-          const bool ok = split_waves(cells, wpg, nw);
-          double worst = 0;
-          for (size_t j = 0; ok && j < cells.size(); ++j) worst = std::max(worst, cells[j] / nw[j]);
+          shapes.push_back(SlabShape{L, z, t, t < L});
+          const bool ok = split_waves(cells, wpg, nw) && workgroup_step_model(shapes, nw) <= W;
           cells.pop_back();
-          if (ok && worst <= W) {
+          shapes.pop_back();
+          if (ok) {
             take = t;
             break;
           }
         }
       if (take == 0) {
         // the next workgroup; an empty one that takes nothing cannot help
-        if (nsl == 0 || wg + 1 == (uint32_t)nwg) return false;
+        if (shapes.empty() || wg + 1 == (uint32_t)nwg) return false;
         ++wg;
-        nsl = used = 0;
+        used = 0;
         cells.clear();
+        shapes.clear();
         continue;
       }
       pc.push_back(Piece{p, z, take, wg});
       cells.push_back((double)take * LL);
+      shapes.push_back(SlabShape{L, z, take, take < L});
       used += window_words(L, take, take == L);
-      ++nsl;
       z += take;
     }
   }
@@ -495,8 +550,9 @@ bool fill_workgroups(const cvf_sim* s, const std::vector<int>& order, double W, 
 // The resident plan of a group on at most nwg workgroups of wpg waves and
 // cap LDS words each: the points, largest lattice first, cut into z-pieces
 // that fill the workgroups in order (fill_workgroups), with the smallest
-// bound on any slab's cells per wave (its cells over its waves, which set
-// the length of every phase) for which they fit, found by bisection.  A
+// bound on any workgroup's modeled step length for which they fit, found
+// by bisection (the neighbour exchange paces every slab of a point to the
+// slowest workgroup).  A
 // point in one piece is a whole-lattice window; the pieces of a split point
 // are slabs of at least two planes (one with max_T = 1) and at most max_T
 // planes when max_T > 0.  Each workgroup's waves go to its slabs in
@@ -507,17 +563,19 @@ bool plan_resident(const cvf_sim* s, const Group& g, int nwg, uint32_t wpg, uint
                    RDeal& out) {
   if (s->sched != CVF_SCHED_REFERENCE || s->nm != 1 || nwg <= 0 || wpg == 0) return false;
   std::vector<int> order = g.points;
-  double total = 0, biggest = 0;
-  for (int p : order) {
+  for (int p : order)
     if (!s->ph[p].prm.flag_chessboard) return false;
-    total += (double)s->ph[p].N;
-    biggest = std::max(biggest, (double)s->ph[p].N);
-  }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return s->ph[a].L > s->ph[b].L; });
   const uint32_t min_T = max_T == 1 ? 1u : 2u;
   std::vector<Piece> pc, trial;
-  double lo = 0.99 * total / ((double)nwg * wpg), hi = biggest;
+  double lo = 0, hi = 1e300;
   if (!fill_workgroups(s, order, hi, nwg, wpg, cap, min_T, max_T, pc)) return false;
+  hi = 1;  // the smallest power of two bound that fits
+  while (!fill_workgroups(s, order, hi, nwg, wpg, cap, min_T, max_T, trial)) {
+    lo = hi;
+    hi *= 2;
+  }
+  pc.swap(trial);
   for (int it = 0; it < 30 && hi - lo > 0.002 * hi; ++it) {
     const double mid = 0.5 * (lo + hi);
     if (fill_workgroups(s, order, mid, nwg, wpg, cap, min_T, max_T, trial)) {

@@ -41,12 +41,15 @@ def same(a, b, what):
     assert bad_s.size == 0 and bad_e.size == 0, f"{what}: sigma differs at {bad_s[:8]}, eta at {bad_e[:8]}"
 
 
-@pytest.mark.parametrize("L,max_T", [(4, 0), (8, 0), (12, 0), (16, 0), (32, 0), (8, 1), (32, 1), (24, 5)])
+@pytest.mark.parametrize("L,max_T", [(4, 0), (8, 0), (12, 0), (16, 0), (32, 0), (8, 1), (32, 1), (24, 5), (8, 5),
+                                     (16, 5)])
 def test_single_point_matches_oracle_every_step(L, max_T):
-    """One point: thin slabs spread over many workgroups (two planes each by
-    default; one plane each with slab_planes = 1, a plane both the top and
-    the bottom of its slab; at most five planes, slabs of unequal
-    thickness), each exchanging with its neighbours every step."""
+    """One point under the automatic plan (a whole lattice in one window
+    for L <= 12, two 8-plane slabs at L = 16, sixteen 2-plane slabs at
+    L = 32), one-plane slabs (slab_planes = 1: a plane is both the top and
+    the bottom of its slab), and slabs of at most five planes (unequal
+    thickness: 3 + 5 planes at L = 8, five slabs of 2-5 planes in three
+    workgroups at L = 16), each exchanging with its neighbours every step."""
     p = points([L])[0]
     sim = resident([p], slab_planes=max_T)
     T = sim.plan(0)["slab_planes"]
