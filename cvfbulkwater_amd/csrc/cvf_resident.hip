@@ -289,8 +289,22 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
           const uint32_t* hi = rings + F(ring_hi) + prev * LL;
           const uint32_t* lo = rings + F(ring_lo) + (2 * kRingHalf + prev) * LL;
           uint32_t* D = lds + F(lds);
-          for (uint32_t i = li; i < 2 * LL; i += 64 * F(nw))
-            D[i < LL ? (T + 1) * LL + i : i - LL] = ring_load(i < LL ? hi + i : lo + (i - LL));
+          // eight loads in flight per lane (an index past the end re-reads
+          // the last word and is not stored)
+          const uint32_t n2 = 2 * LL, stride = 64 * F(nw);
+          for (uint32_t i0 = li; i0 < n2; i0 += 8 * stride) {
+            uint32_t v[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+              const uint32_t i = min(i0 + u * stride, n2 - 1);
+              v[u] = ring_load(i < LL ? hi + i : lo + (i - LL));
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+              const uint32_t i = i0 + u * stride;
+              if (i < n2) D[i < LL ? (T + 1) * LL + i : i - LL] = v[u];
+            }
+          }
         }
       }
     }
@@ -356,6 +370,36 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
           const int dz = arm == 4 ? (int)LL : -(int)LL;
           const uint32_t nint = bactive ? npl - 1 : npl;
           const uint32_t zi0 = first + (bactive && zb == 0 ? 2u : 0u);
+          // the neighbour's plane as published after its rd-th z half-sweep
+          // of this step (rd = 0: the ghost, or the wrapped plane of a
+          // whole-lattice window): its version is awaited and the lane's
+          // first boundary quad's four neighbour words are loaded before
+          // the interior planes, so the ring's latency overlaps them
+          const uint32_t rd = (sk >> 4) & 0xFFu;
+          const uint32_t stride = 64 * F(nw), bit = nint * Q4;
+          const uint32_t fb = li >= bit ? li : li + (bit - li + stride - 1) / stride * stride;
+          const bool hasb = bactive && fb < bit + Q4;
+          const uint32_t* RG = nullptr;
+          uint32_t pre[4] = {0, 0, 0, 0};
+          if (rd && __ballot(hasb) != 0) {
+            bool ok = true;
+            if (lane == 0)
+              ok = wait_version(ver + 2 * (arm == 4 ? F(above) : F(below)) + (arm == 4 ? 0 : 1),
+                                ring_version(step, rd - 1), status);
+            if (!uni(ok ? 1u : 0u)) {
+              abort_s = 1;
+            } else {
+              RG = arm == 4 ? rings + F(ring_hi) + (half + rd - 1) * LL
+                            : rings + F(ring_lo) + (2 * kRingHalf + half + rd - 1) * LL;
+              if (hasb) {
+                const uint32_t ub = fb - bit, y = fdiv(ub, dH), x = ub - y * hL, o = x + L * y, dy = hL * L;
+                pre[0] = ring_load(RG + o);
+                pre[1] = ring_load(RG + o + dy);
+                pre[2] = ring_load(RG + o + hL);
+                pre[3] = ring_load(RG + o + hL + dy);
+              }
+            }
+          }
           Walk it = walk_start(li, Q4, F(mP4));
           for (; it.pi < nint; advance(it, Q4, dp4, du4)) {
             const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
@@ -373,49 +417,30 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
 #pragma unroll
             for (int m = 0; m < 4; ++m) lds[cell[m]] = w[m];
           }
-          if (bactive) {
-            // the neighbour's plane as published after its rd-th z
-            // half-sweep of this step (rd = 0: the ghost, or the wrapped
-            // plane of a whole-lattice window); the new words also go to
-            // this slab's ring slot wr
-            const uint32_t rd = (sk >> 4) & 0xFFu;
+          if (bactive && (!rd || RG)) {
+            // the boundary plane, as plane nint of the same walk; the new
+            // words also go to this slab's ring slot wr
             const int dzb = ns > 1 ? dz : (arm == 4 ? -(int)((T - 1) * LL) : (int)((T - 1) * LL));
-            const bool mine = __ballot(it.pi == nint) != 0;
-            const uint32_t* RG = nullptr;
-            bool ok = true;
-            if (rd && mine) {
-              if (lane == 0)
-                ok = wait_version(ver + 2 * (arm == 4 ? F(above) : F(below)) + (arm == 4 ? 0 : 1),
-                                  ring_version(step, rd - 1), status);
-              if (!uni(ok ? 1u : 0u)) {
-                ok = false;
-                abort_s = 1;
-              } else {
-                RG = arm == 4 ? rings + F(ring_hi) + (half + rd - 1) * LL
-                              : rings + F(ring_lo) + (2 * kRingHalf + half + rd - 1) * LL;
-              }
-            }
             uint32_t* RW = wr ? rings + F(ring) + ((arm == 4 ? 2 * kRingHalf : 0u) + half + wr - 1) * LL : nullptr;
             const uint32_t pl = lbase + (zb + gofs) * LL, dy = hL * L;
-            if (ok)
-              for (; it.pi == nint; advance(it, Q4, dp4, du4)) {
-                const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
-                const uint32_t o2 = x + L * y;
-                const uint32_t off[4] = {o2, o2 + dy, o2 + hL, o2 + hL + dy};
-                uint32_t w[4], wn[4];
+            for (bool pf = true; it.pi == nint; advance(it, Q4, dp4, du4), pf = false) {
+              const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
+              const uint32_t o2 = x + L * y;
+              const uint32_t off[4] = {o2, o2 + dy, o2 + hL, o2 + hL + dy};
+              uint32_t w[4], wn[4];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                  w[m] = lds[pl + off[m]];
-                  wn[m] = RG ? ring_load(RG + off[m]) : lds[pl + off[m] + dzb];
-                }
-                const U4 rq = draw_k(seed, kStSigma, o2 + LL * (z0 + zb), step, k);
-                sigma_quad_regs(w, wn, thr, arm, rq);
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                  lds[pl + off[m]] = w[m];
-                  if (RW) ring_store(RW + off[m], w[m]);
-                }
+              for (int m = 0; m < 4; ++m) {
+                w[m] = lds[pl + off[m]];
+                wn[m] = RG ? (pf ? pre[m] : ring_load(RG + off[m])) : lds[pl + off[m] + dzb];
               }
+              const U4 rq = draw_k(seed, kStSigma, o2 + LL * (z0 + zb), step, k);
+              sigma_quad_regs(w, wn, thr, arm, rq);
+#pragma unroll
+              for (int m = 0; m < 4; ++m) {
+                lds[pl + off[m]] = w[m];
+                if (RW) ring_store(RW + off[m], w[m]);
+              }
+            }
             if (RW) ring_stores_done();
           }
         }
