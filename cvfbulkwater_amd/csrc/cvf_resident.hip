@@ -255,7 +255,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     if (has) {
       const uint32_t L = F(L), z0 = F(z0), T = F(T);
       const uint64_t seed = ((uint64_t)F(seed_hi) << 32) | F(seed_lo);
-      uint32_t schv = 0, rdv = 0, wrv = 0;
+      uint32_t schv = 0;
       if (lane < 12) {
         const U4 r = draw(seed, kStSched, 0, step, lane);
         schv = uniform_int(r.x, 6) | ((r.y >> 31) << 3);
@@ -264,18 +264,45 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
       const bool in = lane < 12 && F(ns) > 1;
       const uint32_t p_bot = parity_of((int)z0, L), p_top = parity_of((int)(z0 + T - 1), L);
       const uint32_t p_abv = parity_of((int)(z0 + T), L), p_blw = parity_of((int)z0 - 1, L);
-      const uint64_t before = (1ull << lane) - 1;  // lanes before this one
+      const uint64_t before = (1ull << lane) - 1;                    // lanes before this one
+      const uint64_t after = lane < 63 ? ~((2ull << lane) - 1) : 0ull;  // lanes after it
+      // the half-sweeps that move this slab's top (up) and bottom (dn)
+      // plane, and those that move the neighbours' facing planes: the
+      // slab above's bottom (ab), the slab below's top (bl)
       const uint64_t up = __ballot(in && arm == 4 && q == p_top), dn = __ballot(in && arm == 5 && q == p_bot);
       const uint64_t ab = __ballot(in && arm == 5 && q == p_abv), bl = __ballot(in && arm == 4 && q == p_blw);
-      if ((up >> lane) & 1) {
-        rdv = __popcll(ab & before);
-        wrv = __popcll(up & before) + 1;
-      } else if ((dn >> lane) & 1) {
-        rdv = __popcll(bl & before);
-        wrv = __popcll(dn & before) + 1;
+      // A half-sweep that moves a boundary plane reads the neighbour's
+      // plane as of the neighbour's latest such half-sweep (rd = its index
+      // + 1; 0: none yet, the ghost holds it), and stores its own moved
+      // plane into ring slot k (wr) only if the neighbour reads it: its
+      // next reading half-sweep comes before this plane moves again.  The
+      // version then goes out after the barrier that ends the half-sweep
+      // just before that reader (due), when the stores have long
+      // completed; each publish is out before the phase that waits for it.
+      uint32_t rd = 0, wr = 0, due = 0xFFu;
+      const bool mine_up = (up >> lane) & 1, mine_dn = (dn >> lane) & 1;
+      if (mine_up || mine_dn) {
+        const uint64_t other = mine_up ? ab : bl, self = mine_up ? up : dn;
+        if (other & before) rd = 64 - (uint32_t)__builtin_clzll(other & before);
+        const uint64_t ra = other & after, ua = self & after;
+        if (ra && (!ua || __builtin_ctzll(ra) < __builtin_ctzll(ua))) {
+          wr = 1;
+          due = (uint32_t)__builtin_ctzll(ra) - 1;
+        }
+      }
+      uint32_t due_top = 0, due_bot = 0;  // 1 + the half-sweep whose publish is due after this one
+      for (uint32_t j = 0; j < 12; ++j) {
+        const uint32_t dj = __builtin_amdgcn_readlane(due, j), aj = __builtin_amdgcn_readlane(arm, j);
+        if (dj == lane) {
+          if (aj == 4)
+            due_top = j + 1;
+          else
+            due_bot = j + 1;
+        }
       }
       // the slab's first wave keeps them for every wave of the slab
-      if (wave == F(w0) && lane < 12) sch_s[gs][lane] = schv | (rdv << 4) | (wrv << 12);
+      if (wave == F(w0) && lane < 12)
+        sch_s[gs][lane] = schv | (rd << 4) | (wr << 8) | (due_top << 12) | (due_bot << 16);
       if (si > 0 && F(ns) > 1) {
         // the ghosts: the neighbours' boundary planes at the end of the last step
         const uint32_t LL = L * L;
@@ -342,7 +369,8 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     // ----------------------------------------------------- 12 half-sweeps
     for (uint32_t k = 0; k < 12; ++k) {
       const uint32_t sk = has ? uni(sch_s[gs][k]) : 0u;
-      const uint32_t wr = (sk >> 12) & 0xFFu;
+      const bool wr = (sk >> 8) & 1u;
+      const uint32_t due_top = (sk >> 12) & 0xFu, due_bot = (sk >> 16) & 0xFu;
       const int arm = (int)(sk & 7u);
       const uint32_t q = ((sk >> 3) & 1u) ? 0u : 1u;
       const int ax = arm >> 1;
@@ -375,7 +403,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
           // whole-lattice window): its version is awaited and the lane's
           // first boundary quad's four neighbour words are loaded before
           // the interior planes, so the ring's latency overlaps them
-          const uint32_t rd = (sk >> 4) & 0xFFu;
+          const uint32_t rd = (sk >> 4) & 0xFu;
           const uint32_t stride = 64 * F(nw), bit = nint * Q4;
           const uint32_t fb = li >= bit ? li : li + (bit - li + stride - 1) / stride * stride;
           const bool hasb = bactive && fb < bit + Q4;
@@ -421,7 +449,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
             // the boundary plane, as plane nint of the same walk; the new
             // words also go to this slab's ring slot wr
             const int dzb = ns > 1 ? dz : (arm == 4 ? -(int)((T - 1) * LL) : (int)((T - 1) * LL));
-            uint32_t* RW = wr ? rings + F(ring) + ((arm == 4 ? 2 * kRingHalf : 0u) + half + wr - 1) * LL : nullptr;
+            uint32_t* RW = wr ? rings + F(ring) + ((arm == 4 ? 2 * kRingHalf : 0u) + half + k) * LL : nullptr;
             const uint32_t pl = lbase + (zb + gofs) * LL, dy = hL * L;
             for (bool pf = true; it.pi == nint; advance(it, Q4, dp4, du4), pf = false) {
               const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
@@ -441,13 +469,17 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
                 if (RW) ring_store(RW + off[m], w[m]);
               }
             }
-            if (RW) ring_stores_done();
           }
         }
       }
+      // a publish due after this half-sweep: the slab's stores of its slot
+      // (made in this or an earlier half-sweep) complete before the barrier
+      if (run && (due_top || due_bot)) ring_stores_done();
       __syncthreads();
-      if (has && wave == F(w0) && lane == 0 && wr && !abort_s)
-        publish(ver + 2 * (G.first + gs) + (arm == 4 ? 1 : 0), ring_version(step, wr - 1));
+      if (has && wave == F(w0) && lane == 0 && !abort_s) {
+        if (due_top) publish(ver + 2 * (G.first + gs) + 1, ring_version(step, due_top - 1));
+        if (due_bot) publish(ver + 2 * (G.first + gs), ring_version(step, due_bot - 1));
+      }
     }
     // ------------------------------------- boundary planes at step end
     if (has && F(ns) > 1 && !abort_s) {
