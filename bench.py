@@ -196,7 +196,8 @@ def calibrate_copy(mib: float, reps: int = 10) -> int:
 
 def tuning_signature(kname: str, t: dict) -> str:
     if kname == "k_resident":
-        return f"per_cu:{t['streams']},block:{t['block_chunk']},fused:2,graph:{t['graph_steps']}"
+        # instructions per step do not depend on the graph length
+        return f"per_cu:{t['streams']},block:{t['block_chunk']},fused:2"
     if kname == "k_chunk":
         whole = ",chunk:12" if t.get("chunk_sweeps") == 12 else ""
         return f"streams:{t['streams']},block:{t['block_chunk']},fused:1,slab:{t['slab_planes']}{whole}"
@@ -263,6 +264,11 @@ def precision_form(steps: int = 5000, interval: int = 10, ev: HipEvents | None =
             rows.append((o.E, o.H, o.V, N / o.V, o.n_hb))
         if ev is not None:
             ms[name] = time_steps(sim, 200, 0, ev)
+            if name == "fp32c":
+                # kernel classes of a step (HIP events around each launch)
+                prof = sim.profile_steps(50)
+                kernels = {k: {"ms_per_launch": v[0], "launches_per_step": v[1] / 50}
+                           for k, v in prof.items() if v[1]}
         sim.close()
         series[name] = np.array(rows)
     a, b = series["fp32c"], series["fp64"]
@@ -280,6 +286,7 @@ def precision_form(steps: int = 5000, interval: int = 10, ev: HipEvents | None =
                        "FP64 everywhere, same seed", "errors": errors}
     if ms:
         out["ms_per_step"] = ms
+        out["kernels_fp32c"] = kernels
     return out
 
 
