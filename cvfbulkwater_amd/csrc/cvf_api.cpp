@@ -162,6 +162,7 @@ struct cvf_sim {
   DevBuf<uint8_t> cov;
   DevBuf<int32_t> csize, slot, cclusters;  // cluster sizes by label, percolation slots, compacted sizes
   DevBuf<uint32_t> cblock;                  // per-workgroup cluster counts of the compaction, then the total
+  DevBuf<uint64_t> cmom;                    // per-workgroup size moments, then their total (4 words each)
   DevBuf<PointDev> pts;
   DevBuf<PointState> pst;
   DevBuf<float> dist;
@@ -1325,6 +1326,7 @@ int create_impl(const cvf_params* pts, int32_t n, cvf_sim** out) {
   HIPC(s->slot.alloc(6 * (size_t)maxN));
   HIPC(s->cclusters.alloc(6 * (size_t)maxN));
   HIPC(s->cblock.alloc(compact_blocks(6 * maxN) + 1));
+  HIPC(s->cmom.alloc(4 * ((size_t)compact_blocks(6 * maxN) + 1)));
   s->max_slots = 3 * maxL * maxL + 1;  // a cluster of >= 2L nodes: at most 6N/(2L)
   HIPC(s->cover.alloc((size_t)s->max_slots * 3 * ((maxL + 31) / 32)));
   HIPC(s->nslot_nperc.alloc(2));
@@ -1681,25 +1683,47 @@ int cvf_cluster_stats(cvf_sim* s, int32_t pt, cvf_cstat* c) {
   const uint32_t* label = s->parent.p + P.loff;
   const int32_t minsize = 2 * (int32_t)L;
   const uint32_t nw = (L + 31) / 32;
-  launch_zero_u32(s->stream, (uint32_t*)s->csize.p, nn);
-  launch_zero_u32(s->stream, s->nslot_nperc.p, 2);
-  launch_zero_u32(s->stream, s->cover.p, (size_t)s->max_slots * 3 * nw);
+  launch_zero3_u32(s->stream, (uint32_t*)s->csize.p, nn, s->nslot_nperc.p, 2, s->cover.p,
+                   (size_t)s->max_slots * 3 * nw);
   launch_cluster_sizes(s->stream, label, nn, s->csize.p);
   launch_perc_slots(s->stream, label, nn, s->csize.p, minsize, s->slot.p, s->nslot_nperc.p,
                     s->max_slots);
   launch_perc_cover(s->stream, label, nn, L, s->csize.p, minsize, s->slot.p, s->cover.p, s->max_slots);
   launch_perc_count(s->stream, s->cover.p, s->nslot_nperc.p, L, s->max_slots, s->nslot_nperc.p + 1);
-  // the clusters' sizes only, still in label order
+  // the clusters' sizes in label order, and their moments
   const uint32_t nb = compact_blocks(nn);
-  launch_compact_sizes(s->stream, s->csize.p, nn, s->cblock.p, s->cblock.p + nb, s->cclusters.p);
+  launch_compact_count(s->stream, s->csize.p, nn, s->cblock.p, s->cblock.p + nb, s->cmom.p);
   uint32_t np[2], ncl = 0;
+  uint64_t mom[4];
   HIPC(hipMemcpyAsync(np, s->nslot_nperc.p, 8, hipMemcpyDeviceToHost, s->stream));
   HIPC(hipMemcpyAsync(&ncl, s->cblock.p + nb, 4, hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipMemcpyAsync(mom, s->cmom.p + 4 * (size_t)nb, sizeof mom, hipMemcpyDeviceToHost, s->stream));
   HIPC(hipStreamSynchronize(s->stream));
   if (np[0] > s->max_slots) return fail(CVF_EDEVICE, "percolation slot overflow");
-  if (ncl > nn) return fail(CVF_EDEVICE, "cluster compaction overflow");
+  if (ncl == 0 || ncl > nn) return fail(CVF_EDEVICE, "cluster compaction overflow");
+  constexpr uint64_t kExact = 1ull << 24;  // every integer up to 2^24 is a float
+  if (mom[0] <= kExact && mom[1] <= kExact) {
+    // print_cluster_statistics (gpu_water3D.cu:2691-2718) adds the sizes and
+    // their squares as floats in label order: with every partial sum an
+    // integer up to 2^24 (and every size below 4097, so no int32 square
+    // wraps) each addition is exact, and the sums are these integers
+    float average = (float)mom[0], average2 = (float)mom[1];
+    average /= (float)ncl;
+    average2 /= (float)ncl;
+    average2 -= average * average;
+    c->largest = (int32_t)(uint32_t)mom[2];
+    c->second = (int32_t)(uint32_t)(mom[2] >> 32);
+    c->n_perc = (int32_t)np[1];
+    c->n_clusters = (int32_t)ncl;
+    c->mean = average;
+    c->var = average2;
+    return CVF_OK;
+  }
+  // larger sums round: the host adds the sizes in label order
+  launch_compact_write(s->stream, s->csize.p, nn, s->cblock.p, s->cclusters.p);
   std::vector<int32_t> size(ncl);
-  HIPC(hipMemcpy(size.data(), s->cclusters.p, (size_t)ncl * 4, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpyAsync(size.data(), s->cclusters.p, (size_t)ncl * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPC(hipStreamSynchronize(s->stream));
   assemble_cluster_row(size.data(), ncl, (int32_t)np[1], c);
   return CVF_OK;
 }

@@ -86,6 +86,7 @@ void launch_chunk(hipStream_t st, Launch ln, const PointDev* pts, const uint32_t
 // largest dynamic LDS a fused launch may use
 uint32_t chunk_lds_limit();
 void launch_zero_u32(hipStream_t st, uint32_t* p, size_t n);
+void launch_zero3_u32(hipStream_t st, uint32_t* a, size_t na, uint32_t* b, size_t nb, uint32_t* c, size_t nc);
 void launch_observe(hipStream_t st, Launch ln, const uint32_t* words, unsigned long long* counters);
 void launch_sw_local(hipStream_t st, Launch ln, const uint32_t* words, uint32_t* parent, uint8_t* cov,
                      uint32_t jc);
@@ -102,12 +103,16 @@ void launch_sigma_hist_slot(hipStream_t st, const uint32_t* W, uint32_t N, unsig
                             const uint64_t* gstep, uint32_t step_off, const uint64_t* base, uint32_t span);
 // cluster statistics of one point (labels at `label`, 6N nodes)
 void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, int32_t* size);
-// the non-zero cluster sizes of size[0..nn) in label order into out[0..*total)
-// (bcount: compact_blocks(nn) scratch words)
+// Cluster sizes size[0..nn) (zero: no cluster): the count into *total and
+// each workgroup's offset into bcount (compact_blocks(nn) words), and the
+// sizes' moments bmom[4 nb .. 4 nb + 3] = sum, sum of squares, largest |
+// second largest << 32 (bmom: 4 (compact_blocks(nn) + 1) words); then, with
+// those offsets, the non-zero sizes in label order into out[0..*total)
 constexpr uint32_t kCompactSpan = 1024;
 uint32_t compact_blocks(uint32_t nn);
-void launch_compact_sizes(hipStream_t st, const int32_t* size, uint32_t nn, uint32_t* bcount, uint32_t* total,
-                          int32_t* out);
+void launch_compact_count(hipStream_t st, const int32_t* size, uint32_t nn, uint32_t* bcount, uint32_t* total,
+                          uint64_t* bmom);
+void launch_compact_write(hipStream_t st, const int32_t* size, uint32_t nn, const uint32_t* bcount, int32_t* out);
 void launch_perc_slots(hipStream_t st, const uint32_t* label, uint32_t nn, const int32_t* size,
                        int32_t minsize, int32_t* slot, uint32_t* nslot, uint32_t max_slots);
 void launch_perc_cover(hipStream_t st, const uint32_t* label, uint32_t nn, uint32_t L,

@@ -677,6 +677,19 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
     p[i] = 0;
 }
 
+// three arrays zeroed by one launch (a, then b, then c in one index range)
+__global__ void k_zero3_u32(uint32_t* a, size_t na, uint32_t* b, size_t nb, uint32_t* c, size_t nc) {
+  const size_t n = na + nb + nc;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    if (i < na)
+      a[i] = 0;
+    else if (i < na + nb)
+      b[i - na] = 0;
+    else
+      c[i - na - nb] = 0;
+  }
+}
+
 // One workgroup of 256 threads per kObserveSpan cells of a point: each
 // thread sums kObserveSpan / 256 cells, the workgroup reduces through LDS,
 // and one pair of 64-bit atomics per workgroup adds to the point's totals
@@ -980,26 +993,94 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* sh,
   return before + x - v;
 }
 
+// Moments of the cluster sizes (count, sum, sum of squares, the largest two
+// with repetition), reduced per workgroup and then over the workgroups.
+// While every partial sum stays below 2^24 the reference's float
+// accumulation of the sizes and of their squares in label order is exact
+// (integers below 2^24 are floats), so these integers give its cluster.out
+// row without the host loop.
+struct Mom {
+  uint64_t sum, sq;
+  uint32_t m1, m2;  // m1 >= m2
+};
+__device__ __forceinline__ void mom_add(Mom& m, uint32_t v) {
+  m.sum += v;
+  m.sq += (uint64_t)v * v;
+  if (v > m.m1) {
+    m.m2 = m.m1;
+    m.m1 = v;
+  } else if (v > m.m2) {
+    m.m2 = v;
+  }
+}
+__device__ __forceinline__ void mom_merge(Mom& m, uint64_t sum, uint64_t sq, uint32_t a1, uint32_t a2) {
+  m.sum += sum;
+  m.sq += sq;
+  const uint32_t n1 = max(m.m1, a1), n2 = max(min(m.m1, a1), max(m.m2, a2));
+  m.m1 = n1;
+  m.m2 = n2;
+}
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int d) {
+  const uint32_t lo = __shfl_xor((uint32_t)v, d, 64), hi = __shfl_xor((uint32_t)(v >> 32), d, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+// the workgroup's moments into out[0..3] (sum, sum of squares, m1 | m2 << 32)
+// by thread 0; sh: 4 x 4 words of LDS
+__device__ __forceinline__ void mom_block_store(Mom m, uint64_t* sh, uint64_t* out) {
+  for (int d = 32; d >= 1; d >>= 1) {
+    const uint64_t s2 = shfl_xor_u64(m.sum, d), q2 = shfl_xor_u64(m.sq, d);
+    const uint32_t a1 = __shfl_xor(m.m1, d, 64), a2 = __shfl_xor(m.m2, d, 64);
+    mom_merge(m, s2, q2, a1, a2);
+  }
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[4 * w] = m.sum;
+    sh[4 * w + 1] = m.sq;
+    sh[4 * w + 2] = m.m1 | ((uint64_t)m.m2 << 32);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Mom t{0, 0, 0, 0};
+    for (uint32_t k = 0; k < blockDim.x / 64; ++k)
+      mom_merge(t, sh[4 * k], sh[4 * k + 1], (uint32_t)sh[4 * k + 2], (uint32_t)(sh[4 * k + 2] >> 32));
+    out[0] = t.sum;
+    out[1] = t.sq;
+    out[2] = t.m1 | ((uint64_t)t.m2 << 32);
+    out[3] = 0;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_compact_count(const int32_t* __restrict__ size, uint32_t nn,
-                                                       uint32_t* __restrict__ bcount) {
+                                                       uint32_t* __restrict__ bcount, uint64_t* __restrict__ bmom) {
   __shared__ uint32_t sh[4];
+  __shared__ uint64_t shm[16];
   const uint32_t base = blockIdx.x * kCompactSpan + threadIdx.x * 4;
   uint32_t c = 0;
+  Mom m{0, 0, 0, 0};
 #pragma unroll
-  for (uint32_t k = 0; k < 4; ++k) c += base + k < nn && size[base + k] != 0;
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t v = base + k < nn ? (uint32_t)size[base + k] : 0u;
+    c += v != 0;
+    mom_add(m, v);
+  }
   uint32_t total;
   block_excl_scan256(c, sh, &total);
   if (threadIdx.x == 0) bcount[blockIdx.x] = total;
+  mom_block_store(m, shm, bmom + 4 * blockIdx.x);
 }
 
-// one workgroup: bcount[0..nb) -> exclusive offsets, the sum into *total
+// one workgroup: bcount[0..nb) -> exclusive offsets, the sum into *total;
+// the workgroups' moments bmom[0..nb) into bmom[nb]
 __global__ __launch_bounds__(256) void k_compact_scan(uint32_t* __restrict__ bcount, uint32_t nb,
-                                                      uint32_t* __restrict__ total) {
+                                                      uint32_t* __restrict__ total, uint64_t* __restrict__ bmom) {
   __shared__ uint32_t sh[4];
+  __shared__ uint64_t shm[16];
   uint32_t run = 0;
+  Mom m{0, 0, 0, 0};
   for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
     const uint32_t i = b0 + threadIdx.x;
     const uint32_t v = i < nb ? bcount[i] : 0u;
+    if (i < nb) mom_merge(m, bmom[4 * i], bmom[4 * i + 1], (uint32_t)bmom[4 * i + 2], (uint32_t)(bmom[4 * i + 2] >> 32));
     uint32_t sum;
     const uint32_t ex = block_excl_scan256(v, sh, &sum);
     if (i < nb) bcount[i] = run + ex;
@@ -1007,6 +1088,7 @@ __global__ __launch_bounds__(256) void k_compact_scan(uint32_t* __restrict__ bco
     __syncthreads();  // sh is reused by the next round
   }
   if (threadIdx.x == 0) *total = run;
+  mom_block_store(m, shm, bmom + 4 * nb);
 }
 
 __global__ __launch_bounds__(256) void k_compact_write(const int32_t* __restrict__ size, uint32_t nn,
@@ -1280,6 +1362,12 @@ void launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
   if (blocks == 0) blocks = 1;
   launch(k_zero_u32, dim3((unsigned)blocks), dim3(256), st, p, n);
 }
+void launch_zero3_u32(hipStream_t st, uint32_t* a, size_t na, uint32_t* b, size_t nb, uint32_t* c, size_t nc) {
+  size_t blocks = (na + nb + nc + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks == 0) blocks = 1;
+  launch(k_zero3_u32, dim3((unsigned)blocks), dim3(256), st, a, na, b, nb, c, nc);
+}
 void launch_observe(hipStream_t st, Launch ln, const uint32_t* words, unsigned long long* counters) {
   launch(k_observe, dim3(ln.nblocks), dim3(ln.block), st, ln.tab, ln.nblocks, words, counters);
 }
@@ -1309,12 +1397,14 @@ void launch_cluster_sizes(hipStream_t st, const uint32_t* label, uint32_t nn, in
   launch(k_cluster_sizes, grid1(nn, 256), dim3(256), st, label, nn, size);
 }
 uint32_t compact_blocks(uint32_t nn) { return (nn + kCompactSpan - 1) / kCompactSpan; }
-void launch_compact_sizes(hipStream_t st, const int32_t* size, uint32_t nn, uint32_t* bcount, uint32_t* total,
-                          int32_t* out) {
+void launch_compact_count(hipStream_t st, const int32_t* size, uint32_t nn, uint32_t* bcount, uint32_t* total,
+                          uint64_t* bmom) {
   const uint32_t nb = compact_blocks(nn);
-  launch(k_compact_count, dim3(nb), dim3(256), st, size, nn, bcount);
-  launch(k_compact_scan, dim3(1), dim3(256), st, bcount, nb, total);
-  launch(k_compact_write, dim3(nb), dim3(256), st, size, nn, (const uint32_t*)bcount, out);
+  launch(k_compact_count, dim3(nb), dim3(256), st, size, nn, bcount, bmom);
+  launch(k_compact_scan, dim3(1), dim3(256), st, bcount, nb, total, bmom);
+}
+void launch_compact_write(hipStream_t st, const int32_t* size, uint32_t nn, const uint32_t* bcount, int32_t* out) {
+  launch(k_compact_write, dim3(compact_blocks(nn)), dim3(256), st, size, nn, bcount, out);
 }
 void launch_perc_slots(hipStream_t st, const uint32_t* label, uint32_t nn, const int32_t* size,
                        int32_t minsize, int32_t* slot, uint32_t* nslot, uint32_t max_slots) {

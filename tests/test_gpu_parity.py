@@ -93,6 +93,28 @@ def test_cluster_form_labels_every_step(L, T):
     assert_same_obs(eng, ora)
 
 
+@pytest.mark.parametrize("T,P,rounded", [(0.05, 0.3, True), (0.2, 2.0, False), (0.815, 0.45, False)])
+def test_cluster_row_exact_and_rounded_sums(T, P, rounded):
+    """cluster.out's float sums (print_cluster_statistics, gpu_water3D.cu:
+    2691-2718, sizes and squares added as floats in label order): with
+    every partial sum at most 2^24 each addition is exact and the engine
+    takes the row from device integer totals; a percolating cluster
+    (T=0.05: sum of squares about 6e8) rounds, and the host adds the sizes
+    in label order.  Both rows equal the oracle's, float bits included."""
+    eng, ora = pair(16, T=T, P=P, seed=91, nc=1)
+    for t in range(40):
+        eng.step(1)
+        ora.step(1)
+        if t % 10 == 9:
+            lab = eng.labels()
+            assert np.array_equal(lab, ora.labels()), t
+            sz = np.bincount(lab)
+            sq = int((sz[sz > 0].astype(np.int64) ** 2).sum())
+            assert (sq > 1 << 24) == rounded, (t, sq)
+            a, b = eng.cluster_stats(), ora.cluster_stats()
+            assert (a.largest, a.second, a.n_perc, a.n_clusters, a.mean, a.var) == tuple(b), (t, sq)
+
+
 def test_config3_L32_labels_every_step():
     """Config 3 shape: L=32, T=0.70, P=0.45, 1 Metropolis + 1 Swendsen-Wang
     step per MC step (gpu_water3D.cu:1444-1603, labels 2440-2529): sigma,

@@ -152,3 +152,39 @@ def test_resident_profile_and_histograms():
         same(b.get_config(i), c.get_config(i), f"record point {i}")
     s, _ = c.get_config(1)
     assert np.array_equal(rec[-1], np.bincount(s, minlength=6))
+
+
+@pytest.mark.parametrize("block", [512, 256])
+def test_two_workgroups_per_cu(block):
+    """streams = 2 under the resident kernel: two workgroups per CU, each
+    with half the LDS, so slabs are thinner and more of them exchange;
+    lattices equal to each point's default-plan run."""
+    pts = points([16, 24, 32, 48] * 4)
+    sim = resident(pts, streams=2, block_chunk=block, graph_steps=16)
+    sim.step(21)
+    for i in (0, 3, 7, 11, 15):
+        single = Simulation(pts[i])
+        single.step(21)
+        same(sim.get_config(i), single.get_config(), f"point {i} L={pts[i].L}")
+        assert sim.volume(i) == single.volume(), i
+        single.close()
+
+
+@pytest.mark.parametrize("max_T", [0, 1])
+def test_large_lattice_L96(max_T):
+    """One L=96 point (an L^2 plane is 36 KB of LDS): 2-plane slabs on 48
+    workgroups, or one-plane slabs on 96, against the unfused kernels for
+    8 steps (the oracle takes the point at its first step)."""
+    p = points([96])[0]
+    sim = resident([p], slab_planes=max_T)
+    ref = Simulation(p)
+    ref.tuning = {"fused": 0}
+    ora = O.OracleSim(L=96, T=p.T, P=p.P, seed=p.seed)
+    sim.step(1)
+    ref.step(1)
+    ora.step(1)
+    same(sim.get_config(), ora.get_config(), "after step 1 (oracle)")
+    sim.step(7)
+    ref.step(7)
+    same(sim.get_config(), ref.get_config(), "after 8 steps (unfused)")
+    assert sim.volume() == ref.volume()
