@@ -1239,9 +1239,20 @@ cvf_tuning default_tuning(const cvf_sim* s) {
   // small handles are bound by per-launch cost: a whole Metropolis pass per
   // fused launch where the window fits (not with the non-chessboard rule,
   // whose 24-plane halo reserve costs more than the launch it saves)
-  bool any_nc = false;
-  for (const auto& P : s->ph) any_nc = any_nc || !P.prm.flag_chessboard;
+  bool any_nc = false, all_wide = true;
+  for (const auto& P : s->ph) {
+    any_nc = any_nc || !P.prm.flag_chessboard;
+    all_wide = all_wide && P.L > 64;
+  }
   t.chunk_sweeps = s->total_cells < (1u << 20) && !any_nc ? kMaxChunk : kChunk;
+  if (all_wide) {
+    // no fused window fits a plane beyond L = 64: one-plane resident slabs,
+    // one workgroup per CU (49.7 against 59.8 us per step at L = 96 for the
+    // unfused kernels); points that cannot run resident fall back to those
+    t.fused = 2;
+    t.slab_planes = 1;
+    t.block_chunk = 1024;
+  }
   return t;
 }
 
@@ -2033,16 +2044,20 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
         }
     if (streams <= 2)
       for (int bc : {256, 512, 1024})
-        for (int gs : {16, 64}) {
-          // resident kernel: one stream group, `streams` workgroups per CU
-          cvf_tuning t = base;
-          t.streams = streams;
-          t.fused = 2;
-          t.block_chunk = bc;
-          t.graph_steps = gs;
-          t.chunk_sweeps = kChunk;
-          cand.push_back(t);
-        }
+        for (int gs : {16, 64})
+          for (int slab : {0, 1}) {
+            // resident kernel: one stream group, `streams` workgroups per CU;
+            // automatic slabs, or one-plane slabs (wide lattices) at 1024
+            if (slab && bc != 1024) continue;
+            cvf_tuning t = base;
+            t.streams = streams;
+            t.fused = 2;
+            t.slab_planes = slab;
+            t.block_chunk = bc;
+            t.graph_steps = gs;
+            t.chunk_sweeps = kChunk;
+            cand.push_back(t);
+          }
     for (int bs : {64, 256}) {
       cvf_tuning t = base;
       t.streams = streams;

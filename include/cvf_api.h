@@ -156,7 +156,8 @@ typedef struct cvf_tuning {
                            2: the resident kernel (every step of a graph in one launch, the
                            lattice held in LDS, one stream group) where the points allow it
                            (reference schedule, one Metropolis sweep per step, chessboard
-                           rule), else as 1 */
+                           rule), else as 1.  The default is 1, or 2 with one-plane slabs
+                           when every point's L exceeds 64 (no fused window fits) */
   int32_t slab_planes;  /* planes per slab of the fused kernel (<= 16), or the thickest slab
                            of the resident kernel; 0 = automatic */
   int32_t block_chunk;  /* workgroup size of the fused kernel */
