@@ -62,16 +62,22 @@ __host__ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32
 #endif
 }
 
-__host__ __device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                              uint32_t k0, uint32_t k1) {
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u, kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
+
+// Rounds R0 .. 9 of Philox4x32-10 on (c0, c1, c2, c3), (k0, k1) being the
+// key of round R0.
+template <int R0>
+__host__ __device__ __forceinline__ U4 philox_rounds(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                     uint32_t k0, uint32_t k1) {
 #pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    if (r) {
-      k0 += 0x9E3779B9u;
-      k1 += 0xBB67AE85u;
+  for (int r = R0; r < 10; ++r) {
+    if (r > R0) {
+      k0 += kPhiloxW0;
+      k1 += kPhiloxW1;
     }
-    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
-    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
+    const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
     const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
     const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
     c1 = (uint32_t)p1;
@@ -80,6 +86,39 @@ __host__ __device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t
     c2 = n2;
   }
   return U4{c0, c1, c2, c3};
+}
+
+__host__ __device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1) {
+  return philox_rounds<0>(c0, c1, c2, c3, k0, k1);
+}
+
+// Philox4x32-10 for many counters that differ in word c0 only.  Neither
+// round 0's product of c2 nor round 1's product of round 0's word 0 (a
+// function of c1, c2 and the key) involves c0: philox_fixed computes them
+// once, philox_site completes one counter from them.
+// philox_site(c0, philox_fixed(c1, c2, c3, k0, k1)) == philox(c0, c1, c2, c3, k0, k1).
+struct PhiloxFixed {
+  uint32_t t0, t1, t2, t3;  // round 0: word 2's term c3 ^ k1; round 1: the terms of words 0, 2 and word 3
+  uint32_t k0, k1;          // key of round 2
+};
+__host__ __device__ __forceinline__ PhiloxFixed philox_fixed(uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                                             uint32_t k1) {
+  const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
+  const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;  // round 0, word 0
+  const uint32_t k0b = k0 + kPhiloxW0, k1b = k1 + kPhiloxW1;
+  const uint64_t q0 = (uint64_t)kPhiloxM0 * n0;        // round 1, word 0's product
+  return PhiloxFixed{c3 ^ k1, (uint32_t)p1 ^ k0b, (uint32_t)(q0 >> 32) ^ k1b, (uint32_t)q0, k0b + kPhiloxW0,
+                     k1b + kPhiloxW1};
+}
+__host__ __device__ __forceinline__ U4 philox_site(uint32_t c0, const PhiloxFixed& f, uint32_t k0, uint32_t k1) {
+  const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;                  // round 0, word 2's product
+  const uint32_t y2 = (uint32_t)(p0 >> 32) ^ f.t0, y3 = (uint32_t)p0;
+  const uint64_t p1 = (uint64_t)kPhiloxM1 * y2;                  // round 1, word 0's product
+  return philox_rounds<2>((uint32_t)(p1 >> 32) ^ f.t1, (uint32_t)p1, f.t2 ^ y3, f.t3, k0, k1);
+}
+__host__ __device__ __forceinline__ U4 philox_site(uint32_t c0, const PhiloxFixed& f) {
+  return philox_site(c0, f, f.k0, f.k1);
 }
 
 // One draw of a stream: site, step, stream id, sub-index.

@@ -15,6 +15,15 @@
 // copy of the plane below (window plane 0) and of the plane above (window
 // plane T + 1), owned by the neighbouring slabs.
 //
+// Window planes are stored by quads: quad (x, y), x, y < L/2, is the four
+// cells (x, y), (x, y + L/2), (x + L/2, y), (x + L/2, y + L/2), which share
+// one Philox block in every sigma half-sweep and move together; they are
+// words 4 (y L/2 + x) .. + 3 of the plane, in that order (half h = 2 [x
+// half] + [y half]).  A quad is one 16-byte LDS load and store, and a cube
+// item (four cubes L/2 apart in x and y) reads and writes eight whole
+// quads.  `words` keeps the reference's x + L y + L^2 z order; staging and
+// write-back convert, and the ring planes carry the window layout.
+//
 // Only z half-sweeps and the cube sweep read across a slab boundary:
 //   * a +z half-sweep (arm 4) of the top plane reads sigma_5 of the plane
 //     above, which changes only in -z half-sweeps of that plane, and a -z
@@ -123,14 +132,21 @@ __device__ __forceinline__ void sigma_quad_regs(uint32_t (&w)[4], const uint32_t
   }
 }
 
-// draw() with the key passed through an empty asm statement, so that the
-// compiler rebuilds the Philox round keys with scalar adds in each
-// iteration instead of holding all twenty in scalar registers across the
-// loop (which spills the phase's other scalars).
-__device__ __forceinline__ U4 draw_k(uint64_t seed, uint32_t stream, uint32_t site, uint64_t step, uint32_t sub) {
-  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+// The draws of one phase, draw(seed, stream, site, step, sub) for every
+// item's site: the part that does not depend on the site, once per phase
+// (in scalar registers) ...
+__device__ __forceinline__ PhiloxFixed phase_draws(uint64_t seed, uint32_t stream, uint64_t step, uint32_t sub) {
+  return philox_fixed((uint32_t)step, (stream << 24) | sub, (uint32_t)(step >> 32), (uint32_t)seed,
+                      (uint32_t)(seed >> 32));
+}
+// ... and one item's draw.  The round-2 key goes through an empty asm
+// statement, so that the compiler rebuilds the later round keys with
+// scalar adds in each iteration instead of holding all of them in scalar
+// registers across the loop (which spills the phase's other scalars).
+__device__ __forceinline__ U4 draw_site(uint32_t site, const PhiloxFixed& f) {
+  uint32_t k0 = f.k0, k1 = f.k1;
   asm volatile("" : "+s"(k0), "+s"(k1));
-  return philox(site, (uint32_t)step, (stream << 24) | sub, (uint32_t)(step >> 32), k0, k1);
+  return philox_site(site, f, k0, k1);
 }
 
 // A lane's walk over the items of one phase of its slab: item it = pi * Q +
@@ -152,35 +168,122 @@ __device__ __forceinline__ Walk walk_start(uint32_t it, uint32_t Q, uint32_t mQ)
   return Walk{pi, it - pi * Q};
 }
 
+// (a & m) | (b & ~m) in one instruction
+__device__ __forceinline__ uint32_t bitsel(uint32_t m, uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
+  return r;
+}
+
+// The four words of a quad (16-byte aligned window index w).
+__device__ __forceinline__ void quad_load(const uint32_t* lds, uint32_t w, uint32_t (&v)[4]) {
+  const uint4 t = *reinterpret_cast<const uint4*>(lds + w);
+  v[0] = t.x;
+  v[1] = t.y;
+  v[2] = t.z;
+  v[3] = t.w;
+}
+__device__ __forceinline__ void quad_store(uint32_t* lds, uint32_t w, const uint32_t (&v)[4]) {
+  *reinterpret_cast<uint4*>(lds + w) = make_uint4(v[0], v[1], v[2], v[3]);
+}
+
 // One in-plane half-sweep (arm along axis AX: 0 x, 1 y) of a slab's T
-// planes: the lane's quads from item li on, 64 nw items apart.
+// planes: the lane's quads from item li on, 64 nw items apart.  Item u of
+// a plane is the quad with origin (2 (u mod L/4) + q, u div L/4) (x arm)
+// or (u mod L/2, 2 (u div L/2) + q) (y arm); its index in the plane is
+// 2u + q, resp. u + (L/2)(u div L/2 + q).  The facing quad is the next
+// one along the axis (the previous one for a - arm); across the periodic
+// edge it is the quad at the other end, with the halves along the axis
+// swapped.
 template <int AX>
 __device__ __forceinline__ void sweep_inplane(uint32_t* lds, uint32_t li, uint32_t L, uint32_t T, uint32_t z0,
                                               uint32_t pbase, uint64_t seed, uint64_t step, uint32_t k, int arm,
                                               uint32_t q, FastDiv dH, FastDiv dQ, uint32_t mP8, uint32_t dp8,
                                               uint32_t du8, const uint32_t* thr) {
   const uint32_t hL = L / 2, qL = L / 4, LL = L * L, Q8 = LL / 8;
+  const bool up = (arm & 1) == 0;
+  const uint32_t st = AX == 0 ? 1u : hL;                    // quads per step along the axis
+  const uint32_t edge = up ? hL - 1 : 0u;                   // origin coordinate whose neighbour wraps
+  const int dn = up ? (int)st : -(int)st;                   // facing quad, no wrap
+  const int dw = up ? -(int)((hL - 1) * st) : (int)((hL - 1) * st);  // facing quad across the edge
+  const PhiloxFixed pf = phase_draws(seed, kStSigma, step, k);
   for (Walk it = walk_start(li, Q8, mP8); it.pi < T; advance(it, Q8, dp8, du8)) {
-    uint32_t x, y;
-    if (AX == 0) {  // u = hx + qL y
+    uint32_t x, y, quad;
+    if (AX == 0) {
       y = fdiv(it.u, dQ);
       x = 2 * (it.u - y * qL) + q;
-    } else {  // u = x + hL hy
+      quad = 2 * it.u + q;
+    } else {
       const uint32_t hy = fdiv(it.u, dH);
       x = it.u - hy * hL;
       y = 2 * hy + q;
+      quad = it.u + hL * (hy + q);
     }
-    uint32_t cell[4], nb[4], w[4], wn[4];
-    quad_plane_cells(pbase + it.pi * LL, L, x, y, arm, AX, cell, nb);
+    const bool wrap = (AX == 0 ? x : y) == edge;
+    const uint32_t cw = pbase + it.pi * LL + 4 * quad;
+    const uint32_t nw = cw + 4u * (uint32_t)(wrap ? dw : dn);
+    uint32_t w[4], nv[4], wn[4];
+    quad_load(lds, cw, w);
+    quad_load(lds, nw, nv);
+    // swap the halves along the axis where the facing quad wraps (a bit
+    // select, so that the four selects stay four instructions)
+    const uint32_t sel = wrap ? ~0u : 0u;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      w[m] = lds[cell[m]];
-      wn[m] = lds[nb[m]];
-    }
-    const U4 rq = draw_k(seed, kStSigma, x + L * y + LL * (z0 + it.pi), step, k);
+    for (int m = 0; m < 4; ++m) wn[m] = bitsel(sel, nv[m ^ (AX == 0 ? 2 : 1)], nv[m]);
+    const U4 rq = draw_site(x + L * y + LL * (z0 + it.pi), pf);
     sigma_quad_regs(w, wn, thr, arm, rq);
+    quad_store(lds, cw, w);
+  }
+}
+
+// Cube moves on the four cubes of a cube item in the quad layout: origins
+// (ox + a L/2, oy + b L/2), ox, oy < L/2, on bottom plane B and top plane U
+// (window indices), random words r.x .. r.w in the order (a, b) = (0,0),
+// (0,1), (1,0), (1,1), i.e. cube c = 2a + b takes half c of each corner
+// quad; a corner across the periodic edge (ox or oy = L/2 - 1) lies in
+// quad 0 along that axis with the halves swapped.  The four cubes are
+// disjoint and own all four halves of each of their eight corner quads, so
+// the quads are loaded before the first move and stored back whole after
+// the last.
+__device__ __forceinline__ void swap_halves(uint32_t (&v)[4], uint32_t sel, int bit) {
+  uint32_t t[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) lds[cell[m]] = w[m];
+  for (int h = 0; h < 4; ++h) t[h] = bitsel(sel, v[h ^ bit], v[h]);
+#pragma unroll
+  for (int h = 0; h < 4; ++h) v[h] = t[h];
+}
+__device__ __forceinline__ void eta_cube_item_q(uint32_t* lds, uint32_t B, uint32_t U, const uint64_t* thr,
+                                                const uint64_t* etab, uint32_t hL, uint32_t ox, uint32_t oy,
+                                                const U4& r) {
+  const bool ex = ox + 1 == hL, ey = oy + 1 == hL;
+  const uint32_t x1 = ex ? 0u : ox + 1, y1 = ey ? 0u : oy + 1;
+  const uint32_t sx = ex ? ~0u : 0u, sy = ey ? ~0u : 0u;
+  // corner b = bx + 2 by + 4 bz: quad (bx ? x1 : ox, by ? y1 : oy) of plane bz ? U : B
+  uint32_t qw[8], v[8][4];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    qw[b] = ((b & 4) ? U : B) + 4 * (((b & 2) ? y1 : oy) * hL + ((b & 1) ? x1 : ox));
+    quad_load(lds, qw[b], v[b]);
+  }
+  // half c of every corner quad for cube c
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    if (b & 1) swap_halves(v[b], sx, 2);
+    if (b & 2) swap_halves(v[b], sy, 1);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t rw = c == 0 ? r.x : c == 1 ? r.y : c == 2 ? r.z : r.w;
+    uint32_t w[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) w[b] = v[b][c];
+    eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t nv) { v[b][c] = nv; });
+  }
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    if (b & 2) swap_halves(v[b], sy, 1);
+    if (b & 1) swap_halves(v[b], sx, 2);
+    quad_store(lds, qw[b], v[b]);
   }
 }
 
@@ -230,18 +333,20 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     if (lane < 9) thr_e[gs][lane] = P.thr_eta[lane];
   }
   if (has) {
-    // the slab's window, staged by its waves in 16-byte pieces (L^2 is a
-    // multiple of 16 words)
-    const uint32_t L = F(L), LL = L * L, Q4 = LL / 4, gofs = F(ns) > 1 ? 1u : 0u, np = F(T) + 2 * gofs;
-    const uint4* W4 = reinterpret_cast<const uint4*>(words + F(woff));
+    // the slab's window, staged by its waves one quad (16 bytes of LDS,
+    // four words of `words`) per lane and item
+    const uint32_t L = F(L), hL = L / 2, LL = L * L, Q4 = LL / 4, gofs = F(ns) > 1 ? 1u : 0u,
+                   np = F(T) + 2 * gofs;
+    const uint32_t* Wp = words + F(woff);
     uint4* D4 = reinterpret_cast<uint4*>(lds + F(lds));
-    const FastDiv dq{Q4, F(mP4)};
+    const FastDiv dq{Q4, F(mP4)}, dH{hL, F(mH)};
     const int zs = (int)F(z0) - (int)gofs;
     for (uint32_t i = li; i < np * Q4; i += 64 * F(nw)) {
-      const uint32_t p = fdiv(i, dq), r = i - p * Q4;
+      const uint32_t p = fdiv(i, dq), r = i - p * Q4, y = fdiv(r, dH), x = r - y * hL;
       const int zg = zs + (int)p;
       const uint32_t z = (uint32_t)(zg < 0 ? zg + (int)L : zg >= (int)L ? zg - (int)L : zg);
-      D4[i] = W4[z * Q4 + r];
+      const uint32_t* s = Wp + z * LL + x + L * y;
+      D4[i] = make_uint4(s[0], s[hL * L], s[hL], s[hL * L + hL]);
     }
   }
   __syncthreads();
@@ -350,6 +455,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
         const int zlo = ns > 1 ? -1 : 0;
         const int first = zlo + (parity_of((int)z0 + zlo, L) == fam ? 0 : 1);
         const uint32_t npl = first <= (int)T - 1 ? (uint32_t)((int)T - 1 - first) / 2 + 1 : 0u;
+        const PhiloxFixed pfe = phase_draws(seed, kStEta, step, 0);
         for (Walk it = walk_start(li, Qc, F(mQQ)); it.pi < npl; advance(it, Qc, dpc, duc)) {
           const uint32_t jj = fdiv(it.u, dQ), ii = it.u - jj * qL;
           const int zr = first + 2 * (int)it.pi;
@@ -358,10 +464,10 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
           const uint32_t zgw = (uint32_t)(zg < 0 ? zg + (int)L : zg);
           // cube index x/2 + (L/2)(y/2) + (L^2/4) z (gpu_water3D.cu:3405)
           const uint32_t cube_ref = ii + hL * jj + (LL / 4) * zgw;
-          const U4 r = draw_k(seed, kStEta, cube_ref, step, 0);
+          const U4 r = draw_site(cube_ref, pfe);
           const uint32_t wb = (uint32_t)(zr + (int)gofs);
           const uint32_t wt = ns > 1 ? wb + 1 : (wb + 1 == T ? 0u : wb + 1);
-          eta_cube_item(lds, lbase + wb * LL, lbase + wt * LL, thre, etab, L, ox, oy, r);
+          eta_cube_item_q(lds, lbase + wb * LL, lbase + wt * LL, thre, etab, hL, ox, oy, r);
         }
       }
       __syncthreads();
@@ -420,53 +526,50 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
               RG = arm == 4 ? rings + F(ring_hi) + (half + rd - 1) * LL
                             : rings + F(ring_lo) + (2 * kRingHalf + half + rd - 1) * LL;
               if (hasb) {
-                const uint32_t ub = fb - bit, y = fdiv(ub, dH), x = ub - y * hL, o = x + L * y, dy = hL * L;
-                pre[0] = ring_load(RG + o);
-                pre[1] = ring_load(RG + o + dy);
-                pre[2] = ring_load(RG + o + hL);
-                pre[3] = ring_load(RG + o + hL + dy);
+                const uint32_t o4 = 4 * (fb - bit);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) pre[m] = ring_load(RG + o4 + m);
               }
             }
           }
+          const PhiloxFixed pfz = phase_draws(seed, kStSigma, step, k);
           Walk it = walk_start(li, Q4, F(mP4));
+          // item u of a plane is the quad with origin (u mod L/2, u div L/2),
+          // word 4u of the plane
           for (; it.pi < nint; advance(it, Q4, dp4, du4)) {
             const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
             const uint32_t zr = zi0 + 2 * it.pi;
-            const uint32_t b0 = lbase + (zr + gofs) * LL + x + L * y, dy = hL * L;
-            const uint32_t cell[4] = {b0, b0 + dy, b0 + hL, b0 + hL + dy};
+            const uint32_t cw = lbase + (zr + gofs) * LL + 4 * it.u;
             uint32_t w[4], wn[4];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-              w[m] = lds[cell[m]];
-              wn[m] = lds[cell[m] + dz];
-            }
-            const U4 rq = draw_k(seed, kStSigma, x + L * y + LL * (z0 + zr), step, k);
+            quad_load(lds, cw, w);
+            quad_load(lds, cw + (uint32_t)dz, wn);
+            const U4 rq = draw_site(x + L * y + LL * (z0 + zr), pfz);
             sigma_quad_regs(w, wn, thr, arm, rq);
-#pragma unroll
-            for (int m = 0; m < 4; ++m) lds[cell[m]] = w[m];
+            quad_store(lds, cw, w);
           }
           if (bactive && (!rd || RG)) {
             // the boundary plane, as plane nint of the same walk; the new
             // words also go to this slab's ring slot wr
             const int dzb = ns > 1 ? dz : (arm == 4 ? -(int)((T - 1) * LL) : (int)((T - 1) * LL));
             uint32_t* RW = wr ? rings + F(ring) + ((arm == 4 ? 2 * kRingHalf : 0u) + half + k) * LL : nullptr;
-            const uint32_t pl = lbase + (zb + gofs) * LL, dy = hL * L;
+            const uint32_t pl = lbase + (zb + gofs) * LL;
             for (bool pf = true; it.pi == nint; advance(it, Q4, dp4, du4), pf = false) {
               const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
-              const uint32_t o2 = x + L * y;
-              const uint32_t off[4] = {o2, o2 + dy, o2 + hL, o2 + hL + dy};
+              const uint32_t o4 = 4 * it.u;  // the quad's first word in the plane (and in a ring slot)
               uint32_t w[4], wn[4];
+              quad_load(lds, pl + o4, w);
+              if (!RG) {
+                quad_load(lds, pl + o4 + (uint32_t)dzb, wn);
+              } else {
 #pragma unroll
-              for (int m = 0; m < 4; ++m) {
-                w[m] = lds[pl + off[m]];
-                wn[m] = RG ? (pf ? pre[m] : ring_load(RG + off[m])) : lds[pl + off[m] + dzb];
+                for (int m = 0; m < 4; ++m) wn[m] = pf ? pre[m] : ring_load(RG + o4 + m);
               }
-              const U4 rq = draw_k(seed, kStSigma, o2 + LL * (z0 + zb), step, k);
+              const U4 rq = draw_site(x + L * y + LL * (z0 + zb), pfz);
               sigma_quad_regs(w, wn, thr, arm, rq);
+              quad_store(lds, pl + o4, w);
+              if (RW) {
 #pragma unroll
-              for (int m = 0; m < 4; ++m) {
-                lds[pl + off[m]] = w[m];
-                if (RW) ring_store(RW + off[m], w[m]);
+                for (int m = 0; m < 4; ++m) ring_store(RW + o4 + m, w[m]);
               }
             }
           }
@@ -512,10 +615,20 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
   __syncthreads();
   if (abort_s || !has) return;  // after the last barrier
   {
-    const uint32_t L = F(L), LL = L * L, gofs = F(ns) > 1 ? 1u : 0u;
-    uint4* W4 = reinterpret_cast<uint4*>(words + F(woff) + F(z0) * LL);
+    // the slab's planes back to `words`, one quad per lane and item
+    const uint32_t L = F(L), hL = L / 2, LL = L * L, Q4 = LL / 4, gofs = F(ns) > 1 ? 1u : 0u;
+    uint32_t* Wp = words + F(woff) + F(z0) * LL;
     const uint4* Wl = reinterpret_cast<const uint4*>(lds + F(lds) + gofs * LL);
-    for (uint32_t i = li; i < F(T) * (LL / 4); i += 64 * F(nw)) W4[i] = Wl[i];
+    const FastDiv dq{Q4, F(mP4)}, dH{hL, F(mH)};
+    for (uint32_t i = li; i < F(T) * Q4; i += 64 * F(nw)) {
+      const uint32_t p = fdiv(i, dq), r = i - p * Q4, y = fdiv(r, dH), x = r - y * hL;
+      const uint4 v = Wl[i];
+      uint32_t* d = Wp + p * LL + x + L * y;
+      d[0] = v.x;
+      d[hL * L] = v.y;
+      d[hL] = v.z;
+      d[hL * L + hL] = v.w;
+    }
   }
 #undef F
 }

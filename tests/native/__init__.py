@@ -28,13 +28,16 @@ _lib = None
 def probe() -> C.CDLL:
     global _lib
     if _lib is None:
-        if not OUT.exists():
+        srcs = [HERE / "cvf_core_probe.cpp", ROOT / "cvfbulkwater_amd" / "csrc" / "cvf_core.h",
+                ROOT / "cvfbulkwater_amd" / "csrc" / "cvf_host.cpp", ROOT / "cvfbulkwater_amd" / "csrc" / "cvf_host.h"]
+        if not OUT.exists() or any(f.stat().st_mtime > OUT.stat().st_mtime for f in srcs):
             build_probe()
         L = C.CDLL(str(OUT))
         U32P, U64P, DP, FP, I32P = (C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_double),
                                     C.POINTER(C.c_float), C.POINTER(C.c_int32))
         sig = {
             "probe_philox": (None, [U32P, U32P, U32P]),
+            "probe_philox_site": (None, [U32P, U32P, U32P]),
             "probe_rocrand_philox": (None, [U32P, U32P, U32P]),
             "probe_det_exp": (C.c_double, [C.c_double]),
             "probe_det_log": (C.c_double, [C.c_double]),

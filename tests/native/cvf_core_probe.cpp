@@ -24,6 +24,12 @@ void probe_philox(const uint32_t* c, const uint32_t* k, uint32_t* out) {
   out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
 }
 
+// the split form the kernels use for counters that differ in word 0 only
+void probe_philox_site(const uint32_t* c, const uint32_t* k, uint32_t* out) {
+  const U4 r = philox_site(c[0], philox_fixed(c[1], c[2], c[3], k[0], k[1]));
+  out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+
 void probe_rocrand_philox(const uint32_t* c, const uint32_t* k, uint32_t* out) {
   RocPhilox e;
   uint4 cc; cc.x = c[0]; cc.y = c[1]; cc.z = c[2]; cc.w = c[3];

@@ -47,6 +47,7 @@ def load_data_out():
 def test_philox_kat(ctr, key, want):
     assert list(O.philox(ctr, key)) == want
     assert _probe_philox(probe().probe_philox, ctr, key) == want
+    assert _probe_philox(probe().probe_philox_site, ctr, key) == want
     assert _probe_philox(probe().probe_rocrand_philox, ctr, key) == want
 
 
@@ -57,6 +58,7 @@ def test_philox_matches_rocrand_random_inputs():
         key = [int(x) for x in rng.integers(0, 2 ** 32, 2, dtype=np.uint64)]
         a = list(O.philox(ctr, key))
         assert a == _probe_philox(probe().probe_philox, ctr, key)
+        assert a == _probe_philox(probe().probe_philox_site, ctr, key)
         assert a == _probe_philox(probe().probe_rocrand_philox, ctr, key)
 
 
