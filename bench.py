@@ -546,10 +546,12 @@ def reweight_form(sweeps: int = 300, cpu_sweeps: int = 2, with_cpu: bool = True)
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--plain-steps", type=int, default=2000)
-    ap.add_argument("--profile-steps", type=int, default=20)
+    # step counts are whole numbers of the longest captured step graph (64
+    # steps), so that no shorter tail graph's launch cost lands in a figure
+    ap.add_argument("--steps", type=int, default=256)
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--plain-steps", type=int, default=2048)
+    ap.add_argument("--profile-steps", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tune-steps", type=int, default=20,
                     help="MC steps per autotuner trial (0: default launch configuration)")

@@ -746,7 +746,11 @@ int build_resident(cvf_sim* s, Group& g) {
   if (occ < per_cu) return CVF_OK;
   int rc;
   if ((rc = upload_vec(g.rslabs, d.slabs)) || (rc = upload_vec(g.rgroups, d.groups))) return rc;
+  // versions and rings start cleared: a plan is rebuilt whenever the step
+  // counters may have gone back (the autotuner's restore), so no version
+  // or step-end tag left from later steps survives into the new plan
   HIPC(g.rings.alloc(std::max<size_t>(d.ring_words, 4)));
+  HIPC(hipMemset(g.rings.p, 0, std::max<size_t>(d.ring_words, 4) * sizeof(uint32_t)));
   HIPC(g.rver.alloc(2 * d.slabs.size()));
   HIPC(hipMemset(g.rver.p, 0, 2 * d.slabs.size() * sizeof(uint64_t)));
   g.r_ngroups = (uint32_t)d.groups.size();
@@ -2077,14 +2081,18 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
     // as its 6-sweep twin: not timed, so the result names what runs
     if (t.fused && t.chunk_sweeps == kMaxChunk && !whole_passes_possible(s)) continue;
     if ((rc = cvf_step(s, 2))) break;
+    // at least one replay of the candidate's longest step graph (shorter
+    // trials would replay the same shorter graphs for every graph length);
+    // candidates compare by time per step
+    const int n = std::max<int>(steps_per_trial, t.graph_steps);
     HIPC(hipEventRecord(a, s->stream));
-    if ((rc = cvf_step(s, steps_per_trial))) break;
+    if ((rc = cvf_step(s, n))) break;
     HIPC(hipEventRecord(b, s->stream));
     HIPC(hipEventSynchronize(b));
     float ms = 0;
     HIPC(hipEventElapsedTime(&ms, a, b));
-    if (ms < best_ms) {
-      best_ms = ms;
+    if ((double)ms / n < best_ms) {
+      best_ms = (double)ms / n;
       best = t;
     }
   }

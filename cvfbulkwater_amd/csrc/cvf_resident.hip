@@ -33,10 +33,16 @@
 //     and the same cube-keyed random numbers, so both copies agree.
 // A slab therefore publishes its bottom (top) plane after every -z (+z)
 // half-sweep in which that plane is active, and both boundary planes at the
-// end of every step, into rings in global memory: slot (step & 1) * 16 + j
-// for the j-th publish of the step, slot (step & 1) * 16 + 15 for the end
-// of the step, each followed by a version ((step + 1) << 4 | j).  A reader
-// waits for the version it needs, then reads the slot.
+// end of every step, into rings in global memory.  The j-th publish of a
+// step goes to slot (step & 1) * 16 + j and is followed by a version
+// ((step + 1) << 4 | j); a reader waits for the version it needs, then
+// reads the slot.  The end-of-step planes go to slots (step & 1) * 16 +
+// 14, 15 as 64-bit words that carry a tag of the step beside each cell
+// word; the neighbours load them at the start of the next step without a
+// version, loading a word again while its tag is older, so neither the
+// writer's store completion nor a version round trip lies between one
+// step and the next.  Only the launch's last step also publishes its end
+// version (the neighbours await it before they write their planes back).
 //
 // Coherence across the XCDs' L2 caches: every ring word and every version
 // is accessed with agent-scope relaxed atomics (single-location coherent
@@ -44,7 +50,8 @@
 // L2 writeback or invalidation is needed.  Order: a writer's ring stores
 // complete (s_waitcnt, then the workgroup barrier) before its version
 // store is issued; a reader issues its ring loads only after its version
-// load has returned a large enough value.
+// load has returned a large enough value.  A tagged word needs no order:
+// its tag and cell word are one 64-bit store.
 // A neighbour can be at most one step ahead (each step starts with both
 // neighbours' end-of-step planes), so the two halves of the ring are never
 // overwritten before they are read.
@@ -75,6 +82,20 @@ __device__ __forceinline__ void ring_store(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Step-end planes: 64-bit ring words, the cell word in the low half and
+// end_tag(step) in the high half, in ring slots kEndSlot, kEndSlot + 1 of
+// the step's half.  The tag of a slot's previous contents (two steps back,
+// or a stale value from before the rings were cleared) is never the tag a
+// reader waits for.
+constexpr uint32_t kEndSlot = kRingHalf - 2;
+__device__ __forceinline__ uint32_t end_tag(uint64_t step) { return (uint32_t)(step + 1) | 0x80000000u; }
+__device__ __forceinline__ uint64_t ring_load64(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ring_store64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Wait until *v >= want.  False on time-out or abort.
 __device__ bool wait_version(const uint64_t* v, uint64_t want, uint32_t* status) {
   if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
@@ -91,6 +112,21 @@ __device__ bool wait_version(const uint64_t* v, uint64_t want, uint32_t* status)
     }
     __builtin_amdgcn_s_sleep(2);
   }
+}
+
+// Load *p again until its tag (high half) is `tag`, starting from the
+// value x already loaded.  False on time-out or abort (x is then stale).
+__device__ __forceinline__ bool wait_tag(const uint64_t* p, uint32_t tag, uint64_t& x, uint32_t* status) {
+  for (uint32_t it = 0; (uint32_t)(x >> 32) != tag; ++it) {
+    if (((it & 63u) == 63u && __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ||
+        it >= kSpinLimit) {
+      __hip_atomic_fetch_or(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    x = ring_load64(p);
+  }
+  return true;
 }
 
 // Store a version after the workgroup barrier that follows the slot's ring
@@ -409,35 +445,36 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
       if (wave == F(w0) && lane < 12)
         sch_s[gs][lane] = schv | (rd << 4) | (wr << 8) | (due_top << 12) | (due_bot << 16);
       if (si > 0 && F(ns) > 1) {
-        // the ghosts: the neighbours' boundary planes at the end of the last step
-        const uint32_t LL = L * L;
-        const uint64_t want = ring_version(step - 1, kRingHalf - 1);
+        // the ghosts: the neighbours' boundary planes at the end of the last
+        // step, tagged words (a word whose tag is not yet the last step's is
+        // loaded again until it is; bounded)
+        const uint32_t LL = L * L, tag = end_tag(step - 1);
+        const uint32_t prev = (uint32_t)((step - 1) & 1u) * kRingHalf + kEndSlot;
+        const uint64_t* hi = reinterpret_cast<const uint64_t*>(rings + F(ring_hi) + prev * LL);
+        const uint64_t* lo = reinterpret_cast<const uint64_t*>(rings + F(ring_lo) + (2 * kRingHalf + prev) * LL);
+        uint32_t* D = lds + F(lds);
+        // eight loads in flight per lane (an index past the end re-reads
+        // the last word and is not stored)
+        const uint32_t n2 = 2 * LL, stride = 64 * F(nw);
         bool ok = true;
-        if (lane == 0) ok = wait_version(ver + 2 * F(above), want, status) && wait_version(ver + 2 * F(below) + 1, want, status);
-        if (!uni(ok ? 1u : 0u)) {
-          abort_s = 1;
-        } else {
-          const uint32_t prev = (uint32_t)((step - 1) & 1u) * kRingHalf + kRingHalf - 1;
-          const uint32_t* hi = rings + F(ring_hi) + prev * LL;
-          const uint32_t* lo = rings + F(ring_lo) + (2 * kRingHalf + prev) * LL;
-          uint32_t* D = lds + F(lds);
-          // eight loads in flight per lane (an index past the end re-reads
-          // the last word and is not stored)
-          const uint32_t n2 = 2 * LL, stride = 64 * F(nw);
-          for (uint32_t i0 = li; i0 < n2; i0 += 8 * stride) {
-            uint32_t v[8];
+        for (uint32_t i0 = li; i0 < n2; i0 += 8 * stride) {
+          uint64_t v[8];
 #pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) {
-              const uint32_t i = min(i0 + u * stride, n2 - 1);
-              v[u] = ring_load(i < LL ? hi + i : lo + (i - LL));
-            }
+          for (uint32_t u = 0; u < 8; ++u) {
+            const uint32_t i = min(i0 + u * stride, n2 - 1);
+            v[u] = ring_load64(i < LL ? hi + i : lo + (i - LL));
+          }
 #pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) {
-              const uint32_t i = i0 + u * stride;
-              if (i < n2) D[i < LL ? (T + 1) * LL + i : i - LL] = v[u];
+          for (uint32_t u = 0; u < 8; ++u) {
+            const uint32_t i = i0 + u * stride;
+            if (i < n2) {
+              const uint64_t* src = i < LL ? hi + i : lo + (i - LL);
+              ok = ok && wait_tag(src, tag, v[u], status);
+              D[i < LL ? (T + 1) * LL + i : i - LL] = (uint32_t)v[u];
             }
           }
         }
+        if (__ballot(!ok)) abort_s = 1;
       }
     }
     __syncthreads();
@@ -585,22 +622,27 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
       }
     }
     // ------------------------------------- boundary planes at step end
+    // (tagged words; the readers need no version.  The last step's end
+    // also publishes its version, which the neighbours await before they
+    // write back.)
+    const bool last = si + 1 == nsteps;
     if (has && F(ns) > 1 && !abort_s) {
       const uint32_t L = F(L), LL = L * L, T = F(T);
-      uint32_t* dn = rings + F(ring) + (half + kRingHalf - 1) * LL;
-      uint32_t* up = rings + F(ring) + (2 * kRingHalf + half + kRingHalf - 1) * LL;
+      const uint64_t tag = (uint64_t)end_tag(step) << 32;
+      uint64_t* dn = reinterpret_cast<uint64_t*>(rings + F(ring) + (half + kEndSlot) * LL);
+      uint64_t* up = reinterpret_cast<uint64_t*>(rings + F(ring) + (2 * kRingHalf + half + kEndSlot) * LL);
       const uint32_t* bot = lds + F(lds) + LL;
       const uint32_t* top = lds + F(lds) + T * LL;
       for (uint32_t i = li; i < 2 * LL; i += 64 * F(nw)) {
         if (i < LL)
-          ring_store(dn + i, bot[i]);
+          ring_store64(dn + i, tag | bot[i]);
         else
-          ring_store(up + (i - LL), top[i - LL]);
+          ring_store64(up + (i - LL), tag | top[i - LL]);
       }
-      ring_stores_done();
+      if (last) ring_stores_done();
     }
     __syncthreads();
-    if (has && F(ns) > 1 && wave == F(w0) && lane == 0 && !abort_s) {
+    if (last && has && F(ns) > 1 && wave == F(w0) && lane == 0 && !abort_s) {
       publish(ver + 2 * (G.first + gs), ring_version(step, kRingHalf - 1));
       publish(ver + 2 * (G.first + gs) + 1, ring_version(step, kRingHalf - 1));
     }

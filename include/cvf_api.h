@@ -181,9 +181,11 @@ int cvf_get_plan(cvf_sim *s, int32_t pt, int32_t *group, int32_t *slab_planes, i
 int cvf_set_tuning(cvf_sim *s, const cvf_tuning *t);
 
 /* Project-written autotuner: times candidate tunings (fused slab kernel
- * with several slab thicknesses and workgroup sizes, unfused kernels, 1/2/4
- * streams) for `steps_per_trial` MC steps each, from a snapshot of the
- * state that is restored afterwards, and applies the fastest.  The result
+ * with several slab thicknesses and workgroup sizes, the resident kernel
+ * with several step-graph lengths, unfused kernels, 1/2/4 streams) for
+ * max(`steps_per_trial`, the candidate's graph_steps) MC steps each, from a
+ * snapshot of the state that is restored afterwards, and applies the one
+ * with the shortest time per step.  The result
  * is cached in the file named by CVF_TUNE_CACHE (if set), keyed by the
  * device architecture and the batch shape.  The simulation state and step
  * count are unchanged on return. */
