@@ -558,6 +558,8 @@ def main() -> int:
     ap.add_argument("--tuning", type=json.loads, default=None, metavar="JSON",
                     help="launch configuration of the batch leg, e.g. '{\"fused\": 0}' "
                          "(skips the autotuner)")
+    ap.add_argument("--plain-tuning", type=json.loads, default=None, metavar="JSON",
+                    help="launch configuration of the plain leg (skips its autotuner)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-steps", type=int, default=5000)
     ap.add_argument("--only", choices=["batch", "plain", "plain1", "precision", "cluster", "reweight", "fast"],
@@ -618,7 +620,11 @@ def main() -> int:
     if args.only in (None, "plain", "plain1"):
         sim = Simulation(Params(L=32, T=0.815, P=0.45, seed=44648604, precision=FP64,
                                 volume_mode=VOLUME_HOST))
-        ptuning = sim.autotune(args.tune_steps) if args.tune_steps else sim.tuning
+        if args.plain_tuning is not None:
+            sim.tuning = args.plain_tuning
+            ptuning = sim.tuning
+        else:
+            ptuning = sim.autotune(args.tune_steps) if args.tune_steps else sim.tuning
         pms = time_steps(sim, args.plain_steps, args.warmup, ev)
         sim.close()
         out.update(plain_ms=pms, plain_rate=32 ** 3 / (pms * 1e-3), plain_tuning=ptuning)
