@@ -135,8 +135,9 @@ void get_launch_timing(hipEvent_t* start, hipEvent_t* stop);
 // window: the ghost of the plane below, its T planes, the ghost of the
 // plane above (no ghosts when ns = 1: the whole lattice, periodic in LDS).  Its rings in `rings` (words): [ring, ring +
 // 2 kRingHalf L^2) its bottom plane as published for the slab below, then
-// its top plane for the slab above; versions ver[2 i] (bottom), ver[2 i + 1]
-// (top) for slab i.
+// its top plane for the slab above, each a step-parity half of kRingHalf
+// L^2-word slots holding 64-bit tagged words (two slots per plane);
+// versions ver[2 i] (bottom), ver[2 i + 1] (top) for slab i.
 struct RSlab {
   int32_t point;
   uint32_t L, z0, T, ns;      // ns: slabs of the point
@@ -159,7 +160,7 @@ struct RGroupEnt {
   uint32_t first, count;
 };
 constexpr uint32_t kMaxRSlabs = 8;   // slabs per resident workgroup
-constexpr uint32_t kRingHalf = 16;   // ring slots per step parity (15: end of step)
+constexpr uint32_t kRingHalf = 32;   // ring slots per step parity: 2k, 2k + 1 after half-sweep k; 30, 31 at step end
 constexpr uint32_t kResidentStatic = 6144;  // static LDS reserved for a resident workgroup
 // largest dynamic LDS of a resident workgroup (its static arrays take the
 // rest of the CU's 160 KB; each further workgroup on a CU takes another

@@ -32,26 +32,25 @@
 //     below) spans the boundary; both slabs compute it, from the same words
 //     and the same cube-keyed random numbers, so both copies agree.
 // A slab therefore publishes its bottom (top) plane after every -z (+z)
-// half-sweep in which that plane is active, and both boundary planes at the
-// end of every step, into rings in global memory.  The j-th publish of a
-// step goes to slot (step & 1) * 16 + j and is followed by a version
-// ((step + 1) << 4 | j); a reader waits for the version it needs, then
-// reads the slot.  The end-of-step planes go to slots (step & 1) * 16 +
-// 14, 15 as 64-bit words that carry a tag of the step beside each cell
-// word; the neighbours load them at the start of the next step without a
-// version, loading a word again while its tag is older, so neither the
-// writer's store completion nor a version round trip lies between one
-// step and the next.  Only the launch's last step also publishes its end
-// version (the neighbours await it before they write their planes back).
+// half-sweep in which that plane is active and a neighbour reads it before
+// it moves again, and both boundary planes at the end of every step, into
+// rings in global memory: slots (step & 1) * 32 + 2k, + 1 after half-sweep
+// k, slots (step & 1) * 32 + 30, 31 at the step end.  Every ring word is 64
+// bits, a cell word and a tag of the step; a reader loads the words it
+// needs and loads a word again while its tag is older, so neither a
+// writer's store completion nor a version round trip lies on the path from
+// one slab to the next.  A slot's previous contents (two steps back, or
+// from before the rings were cleared) never carry the awaited tag.  Only
+// the launch's last step also publishes a version, which the neighbours
+// await before they write their planes back to `words`.
 //
 // Coherence across the XCDs' L2 caches: every ring word and every version
 // is accessed with agent-scope relaxed atomics (single-location coherent
 // loads and stores that bypass the XCDs' non-coherent L2 caching), so no
-// L2 writeback or invalidation is needed.  Order: a writer's ring stores
-// complete (s_waitcnt, then the workgroup barrier) before its version
-// store is issued; a reader issues its ring loads only after its version
-// load has returned a large enough value.  A tagged word needs no order:
-// its tag and cell word are one 64-bit store.
+// L2 writeback or invalidation is needed.  A tagged word needs no order:
+// its tag and cell word are one 64-bit store.  The last step's version is
+// stored after the workgroup barrier that follows the completion of the
+// slab's ring stores (s_waitcnt).
 // A neighbour can be at most one step ahead (each step starts with both
 // neighbours' end-of-step planes), so the two halves of the ring are never
 // overwritten before they are read.
@@ -73,14 +72,8 @@ namespace {
 
 constexpr uint32_t kSpinLimit = 1u << 24;  // polls of one wait (~1 s with s_sleep 2)
 
-__device__ __forceinline__ uint64_t ring_version(uint64_t step, uint32_t j) { return ((step + 1) << 4) | j; }
-
-__device__ __forceinline__ uint32_t ring_load(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void ring_store(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// The version a slab publishes after the last step of a launch.
+__device__ __forceinline__ uint64_t end_version(uint64_t step) { return step + 1; }
 
 // Step-end planes: 64-bit ring words, the cell word in the low half and
 // end_tag(step) in the high half, in ring slots kEndSlot, kEndSlot + 1 of
@@ -88,6 +81,8 @@ __device__ __forceinline__ void ring_store(uint32_t* p, uint32_t v) {
 // or a stale value from before the rings were cleared) is never the tag a
 // reader waits for.
 constexpr uint32_t kEndSlot = kRingHalf - 2;
+// first ring slot (of two) of the publish after half-sweep k
+__device__ __forceinline__ uint32_t ring_slot(uint32_t k) { return 2 * k; }
 __device__ __forceinline__ uint32_t end_tag(uint64_t step) { return (uint32_t)(step + 1) | 0x80000000u; }
 __device__ __forceinline__ uint64_t ring_load64(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -415,35 +410,19 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
       // A half-sweep that moves a boundary plane reads the neighbour's
       // plane as of the neighbour's latest such half-sweep (rd = its index
       // + 1; 0: none yet, the ghost holds it), and stores its own moved
-      // plane into ring slot k (wr) only if the neighbour reads it: its
-      // next reading half-sweep comes before this plane moves again.  The
-      // version then goes out after the barrier that ends the half-sweep
-      // just before that reader (due), when the stores have long
-      // completed; each publish is out before the phase that waits for it.
-      uint32_t rd = 0, wr = 0, due = 0xFFu;
+      // plane into ring slot k, as tagged words (wr), only if the neighbour
+      // reads it: its next reading half-sweep comes before this plane
+      // moves again.
+      uint32_t rd = 0, wr = 0;
       const bool mine_up = (up >> lane) & 1, mine_dn = (dn >> lane) & 1;
       if (mine_up || mine_dn) {
         const uint64_t other = mine_up ? ab : bl, self = mine_up ? up : dn;
         if (other & before) rd = 64 - (uint32_t)__builtin_clzll(other & before);
         const uint64_t ra = other & after, ua = self & after;
-        if (ra && (!ua || __builtin_ctzll(ra) < __builtin_ctzll(ua))) {
-          wr = 1;
-          due = (uint32_t)__builtin_ctzll(ra) - 1;
-        }
-      }
-      uint32_t due_top = 0, due_bot = 0;  // 1 + the half-sweep whose publish is due after this one
-      for (uint32_t j = 0; j < 12; ++j) {
-        const uint32_t dj = __builtin_amdgcn_readlane(due, j), aj = __builtin_amdgcn_readlane(arm, j);
-        if (dj == lane) {
-          if (aj == 4)
-            due_top = j + 1;
-          else
-            due_bot = j + 1;
-        }
+        if (ra && (!ua || __builtin_ctzll(ra) < __builtin_ctzll(ua))) wr = 1;
       }
       // the slab's first wave keeps them for every wave of the slab
-      if (wave == F(w0) && lane < 12)
-        sch_s[gs][lane] = schv | (rd << 4) | (wr << 8) | (due_top << 12) | (due_bot << 16);
+      if (wave == F(w0) && lane < 12) sch_s[gs][lane] = schv | (rd << 4) | (wr << 8);
       if (si > 0 && F(ns) > 1) {
         // the ghosts: the neighbours' boundary planes at the end of the last
         // step, tagged words (a word whose tag is not yet the last step's is
@@ -513,7 +492,6 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     for (uint32_t k = 0; k < 12; ++k) {
       const uint32_t sk = has ? uni(sch_s[gs][k]) : 0u;
       const bool wr = (sk >> 8) & 1u;
-      const uint32_t due_top = (sk >> 12) & 0xFu, due_bot = (sk >> 16) & 0xFu;
       const int arm = (int)(sk & 7u);
       const uint32_t q = ((sk >> 3) & 1u) ? 0u : 1u;
       const int ax = arm >> 1;
@@ -541,32 +519,25 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
           const int dz = arm == 4 ? (int)LL : -(int)LL;
           const uint32_t nint = bactive ? npl - 1 : npl;
           const uint32_t zi0 = first + (bactive && zb == 0 ? 2u : 0u);
-          // the neighbour's plane as published after its rd-th z half-sweep
-          // of this step (rd = 0: the ghost, or the wrapped plane of a
-          // whole-lattice window): its version is awaited and the lane's
-          // first boundary quad's four neighbour words are loaded before
-          // the interior planes, so the ring's latency overlaps them
-          const uint32_t rd = (sk >> 4) & 0xFu;
+          // the neighbour's plane as stored after its rd-th z half-sweep of
+          // this step (rd = 0: the ghost, or the wrapped plane of a
+          // whole-lattice window), tagged words of ring slot rd - 1: the
+          // lane's first boundary quad's four are loaded before the interior
+          // planes, so the ring's latency overlaps them, and checked after
+          const uint32_t rd = (sk >> 4) & 0xFu, tag = end_tag(step);
           const uint32_t stride = 64 * F(nw), bit = nint * Q4;
           const uint32_t fb = li >= bit ? li : li + (bit - li + stride - 1) / stride * stride;
           const bool hasb = bactive && fb < bit + Q4;
-          const uint32_t* RG = nullptr;
-          uint32_t pre[4] = {0, 0, 0, 0};
-          if (rd && __ballot(hasb) != 0) {
-            bool ok = true;
-            if (lane == 0)
-              ok = wait_version(ver + 2 * (arm == 4 ? F(above) : F(below)) + (arm == 4 ? 0 : 1),
-                                ring_version(step, rd - 1), status);
-            if (!uni(ok ? 1u : 0u)) {
-              abort_s = 1;
-            } else {
-              RG = arm == 4 ? rings + F(ring_hi) + (half + rd - 1) * LL
-                            : rings + F(ring_lo) + (2 * kRingHalf + half + rd - 1) * LL;
-              if (hasb) {
-                const uint32_t o4 = 4 * (fb - bit);
+          const uint64_t* RG = nullptr;
+          uint64_t pre[4] = {0, 0, 0, 0};
+          if (rd) {
+            RG = reinterpret_cast<const uint64_t*>(
+                arm == 4 ? rings + F(ring_hi) + (half + ring_slot(rd - 1)) * LL
+                         : rings + F(ring_lo) + (2 * kRingHalf + half + ring_slot(rd - 1)) * LL);
+            if (hasb) {
+              const uint32_t o4 = 4 * (fb - bit);
 #pragma unroll
-                for (int m = 0; m < 4; ++m) pre[m] = ring_load(RG + o4 + m);
-              }
+              for (int m = 0; m < 4; ++m) pre[m] = ring_load64(RG + o4 + m);
             }
           }
           const PhiloxFixed pfz = phase_draws(seed, kStSigma, step, k);
@@ -584,12 +555,16 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
             sigma_quad_regs(w, wn, thr, arm, rq);
             quad_store(lds, cw, w);
           }
-          if (bactive && (!rd || RG)) {
+          if (bactive) {
             // the boundary plane, as plane nint of the same walk; the new
-            // words also go to this slab's ring slot wr
+            // words also go to this slab's ring slot k (wr), tagged
             const int dzb = ns > 1 ? dz : (arm == 4 ? -(int)((T - 1) * LL) : (int)((T - 1) * LL));
-            uint32_t* RW = wr ? rings + F(ring) + ((arm == 4 ? 2 * kRingHalf : 0u) + half + k) * LL : nullptr;
+            uint64_t* RW = wr ? reinterpret_cast<uint64_t*>(
+                                    rings + F(ring) + ((arm == 4 ? 2 * kRingHalf : 0u) + half + ring_slot(k)) * LL)
+                              : nullptr;
+            const uint64_t tw = (uint64_t)tag << 32;
             const uint32_t pl = lbase + (zb + gofs) * LL;
+            bool ok = true;
             for (bool pf = true; it.pi == nint; advance(it, Q4, dp4, du4), pf = false) {
               const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
               const uint32_t o4 = 4 * it.u;  // the quad's first word in the plane (and in a ring slot)
@@ -598,28 +573,28 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
               if (!RG) {
                 quad_load(lds, pl + o4 + (uint32_t)dzb, wn);
               } else {
+                uint64_t v[4];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) wn[m] = pf ? pre[m] : ring_load(RG + o4 + m);
+                for (int m = 0; m < 4; ++m) v[m] = pf ? pre[m] : ring_load64(RG + o4 + m);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                  ok = ok && wait_tag(RG + o4 + m, tag, v[m], status);
+                  wn[m] = (uint32_t)v[m];
+                }
               }
               const U4 rq = draw_site(x + L * y + LL * (z0 + zb), pfz);
               sigma_quad_regs(w, wn, thr, arm, rq);
               quad_store(lds, pl + o4, w);
               if (RW) {
 #pragma unroll
-                for (int m = 0; m < 4; ++m) ring_store(RW + o4 + m, w[m]);
+                for (int m = 0; m < 4; ++m) ring_store64(RW + o4 + m, tw | w[m]);
               }
             }
+            if (__ballot(!ok)) abort_s = 1;
           }
         }
       }
-      // a publish due after this half-sweep: the slab's stores of its slot
-      // (made in this or an earlier half-sweep) complete before the barrier
-      if (run && (due_top || due_bot)) ring_stores_done();
       __syncthreads();
-      if (has && wave == F(w0) && lane == 0 && !abort_s) {
-        if (due_top) publish(ver + 2 * (G.first + gs) + 1, ring_version(step, due_top - 1));
-        if (due_bot) publish(ver + 2 * (G.first + gs), ring_version(step, due_bot - 1));
-      }
     }
     // ------------------------------------- boundary planes at step end
     // (tagged words; the readers need no version.  The last step's end
@@ -643,14 +618,14 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     }
     __syncthreads();
     if (last && has && F(ns) > 1 && wave == F(w0) && lane == 0 && !abort_s) {
-      publish(ver + 2 * (G.first + gs), ring_version(step, kRingHalf - 1));
-      publish(ver + 2 * (G.first + gs) + 1, ring_version(step, kRingHalf - 1));
+      publish(ver + 2 * (G.first + gs), end_version(step));
+      publish(ver + 2 * (G.first + gs) + 1, end_version(step));
     }
   }
   // the neighbours have read their windows (they published after staging)
   // before this workgroup overwrites its planes in `words`
   if (has && F(ns) > 1 && nsteps > 0 && lane == 0 && wave == F(w0)) {
-    const uint64_t want = ring_version(step0 + nsteps - 1, kRingHalf - 1);
+    const uint64_t want = end_version(step0 + nsteps - 1);
     if (!wait_version(ver + 2 * F(above), want, status) || !wait_version(ver + 2 * F(below) + 1, want, status))
       abort_s = 1;
   }
