@@ -636,6 +636,7 @@ bool plan_resident(const cvf_sim* s, const Group& g, int nwg, uint32_t wpg, uint
       r.du4 = stride % Q4;
       r.dpc = stride / Qc;
       r.duc = stride % Qc;
+      r.mS = make_fastdiv(stride).m;
       if (!whole) {
         r.ring = (uint32_t)ring;
         ring += 4 * (size_t)kRingHalf * L * L;

@@ -152,7 +152,7 @@ struct RSlab {
   // and du items of L^2/8 (in-plane quads), L^2/4 (z quads) or (L/4)^2
   // (four-cube items) per plane
   uint32_t dp8, du8, dp4, du4, dpc, duc;
-  uint32_t pad;
+  uint32_t mS;  // fast-division multiplier of that stride, 64 nw
 };
 static_assert(sizeof(RSlab) % 16 == 0, "RSlab is copied in words");
 // Workgroup g of a resident launch takes slabs [first, first + count).

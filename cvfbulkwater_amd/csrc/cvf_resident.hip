@@ -372,12 +372,27 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     uint4* D4 = reinterpret_cast<uint4*>(lds + F(lds));
     const FastDiv dq{Q4, F(mP4)}, dH{hL, F(mH)};
     const int zs = (int)F(z0) - (int)gofs;
-    for (uint32_t i = li; i < np * Q4; i += 64 * F(nw)) {
-      const uint32_t p = fdiv(i, dq), r = i - p * Q4, y = fdiv(r, dH), x = r - y * hL;
-      const int zg = zs + (int)p;
-      const uint32_t z = (uint32_t)(zg < 0 ? zg + (int)L : zg >= (int)L ? zg - (int)L : zg);
-      const uint32_t* s = Wp + z * LL + x + L * y;
-      D4[i] = make_uint4(s[0], s[hL * L], s[hL], s[hL * L + hL]);
+    if ((hL & 3u) == 0) {
+      // four quads along x per lane and item: 16-byte rows of `words`
+      for (uint32_t i = 4 * li; i < np * Q4; i += 256 * F(nw)) {
+        const uint32_t p = fdiv(i, dq), r = i - p * Q4, y = fdiv(r, dH), x = r - y * hL;
+        const int zg = zs + (int)p;
+        const uint32_t z = (uint32_t)(zg < 0 ? zg + (int)L : zg >= (int)L ? zg - (int)L : zg);
+        const uint4* s = reinterpret_cast<const uint4*>(Wp + z * LL + x + L * y);
+        const uint4 a = s[0], c = s[hL / 4], b = s[hL * L / 4], d = s[(hL * L + hL) / 4];
+        D4[i] = make_uint4(a.x, b.x, c.x, d.x);
+        D4[i + 1] = make_uint4(a.y, b.y, c.y, d.y);
+        D4[i + 2] = make_uint4(a.z, b.z, c.z, d.z);
+        D4[i + 3] = make_uint4(a.w, b.w, c.w, d.w);
+      }
+    } else {
+      for (uint32_t i = li; i < np * Q4; i += 64 * F(nw)) {
+        const uint32_t p = fdiv(i, dq), r = i - p * Q4, y = fdiv(r, dH), x = r - y * hL;
+        const int zg = zs + (int)p;
+        const uint32_t z = (uint32_t)(zg < 0 ? zg + (int)L : zg >= (int)L ? zg - (int)L : zg);
+        const uint32_t* s = Wp + z * LL + x + L * y;
+        D4[i] = make_uint4(s[0], s[hL * L], s[hL], s[hL * L + hL]);
+      }
     }
   }
   __syncthreads();
@@ -526,7 +541,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
           // planes, so the ring's latency overlaps them, and checked after
           const uint32_t rd = (sk >> 4) & 0xFu, tag = end_tag(step);
           const uint32_t stride = 64 * F(nw), bit = nint * Q4;
-          const uint32_t fb = li >= bit ? li : li + (bit - li + stride - 1) / stride * stride;
+          const uint32_t fb = li >= bit ? li : li + fdiv(bit - li + stride - 1, FastDiv{stride, F(mS)}) * stride;
           const bool hasb = bactive && fb < bit + Q4;
           const uint64_t* RG = nullptr;
           uint64_t pre[4] = {0, 0, 0, 0};
@@ -637,14 +652,27 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     uint32_t* Wp = words + F(woff) + F(z0) * LL;
     const uint4* Wl = reinterpret_cast<const uint4*>(lds + F(lds) + gofs * LL);
     const FastDiv dq{Q4, F(mP4)}, dH{hL, F(mH)};
-    for (uint32_t i = li; i < F(T) * Q4; i += 64 * F(nw)) {
-      const uint32_t p = fdiv(i, dq), r = i - p * Q4, y = fdiv(r, dH), x = r - y * hL;
-      const uint4 v = Wl[i];
-      uint32_t* d = Wp + p * LL + x + L * y;
-      d[0] = v.x;
-      d[hL * L] = v.y;
-      d[hL] = v.z;
-      d[hL * L + hL] = v.w;
+    if ((hL & 3u) == 0) {
+      // four quads along x per lane and item: 16-byte rows of `words`
+      for (uint32_t i = 4 * li; i < F(T) * Q4; i += 256 * F(nw)) {
+        const uint32_t p = fdiv(i, dq), r = i - p * Q4, y = fdiv(r, dH), x = r - y * hL;
+        const uint4 q0 = Wl[i], q1 = Wl[i + 1], q2 = Wl[i + 2], q3 = Wl[i + 3];
+        uint4* d = reinterpret_cast<uint4*>(Wp + p * LL + x + L * y);
+        d[0] = make_uint4(q0.x, q1.x, q2.x, q3.x);
+        d[hL * L / 4] = make_uint4(q0.y, q1.y, q2.y, q3.y);
+        d[hL / 4] = make_uint4(q0.z, q1.z, q2.z, q3.z);
+        d[(hL * L + hL) / 4] = make_uint4(q0.w, q1.w, q2.w, q3.w);
+      }
+    } else {
+      for (uint32_t i = li; i < F(T) * Q4; i += 64 * F(nw)) {
+        const uint32_t p = fdiv(i, dq), r = i - p * Q4, y = fdiv(r, dH), x = r - y * hL;
+        const uint4 v = Wl[i];
+        uint32_t* d = Wp + p * LL + x + L * y;
+        d[0] = v.x;
+        d[hL * L] = v.y;
+        d[hL] = v.z;
+        d[hL * L + hL] = v.w;
+      }
     }
   }
 #undef F
