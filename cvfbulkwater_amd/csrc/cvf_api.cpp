@@ -99,6 +99,7 @@ struct Group {
   DevBuf<uint32_t> rings;
   DevBuf<uint64_t> rver;
   uint32_t r_ngroups = 0, r_lds = 0;
+  bool r_fine = false;  // every slab at most two planes: the fine instantiation
   // captured step sequences, longest first: graph_steps, then the powers of
   // two below it, so any step count is a few graph launches
   std::vector<int> gsteps;
@@ -743,7 +744,9 @@ int build_resident(cvf_sim* s, Group& g) {
   if (!plan_resident(s, g, ncu * per_cu, wpg, cap, (uint32_t)std::max(0, s->tune.slab_planes), d)) return CVF_OK;
   if (!resident_plan_valid(s, g, d, wpg, cap)) return fail(CVF_EINVAL, "internal: the resident plan is inconsistent");
   int occ = 0;
-  HIPC(resident_occupancy(&occ, (uint32_t)s->tune.block_chunk, d.lds_bytes));
+  bool fine = true;
+  for (const auto& r : d.slabs) fine = fine && r.T <= 2;
+  HIPC(resident_occupancy(&occ, (uint32_t)s->tune.block_chunk, d.lds_bytes, fine));
   if (occ < per_cu) return CVF_OK;
   int rc;
   if ((rc = upload_vec(g.rslabs, d.slabs)) || (rc = upload_vec(g.rgroups, d.groups))) return rc;
@@ -756,6 +759,7 @@ int build_resident(cvf_sim* s, Group& g) {
   HIPC(hipMemset(g.rver.p, 0, 2 * d.slabs.size() * sizeof(uint64_t)));
   g.r_ngroups = (uint32_t)d.groups.size();
   g.r_lds = d.lds_bytes;
+  g.r_fine = fine;
   g.resident = true;
   for (int p : g.points) s->ph[p].slab_T = 0;
   // the plan query reports a point's thickest slab
@@ -1042,7 +1046,7 @@ void enqueue_lattice(cvf_sim* s, int gi, hipStream_t st, uint32_t step_off, Time
     // one step of the resident kernel, in place on `words`
     timed(CVF_KCLASS_RESIDENT, [&] {
       launch_resident(st, g.rgroups.p, g.r_ngroups, (uint32_t)s->tune.block_chunk, g.r_lds, g.rslabs.p, s->pts.p,
-                      s->words.p, g.rings.p, g.rver.p, stepp, step_off, 1, s->rstatus.p);
+                      s->words.p, g.rings.p, g.rver.p, stepp, step_off, 1, s->rstatus.p, g.r_fine);
     });
   } else if (s->sched == CVF_SCHED_FAST) {
     // eta cube sweep, then n_metropolis x the two colour passes (in place,
@@ -1119,7 +1123,7 @@ int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps, bool rec = fal
   if (g.resident && s->nc == 0 && !rec) {
     // every step of the sequence in one resident launch
     launch_resident(st, g.rgroups.p, g.r_ngroups, (uint32_t)s->tune.block_chunk, g.r_lds, g.rslabs.p, s->pts.p,
-                    s->words.p, g.rings.p, g.rver.p, s->cur_steps.p + gi, 0, (uint32_t)nsteps, s->rstatus.p);
+                    s->words.p, g.rings.p, g.rver.p, s->cur_steps.p + gi, 0, (uint32_t)nsteps, s->rstatus.p, g.r_fine);
     nsteps = 0;
   }
   for (int i = 0; i < nsteps; ++i) {
@@ -1832,7 +1836,7 @@ int cvf_profile_steps(cvf_sim* s, int64_t n_steps, double* kernel_ms, int64_t* l
     timed(CVF_KCLASS_RESIDENT, [&] {
       launch_resident(g.stream, g.rgroups.p, g.r_ngroups, (uint32_t)s->tune.block_chunk, g.r_lds, g.rslabs.p,
                       s->pts.p, s->words.p, g.rings.p, g.rver.p, s->cur_steps.p + gi, 0, (uint32_t)n_steps,
-                      s->rstatus.p);
+                      s->rstatus.p, g.r_fine);
     });
     timed(CVF_KCLASS_BEGIN, [&] {
       launch_vol_chain(g.stream, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, s->cur_steps.p + gi,

@@ -168,9 +168,12 @@ constexpr uint32_t kResidentStatic = 6144;  // static LDS reserved for a residen
 uint32_t resident_lds_limit();
 // resident workgroups of `block` threads and `lds_bytes` of dynamic LDS that
 // fit on one CU at once
-hipError_t resident_occupancy(int* per_cu, uint32_t block, uint32_t lds_bytes);
+hipError_t resident_occupancy(int* per_cu, uint32_t block, uint32_t lds_bytes, bool fine);
+// fine: the instantiation for plans of thin slabs (at most two planes), whose
+// phases that fill at most half of a slab's lanes run one cell (one cube)
+// per lane instead of one quad (one four-cube item)
 void launch_resident(hipStream_t st, const RGroupEnt* grp, uint32_t ngroups, uint32_t block, uint32_t lds_bytes,
                      const RSlab* slabs, const PointDev* pts, uint32_t* words, uint32_t* rings, uint64_t* ver,
-                     const uint64_t* stepp, uint32_t step_off, uint32_t nsteps, uint32_t* status);
+                     const uint64_t* stepp, uint32_t step_off, uint32_t nsteps, uint32_t* status, bool fine);
 
 }  // namespace cvf

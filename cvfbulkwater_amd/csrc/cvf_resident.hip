@@ -140,6 +140,22 @@ __device__ __forceinline__ void ring_stores_done() {
 __device__ __forceinline__ uint32_t parity_of(int z, uint32_t L) { return (uint32_t)(z + (int)L) & 1u; }
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Chessboard sigma rule on one word w (facing neighbour word wn, random
+// word rw): the cell's new word, as sigma_quad_regs computes each of four.
+__device__ __forceinline__ uint32_t sigma_cell(uint32_t w, uint32_t wn, const uint32_t* thr, int arm, uint32_t rw) {
+  uint32_t sp, ra;
+  split_word(rw, 6, sp, ra);
+  const uint32_t sa = sigma_of(w, arm);
+  const uint32_t t = thr[sigma_thr_index(w, wn, arm, sa, sp)];
+  const uint32_t nw = (w & ~(7u << (3 * arm))) | (sp << (3 * arm));
+  return sp != sa && ra < t ? nw : w;
+}
+
+// Word h of a Philox block (h lane-dependent).
+__device__ __forceinline__ uint32_t pick4(const U4& r, uint32_t h) {
+  return h == 0 ? r.x : h == 1 ? r.y : h == 2 ? r.z : r.w;
+}
+
 // Chessboard sigma rule on four words held in registers (facing neighbour
 // words wn, one Philox block): the cells' new words.  The four threshold
 // loads are issued together (the index is in range also when the proposal
@@ -226,7 +242,7 @@ __device__ __forceinline__ void quad_store(uint32_t* lds, uint32_t w, const uint
 // one along the axis (the previous one for a - arm); across the periodic
 // edge it is the quad at the other end, with the halves along the axis
 // swapped.
-template <int AX>
+template <int AX, bool FINE>
 __device__ __forceinline__ void sweep_inplane(uint32_t* lds, uint32_t li, uint32_t L, uint32_t T, uint32_t z0,
                                               uint32_t pbase, uint64_t seed, uint64_t step, uint32_t k, int arm,
                                               uint32_t q, FastDiv dH, FastDiv dQ, uint32_t mP8, uint32_t dp8,
@@ -238,6 +254,34 @@ __device__ __forceinline__ void sweep_inplane(uint32_t* lds, uint32_t li, uint32
   const int dn = up ? (int)st : -(int)st;                   // facing quad, no wrap
   const int dw = up ? -(int)((hL - 1) * st) : (int)((hL - 1) * st);  // facing quad across the edge
   const PhiloxFixed pf = phase_draws(seed, kStSigma, step, k);
+  if (FINE && 8 * T * Q8 <= dp8 * Q8 + du8) {
+    // fine: the quads' cells fill at most half of the slab's lanes at one
+    // cell each (at most two waves per SIMD), so lane li takes cell li mod
+    // 4 of quad li div 4: a thin slab's phase is one short dependent chain
+    // per lane instead of one long one
+    const uint32_t h = li & 3u;
+    const Walk it = walk_start(li >> 2, Q8, mP8);
+    if (it.pi < T) {
+      uint32_t x, y, quad;
+      if (AX == 0) {
+        y = fdiv(it.u, dQ);
+        x = 2 * (it.u - y * qL) + q;
+        quad = 2 * it.u + q;
+      } else {
+        const uint32_t hy = fdiv(it.u, dH);
+        x = it.u - hy * hL;
+        y = 2 * hy + q;
+        quad = it.u + hL * (hy + q);
+      }
+      const bool wrap = (AX == 0 ? x : y) == edge;
+      const uint32_t cw = pbase + it.pi * LL + 4 * quad;
+      const uint32_t nw = cw + 4u * (uint32_t)(wrap ? dw : dn) + (wrap ? h ^ (AX == 0 ? 2u : 1u) : h);
+      const uint32_t w = lds[cw + h], wn = lds[nw];
+      const U4 rq = draw_site(x + L * y + LL * (z0 + it.pi), pf);
+      lds[cw + h] = sigma_cell(w, wn, thr, arm, pick4(rq, h));
+    }
+    return;
+  }
   for (Walk it = walk_start(li, Q8, mP8); it.pi < T; advance(it, Q8, dp8, du8)) {
     uint32_t x, y, quad;
     if (AX == 0) {
@@ -283,6 +327,25 @@ __device__ __forceinline__ void swap_halves(uint32_t (&v)[4], uint32_t sel, int 
 #pragma unroll
   for (int h = 0; h < 4; ++h) v[h] = t[h];
 }
+// Cube c alone of the cube item eta_cube_item_q describes (random word
+// rw = word c of the item's block): its eight corner words are half
+// c ^ (the corner's swap) of the eight corner quads.
+__device__ __forceinline__ void eta_cube_one_q(uint32_t* lds, uint32_t B, uint32_t U, const uint64_t* thr,
+                                               const uint64_t* etab, uint32_t hL, uint32_t ox, uint32_t oy, uint32_t c,
+                                               uint32_t rw) {
+  const bool ex = ox + 1 == hL, ey = oy + 1 == hL;
+  const uint32_t x1 = ex ? 0u : ox + 1, y1 = ey ? 0u : oy + 1;
+  uint32_t idx[8], w[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint32_t f = ((b & 1) && ex ? 2u : 0u) | ((b & 2) && ey ? 1u : 0u);
+    idx[b] = ((b & 4) ? U : B) + 4 * (((b & 2) ? y1 : oy) * hL + ((b & 1) ? x1 : ox)) + (c ^ f);
+  }
+#pragma unroll
+  for (int b = 0; b < 8; ++b) w[b] = lds[idx[b]];
+  eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t v) { lds[idx[b]] = v; });
+}
+
 __device__ __forceinline__ void eta_cube_item_q(uint32_t* lds, uint32_t B, uint32_t U, const uint64_t* thr,
                                                 const uint64_t* etab, uint32_t hL, uint32_t ox, uint32_t oy,
                                                 const U4& r) {
@@ -324,6 +387,7 @@ __device__ __forceinline__ void eta_cube_item_q(uint32_t* lds, uint32_t B, uint3
 // window sizes of one workgroup.  The slab records sit in LDS and each
 // phase reads the fields it uses (wave-uniform), so that they are not all
 // held in scalar registers through the step loop.
+template <bool FINE>
 __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__ grp, const RSlab* __restrict__ slabs,
                                                     const PointDev* __restrict__ pts, uint32_t* __restrict__ words,
                                                     uint32_t* __restrict__ rings, uint64_t* __restrict__ ver,
@@ -487,18 +551,37 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
         const int first = zlo + (parity_of((int)z0 + zlo, L) == fam ? 0 : 1);
         const uint32_t npl = first <= (int)T - 1 ? (uint32_t)((int)T - 1 - first) / 2 + 1 : 0u;
         const PhiloxFixed pfe = phase_draws(seed, kStEta, step, 0);
-        for (Walk it = walk_start(li, Qc, F(mQQ)); it.pi < npl; advance(it, Qc, dpc, duc)) {
-          const uint32_t jj = fdiv(it.u, dQ), ii = it.u - jj * qL;
-          const int zr = first + 2 * (int)it.pi;
-          const uint32_t ox = 2 * ii + fam, oy = 2 * jj + fam;
-          const int zg = (int)z0 + zr;
-          const uint32_t zgw = (uint32_t)(zg < 0 ? zg + (int)L : zg);
-          // cube index x/2 + (L/2)(y/2) + (L^2/4) z (gpu_water3D.cu:3405)
-          const uint32_t cube_ref = ii + hL * jj + (LL / 4) * zgw;
-          const U4 r = draw_site(cube_ref, pfe);
-          const uint32_t wb = (uint32_t)(zr + (int)gofs);
-          const uint32_t wt = ns > 1 ? wb + 1 : (wb + 1 == T ? 0u : wb + 1);
-          eta_cube_item_q(lds, lbase + wb * LL, lbase + wt * LL, thre, etab, hL, ox, oy, r);
+        if (FINE && 8 * npl * Qc <= dpc * Qc + duc) {
+          // fine: the items' cubes fill at most half of the slab's lanes at
+          // one cube each, so lane li takes cube li mod 4 of item li div 4
+          const uint32_t c = li & 3u;
+          const Walk it = walk_start(li >> 2, Qc, F(mQQ));
+          if (it.pi < npl) {
+            const uint32_t jj = fdiv(it.u, dQ), ii = it.u - jj * qL;
+            const int zr = first + 2 * (int)it.pi;
+            const uint32_t ox = 2 * ii + fam, oy = 2 * jj + fam;
+            const int zg = (int)z0 + zr;
+            const uint32_t zgw = (uint32_t)(zg < 0 ? zg + (int)L : zg);
+            const uint32_t cube_ref = ii + hL * jj + (LL / 4) * zgw;
+            const U4 r = draw_site(cube_ref, pfe);
+            const uint32_t wb = (uint32_t)(zr + (int)gofs);
+            const uint32_t wt = ns > 1 ? wb + 1 : (wb + 1 == T ? 0u : wb + 1);
+            eta_cube_one_q(lds, lbase + wb * LL, lbase + wt * LL, thre, etab, hL, ox, oy, c, pick4(r, c));
+          }
+        } else {
+          for (Walk it = walk_start(li, Qc, F(mQQ)); it.pi < npl; advance(it, Qc, dpc, duc)) {
+            const uint32_t jj = fdiv(it.u, dQ), ii = it.u - jj * qL;
+            const int zr = first + 2 * (int)it.pi;
+            const uint32_t ox = 2 * ii + fam, oy = 2 * jj + fam;
+            const int zg = (int)z0 + zr;
+            const uint32_t zgw = (uint32_t)(zg < 0 ? zg + (int)L : zg);
+            // cube index x/2 + (L/2)(y/2) + (L^2/4) z (gpu_water3D.cu:3405)
+            const uint32_t cube_ref = ii + hL * jj + (LL / 4) * zgw;
+            const U4 r = draw_site(cube_ref, pfe);
+            const uint32_t wb = (uint32_t)(zr + (int)gofs);
+            const uint32_t wt = ns > 1 ? wb + 1 : (wb + 1 == T ? 0u : wb + 1);
+            eta_cube_item_q(lds, lbase + wb * LL, lbase + wt * LL, thre, etab, hL, ox, oy, r);
+          }
         }
       }
       __syncthreads();
@@ -517,10 +600,10 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
         const FastDiv dH{hL, F(mH)}, dQ{qL, F(mQ)};
         const uint32_t* thr = thr_s[gs];
         if (ax == 0) {
-          sweep_inplane<0>(lds, li, L, T, z0, lbase + gofs * LL, seed, step, k, arm, q, dH, dQ, F(mP8), F(dp8),
+          sweep_inplane<0, FINE>(lds, li, L, T, z0, lbase + gofs * LL, seed, step, k, arm, q, dH, dQ, F(mP8), F(dp8),
                            F(du8), thr);
         } else if (ax == 1) {
-          sweep_inplane<1>(lds, li, L, T, z0, lbase + gofs * LL, seed, step, k, arm, q, dH, dQ, F(mP8), F(dp8),
+          sweep_inplane<1, FINE>(lds, li, L, T, z0, lbase + gofs * LL, seed, step, k, arm, q, dH, dQ, F(mP8), F(dp8),
                            F(du8), thr);
         } else {
           // planes of the kept parity; the boundary plane whose neighbour
@@ -536,76 +619,107 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
           const uint32_t zi0 = first + (bactive && zb == 0 ? 2u : 0u);
           // the neighbour's plane as stored after its rd-th z half-sweep of
           // this step (rd = 0: the ghost, or the wrapped plane of a
-          // whole-lattice window), tagged words of ring slot rd - 1: the
-          // lane's first boundary quad's four are loaded before the interior
-          // planes, so the ring's latency overlaps them, and checked after
+          // whole-lattice window): tagged words of ring slot rd - 1; the
+          // boundary plane's new words go to this slab's ring slot k (wr),
+          // tagged
           const uint32_t rd = (sk >> 4) & 0xFu, tag = end_tag(step);
-          const uint32_t stride = 64 * F(nw), bit = nint * Q4;
-          const uint32_t fb = li >= bit ? li : li + fdiv(bit - li + stride - 1, FastDiv{stride, F(mS)}) * stride;
-          const bool hasb = bactive && fb < bit + Q4;
-          const uint64_t* RG = nullptr;
-          uint64_t pre[4] = {0, 0, 0, 0};
-          if (rd) {
-            RG = reinterpret_cast<const uint64_t*>(
-                arm == 4 ? rings + F(ring_hi) + (half + ring_slot(rd - 1)) * LL
-                         : rings + F(ring_lo) + (2 * kRingHalf + half + ring_slot(rd - 1)) * LL);
-            if (hasb) {
+          const uint32_t stride = 64 * F(nw);
+          const uint64_t* RG =
+              rd ? reinterpret_cast<const uint64_t*>(
+                       arm == 4 ? rings + F(ring_hi) + (half + ring_slot(rd - 1)) * LL
+                                : rings + F(ring_lo) + (2 * kRingHalf + half + ring_slot(rd - 1)) * LL)
+                 : nullptr;
+          uint64_t* RW = bactive && wr ? reinterpret_cast<uint64_t*>(rings + F(ring) +
+                                                                     ((arm == 4 ? 2 * kRingHalf : 0u) + half +
+                                                                      ring_slot(k)) * LL)
+                                       : nullptr;
+          const int dzb = ns > 1 ? dz : (arm == 4 ? -(int)((T - 1) * LL) : (int)((T - 1) * LL));
+          const uint64_t tw = (uint64_t)tag << 32;
+          const PhiloxFixed pfz = phase_draws(seed, kStSigma, step, k);
+          // item u of a plane is the quad with origin (u mod L/2, u div L/2),
+          // word 4u of the plane
+          if (FINE && 8 * npl * Q4 <= stride) {
+            // fine: one cell per lane (cell li mod 4 of item li div 4), the
+            // boundary plane as plane nint
+            const uint32_t h = li & 3u;
+            const Walk it = walk_start(li >> 2, Q4, F(mP4));
+            bool ok = true;
+            if (it.pi < npl) {
+              const bool bnd = bactive && it.pi == nint;
+              const uint32_t zr = bnd ? zb : zi0 + 2 * it.pi;
+              const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
+              const uint32_t o = 4 * it.u + h, cw = lbase + (zr + gofs) * LL + o;
+              const uint32_t w = lds[cw];
+              uint32_t wn;
+              if (bnd && RG) {
+                uint64_t v = ring_load64(RG + o);
+                ok = wait_tag(RG + o, tag, v, status);
+                wn = (uint32_t)v;
+              } else {
+                wn = lds[cw + (uint32_t)(bnd ? dzb : dz)];
+              }
+              const U4 rq = draw_site(x + L * y + LL * (z0 + zr), pfz);
+              const uint32_t nwd = sigma_cell(w, wn, thr, arm, pick4(rq, h));
+              lds[cw] = nwd;
+              if (bnd && RW) ring_store64(RW + o, tw | nwd);
+            }
+            if (__ballot(!ok)) abort_s = 1;
+          } else {
+            // the lane's first boundary quad's four ring words are loaded
+            // before the interior planes, so the ring's latency overlaps
+            // them, and checked after
+            const uint32_t bit = nint * Q4;
+            const uint32_t fb = li >= bit ? li : li + fdiv(bit - li + stride - 1, FastDiv{stride, F(mS)}) * stride;
+            const bool hasb = bactive && fb < bit + Q4;
+            uint64_t pre[4] = {0, 0, 0, 0};
+            if (RG && hasb) {
               const uint32_t o4 = 4 * (fb - bit);
 #pragma unroll
               for (int m = 0; m < 4; ++m) pre[m] = ring_load64(RG + o4 + m);
             }
-          }
-          const PhiloxFixed pfz = phase_draws(seed, kStSigma, step, k);
-          Walk it = walk_start(li, Q4, F(mP4));
-          // item u of a plane is the quad with origin (u mod L/2, u div L/2),
-          // word 4u of the plane
-          for (; it.pi < nint; advance(it, Q4, dp4, du4)) {
-            const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
-            const uint32_t zr = zi0 + 2 * it.pi;
-            const uint32_t cw = lbase + (zr + gofs) * LL + 4 * it.u;
-            uint32_t w[4], wn[4];
-            quad_load(lds, cw, w);
-            quad_load(lds, cw + (uint32_t)dz, wn);
-            const U4 rq = draw_site(x + L * y + LL * (z0 + zr), pfz);
-            sigma_quad_regs(w, wn, thr, arm, rq);
-            quad_store(lds, cw, w);
-          }
-          if (bactive) {
-            // the boundary plane, as plane nint of the same walk; the new
-            // words also go to this slab's ring slot k (wr), tagged
-            const int dzb = ns > 1 ? dz : (arm == 4 ? -(int)((T - 1) * LL) : (int)((T - 1) * LL));
-            uint64_t* RW = wr ? reinterpret_cast<uint64_t*>(
-                                    rings + F(ring) + ((arm == 4 ? 2 * kRingHalf : 0u) + half + ring_slot(k)) * LL)
-                              : nullptr;
-            const uint64_t tw = (uint64_t)tag << 32;
-            const uint32_t pl = lbase + (zb + gofs) * LL;
-            bool ok = true;
-            for (bool pf = true; it.pi == nint; advance(it, Q4, dp4, du4), pf = false) {
+            Walk it = walk_start(li, Q4, F(mP4));
+            for (; it.pi < nint; advance(it, Q4, dp4, du4)) {
               const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
-              const uint32_t o4 = 4 * it.u;  // the quad's first word in the plane (and in a ring slot)
+              const uint32_t zr = zi0 + 2 * it.pi;
+              const uint32_t cw = lbase + (zr + gofs) * LL + 4 * it.u;
               uint32_t w[4], wn[4];
-              quad_load(lds, pl + o4, w);
-              if (!RG) {
-                quad_load(lds, pl + o4 + (uint32_t)dzb, wn);
-              } else {
-                uint64_t v[4];
+              quad_load(lds, cw, w);
+              quad_load(lds, cw + (uint32_t)dz, wn);
+              const U4 rq = draw_site(x + L * y + LL * (z0 + zr), pfz);
+              sigma_quad_regs(w, wn, thr, arm, rq);
+              quad_store(lds, cw, w);
+            }
+            if (bactive) {
+              // the boundary plane, as plane nint of the same walk
+              const uint32_t pl = lbase + (zb + gofs) * LL;
+              bool ok = true;
+              for (bool pf = true; it.pi == nint; advance(it, Q4, dp4, du4), pf = false) {
+                const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
+                const uint32_t o4 = 4 * it.u;  // the quad's first word in the plane (and in a ring slot)
+                uint32_t w[4], wn[4];
+                quad_load(lds, pl + o4, w);
+                if (!RG) {
+                  quad_load(lds, pl + o4 + (uint32_t)dzb, wn);
+                } else {
+                  uint64_t v[4];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) v[m] = pf ? pre[m] : ring_load64(RG + o4 + m);
+                  for (int m = 0; m < 4; ++m) v[m] = pf ? pre[m] : ring_load64(RG + o4 + m);
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                  ok = ok && wait_tag(RG + o4 + m, tag, v[m], status);
-                  wn[m] = (uint32_t)v[m];
+                  for (int m = 0; m < 4; ++m) {
+                    ok = ok && wait_tag(RG + o4 + m, tag, v[m], status);
+                    wn[m] = (uint32_t)v[m];
+                  }
+                }
+                const U4 rq = draw_site(x + L * y + LL * (z0 + zb), pfz);
+                sigma_quad_regs(w, wn, thr, arm, rq);
+                quad_store(lds, pl + o4, w);
+                if (RW) {
+#pragma unroll
+                  for (int m = 0; m < 4; ++m) ring_store64(RW + o4 + m, tw | w[m]);
                 }
               }
-              const U4 rq = draw_site(x + L * y + LL * (z0 + zb), pfz);
-              sigma_quad_regs(w, wn, thr, arm, rq);
-              quad_store(lds, pl + o4, w);
-              if (RW) {
-#pragma unroll
-                for (int m = 0; m < 4; ++m) ring_store64(RW + o4 + m, tw | w[m]);
-              }
+              if (__ballot(!ok)) abort_s = 1;
             }
-            if (__ballot(!ok)) abort_s = 1;
           }
         }
       }
@@ -687,29 +801,33 @@ static void set_resident_attribute() {
   (void)hipGetDevice(&dev);
   const uint64_t bit = 1ull << (dev & 63);
   if (!(attr_set.load(std::memory_order_acquire) & bit)) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_resident), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)resident_lds_limit());
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_resident<false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)resident_lds_limit());
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_resident<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)resident_lds_limit());
     attr_set.fetch_or(bit, std::memory_order_acq_rel);
   }
 }
 
-hipError_t resident_occupancy(int* per_cu, uint32_t block, uint32_t lds_bytes) {
+hipError_t resident_occupancy(int* per_cu, uint32_t block, uint32_t lds_bytes, bool fine) {
   set_resident_attribute();
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_resident, (int)block, (size_t)lds_bytes);
+  return fine ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_resident<true>, (int)block, (size_t)lds_bytes)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_resident<false>, (int)block, (size_t)lds_bytes);
 }
 
 void launch_resident(hipStream_t st, const RGroupEnt* grp, uint32_t ngroups, uint32_t block, uint32_t lds_bytes,
                      const RSlab* slabs, const PointDev* pts, uint32_t* words, uint32_t* rings, uint64_t* ver,
-                     const uint64_t* stepp, uint32_t step_off, uint32_t nsteps, uint32_t* status) {
+                     const uint64_t* stepp, uint32_t step_off, uint32_t nsteps, uint32_t* status, bool fine) {
   set_resident_attribute();
   hipEvent_t t0, t1;
   get_launch_timing(&t0, &t1);
+  auto* k = fine ? k_resident<true> : k_resident<false>;
   if (t0)
-    hipExtLaunchKernelGGL(k_resident, dim3(ngroups), dim3(block), lds_bytes, st, t0, t1, 0, grp, slabs, pts, words,
-                          rings, ver, stepp, step_off, nsteps, status);
+    hipExtLaunchKernelGGL(k, dim3(ngroups), dim3(block), lds_bytes, st, t0, t1, 0, grp, slabs, pts, words, rings, ver,
+                          stepp, step_off, nsteps, status);
   else
-    hipLaunchKernelGGL(k_resident, dim3(ngroups), dim3(block), lds_bytes, st, grp, slabs, pts, words, rings, ver,
-                       stepp, step_off, nsteps, status);
+    hipLaunchKernelGGL(k, dim3(ngroups), dim3(block), lds_bytes, st, grp, slabs, pts, words, rings, ver, stepp,
+                       step_off, nsteps, status);
 }
 
 }  // namespace cvf
