@@ -392,6 +392,9 @@ def correlation_form(steps: int, warmup: int) -> dict:
     from cvfbulkwater_amd import FP64, VOLUME_HOST, Correlator, Params, Simulation
 
     sim = Simulation(Params(L=32, T=0.815, P=0.45, seed=44648604, precision=FP64, volume_mode=VOLUME_HOST))
+    if sim.tuning["fused"] == 2:
+        # as the executable does for recorded steps: the fused slab plan
+        sim.tuning = {"fused": 1, "slab_planes": 0, "block_chunk": 512, "graph_steps": 16}
     corr = Correlator()
     sim.step(warmup)
     sim.sync()

@@ -744,8 +744,17 @@ int build_resident(cvf_sim* s, Group& g) {
   if (!plan_resident(s, g, ncu * per_cu, wpg, cap, (uint32_t)std::max(0, s->tune.slab_planes), d)) return CVF_OK;
   if (!resident_plan_valid(s, g, d, wpg, cap)) return fail(CVF_EINVAL, "internal: the resident plan is inconsistent");
   int occ = 0;
-  bool fine = true;
-  for (const auto& r : d.slabs) fine = fine && r.T <= 2;
+  // the fine instantiation for plans of slabs of at most two planes of
+  // which one has a phase that takes the fine path (its in-plane quads, or
+  // the cube items of its smaller family, fill at most half of its lanes at
+  // one cell or cube each); otherwise the instantiation without those paths
+  bool thin = true, fine_phase = false;
+  for (const auto& r : d.slabs) {
+    thin = thin && r.T <= 2;
+    const uint64_t lanes = 64ull * r.nw, LL = (uint64_t)r.L * r.L;
+    fine_phase = fine_phase || 8 * (uint64_t)r.T * (LL / 8) <= lanes || 8 * ((r.T + 1) / 2) * (LL / 16) <= lanes;
+  }
+  const bool fine = thin && fine_phase;
   HIPC(resident_occupancy(&occ, (uint32_t)s->tune.block_chunk, d.lds_bytes, fine));
   if (occ < per_cu) return CVF_OK;
   int rc;
@@ -1261,6 +1270,20 @@ cvf_tuning default_tuning(const cvf_sim* s) {
     t.fused = 2;
     t.slab_planes = 1;
     t.block_chunk = 1024;
+  }
+  // a small handle of one Metropolis sweep per step, nothing between the
+  // steps, whose planes can each have a workgroup: one-plane resident slabs
+  // in 64-step graphs (one point at L = 16 / 32 / 48 / 64: 14.3 / 17.5 /
+  // 24.4 / 26.6 us per step, against 18.4 / 20.9 / 32.4 / 38.2 for
+  // whole-pass fused launches)
+  uint64_t planes = 0;
+  for (const auto& P : s->ph) planes += P.L;
+  if (!any_nc && s->sched == CVF_SCHED_REFERENCE && s->nm == 1 && s->nc == 0 && planes <= 256 &&
+      s->total_cells < (1u << 20)) {
+    t.fused = 2;
+    t.slab_planes = 1;
+    t.block_chunk = 1024;
+    t.graph_steps = 64;
   }
   return t;
 }

@@ -237,6 +237,20 @@ int main(int argc, char** argv) {
     exit(1);
   }
   if (in.flag_chessboard == 1) fprintf(log, "Starting Chessboard\n");
+  if (in.flag_correlation == 1) {
+    // every step is recorded (a histogram after each): the resident kernel
+    // would run one launch per step, so the fused slab plan runs instead
+    // (L = 32: 36.5 against 40.2 us per step)
+    cvf_tuning t;
+    check(cvf_get_tuning(sim, &t), "tuning");
+    if (t.fused == 2) {
+      t.fused = 1;
+      t.slab_planes = 0;
+      t.block_chunk = 512;
+      t.graph_steps = 16;
+      check(cvf_set_tuning(sim, &t), "tuning");
+    }
+  }
 
   if (in.flag_config == 1) {
     std::vector<uint8_t> s, e;

@@ -54,7 +54,7 @@ def test_headline_batch_equals_single_runs():
     assert sim.steps == 20
     for i, p in enumerate(pts):
         single = Simulation(p)
-        assert single.plan(0)["slab_planes"] == 1  # one-plane slabs, whole-pass launches
+        assert single.plan(0)["slab_planes"] == 1  # one-plane slabs (the default for one small point)
         single.step(20)
         a, b = sim.get_config(i), single.get_config()
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), f"point {i} (L={p.L})"
