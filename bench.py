@@ -397,6 +397,7 @@ def correlation_form(steps: int, warmup: int) -> dict:
         sim.tuning = {"fused": 1, "slab_planes": 0, "block_chunk": 512, "graph_steps": 16}
     corr = Correlator()
     sim.step(warmup)
+    sim.step_histograms(10)  # the recording graphs are captured here, not in the timed loop
     sim.sync()
     t0 = time.perf_counter()
     # as the drop-in runs it: the histograms of the steps between two samples

@@ -745,14 +745,15 @@ int build_resident(cvf_sim* s, Group& g) {
   if (!resident_plan_valid(s, g, d, wpg, cap)) return fail(CVF_EINVAL, "internal: the resident plan is inconsistent");
   int occ = 0;
   // the fine instantiation for plans of slabs of at most two planes of
-  // which one has a phase that takes the fine path (its in-plane quads, or
-  // the cube items of its smaller family, fill at most half of its lanes at
-  // one cell or cube each); otherwise the instantiation without those paths
+  // which one has a phase that takes the fine path (the cells of its
+  // in-plane quads, or the cubes of its smaller family's items, fit its
+  // lanes and at most 512 of them); otherwise the instantiation without
+  // those paths
   bool thin = true, fine_phase = false;
   for (const auto& r : d.slabs) {
     thin = thin && r.T <= 2;
-    const uint64_t lanes = 64ull * r.nw, LL = (uint64_t)r.L * r.L;
-    fine_phase = fine_phase || 8 * (uint64_t)r.T * (LL / 8) <= lanes || 8 * ((r.T + 1) / 2) * (LL / 16) <= lanes;
+    const uint64_t lanes = std::min<uint64_t>(64ull * r.nw, 512), LL = (uint64_t)r.L * r.L;
+    fine_phase = fine_phase || 4 * (uint64_t)r.T * (LL / 8) <= lanes || 4 * ((r.T + 1) / 2) * (LL / 16) <= lanes;
   }
   const bool fine = thin && fine_phase;
   HIPC(resident_occupancy(&occ, (uint32_t)s->tune.block_chunk, d.lds_bytes, fine));
@@ -1273,16 +1274,21 @@ cvf_tuning default_tuning(const cvf_sim* s) {
   }
   // a small handle of one Metropolis sweep per step, nothing between the
   // steps, whose planes can each have a workgroup: one-plane resident slabs
-  // in 64-step graphs (one point at L = 16 / 32 / 48 / 64: 14.3 / 17.5 /
-  // 24.4 / 26.6 us per step, against 18.4 / 20.9 / 32.4 / 38.2 for
-  // whole-pass fused launches)
+  // in 64-step graphs (one point at L = 16 / 32 / 48 / 64: 13.7 / 16.9 /
+  // 23.6 / 26.6 us per step, against 18.4 / 20.9 / 32.4 / 38.2 for
+  // whole-pass fused launches), 512-thread workgroups up to L = 32, whose
+  // in-plane phases then run one cell per lane on all eight waves
   uint64_t planes = 0;
-  for (const auto& P : s->ph) planes += P.L;
+  uint32_t maxL = 0;
+  for (const auto& P : s->ph) {
+    planes += P.L;
+    maxL = std::max(maxL, P.L);
+  }
   if (!any_nc && s->sched == CVF_SCHED_REFERENCE && s->nm == 1 && s->nc == 0 && planes <= 256 &&
       s->total_cells < (1u << 20)) {
     t.fused = 2;
     t.slab_planes = 1;
-    t.block_chunk = 1024;
+    t.block_chunk = maxL <= 32 ? 512 : 1024;
     t.graph_steps = 64;
   }
   return t;
@@ -2079,8 +2085,9 @@ extern "C" int cvf_autotune(cvf_sim* s, int32_t steps_per_trial, cvf_tuning* cho
         for (int gs : {16, 64})
           for (int slab : {0, 1}) {
             // resident kernel: one stream group, `streams` workgroups per CU;
-            // automatic slabs, or one-plane slabs (wide lattices) at 1024
-            if (slab && bc != 1024) continue;
+            // automatic slabs, or one-plane slabs (small or wide lattices)
+            // at 512 and 1024 threads
+            if (slab && bc == 256) continue;
             cvf_tuning t = base;
             t.streams = streams;
             t.fused = 2;

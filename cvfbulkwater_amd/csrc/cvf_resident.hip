@@ -151,6 +151,11 @@ __device__ __forceinline__ uint32_t sigma_cell(uint32_t w, uint32_t wn, const ui
   return sp != sa && ra < t ? nw : w;
 }
 
+// A phase runs fine (one cell or cube per lane) when its cells fit the
+// slab's lanes and take at most 512 of them, i.e. at most two waves per
+// SIMD: beyond that the longer per-quad chain on fewer waves is cheaper.
+__device__ __forceinline__ bool fine_fits(uint32_t cells, uint32_t lanes) { return cells <= lanes && cells <= 512; }
+
 // Word h of a Philox block (h lane-dependent).
 __device__ __forceinline__ uint32_t pick4(const U4& r, uint32_t h) {
   return h == 0 ? r.x : h == 1 ? r.y : h == 2 ? r.z : r.w;
@@ -254,11 +259,10 @@ __device__ __forceinline__ void sweep_inplane(uint32_t* lds, uint32_t li, uint32
   const int dn = up ? (int)st : -(int)st;                   // facing quad, no wrap
   const int dw = up ? -(int)((hL - 1) * st) : (int)((hL - 1) * st);  // facing quad across the edge
   const PhiloxFixed pf = phase_draws(seed, kStSigma, step, k);
-  if (FINE && 8 * T * Q8 <= dp8 * Q8 + du8) {
-    // fine: the quads' cells fill at most half of the slab's lanes at one
-    // cell each (at most two waves per SIMD), so lane li takes cell li mod
-    // 4 of quad li div 4: a thin slab's phase is one short dependent chain
-    // per lane instead of one long one
+  if (FINE && fine_fits(4 * T * Q8, dp8 * Q8 + du8)) {
+    // fine: lane li takes cell li mod 4 of quad li div 4, so that a thin
+    // slab's phase is one short dependent chain per lane instead of one
+    // long one
     const uint32_t h = li & 3u;
     const Walk it = walk_start(li >> 2, Q8, mP8);
     if (it.pi < T) {
@@ -551,9 +555,8 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
         const int first = zlo + (parity_of((int)z0 + zlo, L) == fam ? 0 : 1);
         const uint32_t npl = first <= (int)T - 1 ? (uint32_t)((int)T - 1 - first) / 2 + 1 : 0u;
         const PhiloxFixed pfe = phase_draws(seed, kStEta, step, 0);
-        if (FINE && 8 * npl * Qc <= dpc * Qc + duc) {
-          // fine: the items' cubes fill at most half of the slab's lanes at
-          // one cube each, so lane li takes cube li mod 4 of item li div 4
+        if (FINE && fine_fits(4 * npl * Qc, dpc * Qc + duc)) {
+          // fine: lane li takes cube li mod 4 of item li div 4
           const uint32_t c = li & 3u;
           const Walk it = walk_start(li >> 2, Qc, F(mQQ));
           if (it.pi < npl) {
@@ -638,7 +641,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
           const PhiloxFixed pfz = phase_draws(seed, kStSigma, step, k);
           // item u of a plane is the quad with origin (u mod L/2, u div L/2),
           // word 4u of the plane
-          if (FINE && 8 * npl * Q4 <= stride) {
+          if (FINE && fine_fits(4 * npl * Q4, stride)) {
             // fine: one cell per lane (cell li mod 4 of item li div 4), the
             // boundary plane as plane nint
             const uint32_t h = li & 3u;
