@@ -403,7 +403,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
   __shared__ uint64_t thr_e[kMaxRSlabs][9];
   __shared__ uint64_t etab[9];
   // the step's half-sweeps of each slab: arm | parity << 3 | rd << 4 |
-  // wr << 12 (rd, wr: ring slots, see below)
+  // wr << 8 (rd, wr: ring slots, see below)
   __shared__ uint32_t sch_s[kMaxRSlabs][12];
   __shared__ uint32_t abort_s;
   const RGroupEnt G = grp[blockIdx.x];
@@ -469,8 +469,8 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     const uint64_t step = step0 + si;
     const uint32_t half = (uint32_t)(step & 1u) * kRingHalf;
     // the step's 12 half-sweeps (lane k: half-sweep k) and their ring slots:
-    // rd = 1 + the neighbour's publish to read (0: the ghost is current),
-    // wr = 1 + this slab's publish
+    // rd = 1 + the neighbour's half-sweep whose plane to read (0: the ghost
+    // is current), wr = 1 when this slab's plane goes to ring slot k
     if (has) {
       const uint32_t L = F(L), z0 = F(z0), T = F(T);
       const uint64_t seed = ((uint64_t)F(seed_hi) << 32) | F(seed_lo);
@@ -754,8 +754,9 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
       publish(ver + 2 * (G.first + gs) + 1, end_version(step));
     }
   }
-  // the neighbours have read their windows (they published after staging)
-  // before this workgroup overwrites its planes in `words`
+  // the neighbours have read their windows (they published their last
+  // step's version long after staging) before this workgroup overwrites its
+  // planes in `words`
   if (has && F(ns) > 1 && nsteps > 0 && lane == 0 && wave == F(w0)) {
     const uint64_t want = end_version(step0 + nsteps - 1);
     if (!wait_version(ver + 2 * F(above), want, status) || !wait_version(ver + 2 * F(below) + 1, want, status))
