@@ -474,38 +474,41 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     if (has) {
       const uint32_t L = F(L), z0 = F(z0), T = F(T);
       const uint64_t seed = ((uint64_t)F(seed_hi) << 32) | F(seed_lo);
-      uint32_t schv = 0;
-      if (lane < 12) {
-        const U4 r = draw(seed, kStSched, 0, step, lane);
-        schv = uniform_int(r.x, 6) | ((r.y >> 31) << 3);
+      // the slab's first wave draws them and keeps them for every wave of
+      // the slab
+      if (wave == F(w0)) {
+        uint32_t schv = 0;
+        if (lane < 12) {
+          const U4 r = draw(seed, kStSched, 0, step, lane);
+          schv = uniform_int(r.x, 6) | ((r.y >> 31) << 3);
+        }
+        const uint32_t arm = schv & 7u, q = (schv >> 3) ? 0u : 1u;
+        const bool in = lane < 12 && F(ns) > 1;
+        const uint32_t p_bot = parity_of((int)z0, L), p_top = parity_of((int)(z0 + T - 1), L);
+        const uint32_t p_abv = parity_of((int)(z0 + T), L), p_blw = parity_of((int)z0 - 1, L);
+        const uint64_t before = (1ull << lane) - 1;                    // lanes before this one
+        const uint64_t after = lane < 63 ? ~((2ull << lane) - 1) : 0ull;  // lanes after it
+        // the half-sweeps that move this slab's top (up) and bottom (dn)
+        // plane, and those that move the neighbours' facing planes: the
+        // slab above's bottom (ab), the slab below's top (bl)
+        const uint64_t up = __ballot(in && arm == 4 && q == p_top), dn = __ballot(in && arm == 5 && q == p_bot);
+        const uint64_t ab = __ballot(in && arm == 5 && q == p_abv), bl = __ballot(in && arm == 4 && q == p_blw);
+        // A half-sweep that moves a boundary plane reads the neighbour's
+        // plane as of the neighbour's latest such half-sweep (rd = its index
+        // + 1; 0: none yet, the ghost holds it), and stores its own moved
+        // plane into ring slot k, as tagged words (wr), only if the neighbour
+        // reads it: its next reading half-sweep comes before this plane
+        // moves again.
+        uint32_t rd = 0, wr = 0;
+        const bool mine_up = (up >> lane) & 1, mine_dn = (dn >> lane) & 1;
+        if (mine_up || mine_dn) {
+          const uint64_t other = mine_up ? ab : bl, self = mine_up ? up : dn;
+          if (other & before) rd = 64 - (uint32_t)__builtin_clzll(other & before);
+          const uint64_t ra = other & after, ua = self & after;
+          if (ra && (!ua || __builtin_ctzll(ra) < __builtin_ctzll(ua))) wr = 1;
+        }
+        if (lane < 12) sch_s[gs][lane] = schv | (rd << 4) | (wr << 8);
       }
-      const uint32_t arm = schv & 7u, q = (schv >> 3) ? 0u : 1u;
-      const bool in = lane < 12 && F(ns) > 1;
-      const uint32_t p_bot = parity_of((int)z0, L), p_top = parity_of((int)(z0 + T - 1), L);
-      const uint32_t p_abv = parity_of((int)(z0 + T), L), p_blw = parity_of((int)z0 - 1, L);
-      const uint64_t before = (1ull << lane) - 1;                    // lanes before this one
-      const uint64_t after = lane < 63 ? ~((2ull << lane) - 1) : 0ull;  // lanes after it
-      // the half-sweeps that move this slab's top (up) and bottom (dn)
-      // plane, and those that move the neighbours' facing planes: the
-      // slab above's bottom (ab), the slab below's top (bl)
-      const uint64_t up = __ballot(in && arm == 4 && q == p_top), dn = __ballot(in && arm == 5 && q == p_bot);
-      const uint64_t ab = __ballot(in && arm == 5 && q == p_abv), bl = __ballot(in && arm == 4 && q == p_blw);
-      // A half-sweep that moves a boundary plane reads the neighbour's
-      // plane as of the neighbour's latest such half-sweep (rd = its index
-      // + 1; 0: none yet, the ghost holds it), and stores its own moved
-      // plane into ring slot k, as tagged words (wr), only if the neighbour
-      // reads it: its next reading half-sweep comes before this plane
-      // moves again.
-      uint32_t rd = 0, wr = 0;
-      const bool mine_up = (up >> lane) & 1, mine_dn = (dn >> lane) & 1;
-      if (mine_up || mine_dn) {
-        const uint64_t other = mine_up ? ab : bl, self = mine_up ? up : dn;
-        if (other & before) rd = 64 - (uint32_t)__builtin_clzll(other & before);
-        const uint64_t ra = other & after, ua = self & after;
-        if (ra && (!ua || __builtin_ctzll(ra) < __builtin_ctzll(ua))) wr = 1;
-      }
-      // the slab's first wave keeps them for every wave of the slab
-      if (wave == F(w0) && lane < 12) sch_s[gs][lane] = schv | (rd << 4) | (wr << 8);
       if (si > 0 && F(ns) > 1) {
         // the ghosts: the neighbours' boundary planes at the end of the last
         // step, tagged words (a word whose tag is not yet the last step's is
