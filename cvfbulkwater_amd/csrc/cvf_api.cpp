@@ -100,6 +100,10 @@ struct Group {
   DevBuf<uint64_t> rver;
   uint32_t r_ngroups = 0, r_lds = 0;
   bool r_fine = false;  // every slab at most two planes: the fine instantiation
+  // a second stream beside `stream` for the device volume chain of a
+  // resident sequence, which never reads the lattice (fork / join events)
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   // captured step sequences, longest first: graph_steps, then the powers of
   // two below it, so any step count is a few graph launches
   std::vector<int> gsteps;
@@ -131,6 +135,9 @@ struct Group {
   ~Group() {
     drop_graphs();
     if (done) (void)hipEventDestroy(done);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -770,6 +777,11 @@ int build_resident(cvf_sim* s, Group& g) {
   g.r_ngroups = (uint32_t)d.groups.size();
   g.r_lds = d.lds_bytes;
   g.r_fine = fine;
+  if (!g.side) {
+    HIPC(hipStreamCreateWithFlags(&g.side, hipStreamNonBlocking));
+    HIPC(hipEventCreateWithFlags(&g.fork, hipEventDisableTiming));
+    HIPC(hipEventCreateWithFlags(&g.join, hipEventDisableTiming));
+  }
   g.resident = true;
   for (int p : g.points) s->ph[p].slab_T = 0;
   // the plan query reports a point's thickest slab
@@ -1131,6 +1143,24 @@ int enqueue_steps(cvf_sim* s, int gi, hipStream_t st, int nsteps, bool rec = fal
   }
   int par = 0;
   if (g.resident && s->nc == 0 && !rec) {
+    if (s->vol_device) {
+      // every step of the sequence in one resident launch, the volume
+      // chain of the same steps beside it (it never reads the lattice, and
+      // takes the volume state only from its own last run); the group
+      // counter moves on once both have read it
+      HIPC(hipEventRecord(g.fork, st));
+      HIPC(hipStreamWaitEvent(g.side, g.fork, 0));
+      launch_vol_chain(g.side, s->pts.p, g.plist.p, (int)g.points.size(), s->pst.p, nullptr,
+                       (uint32_t)nsteps_all, s->dist.p, s->freq.p, s->precision, 1);
+      HIPC(hipEventRecord(g.join, g.side));
+      launch_resident(st, g.rgroups.p, g.r_ngroups, (uint32_t)s->tune.block_chunk, g.r_lds, g.rslabs.p, s->pts.p,
+                      s->words.p, g.rings.p, g.rver.p, s->cur_steps.p + gi, 0, (uint32_t)nsteps, s->rstatus.p,
+                      g.r_fine);
+      HIPC(hipStreamWaitEvent(st, g.join, 0));
+      launch_vol_chain(st, s->pts.p, g.plist.p, 0, s->pst.p, s->cur_steps.p + gi, (uint32_t)nsteps_all, s->dist.p,
+                       s->freq.p, s->precision, 0);
+      return CVF_OK;
+    }
     // every step of the sequence in one resident launch
     launch_resident(st, g.rgroups.p, g.r_ngroups, (uint32_t)s->tune.block_chunk, g.r_lds, g.rslabs.p, s->pts.p,
                     s->words.p, g.rings.p, g.rver.p, s->cur_steps.p + gi, 0, (uint32_t)nsteps, s->rstatus.p, g.r_fine);

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(64) void k_vol_chain(const PointDev* __restrict__ p
                                                   const int32_t* __restrict__ freq, int precision,
                                                   int moves) {
   const int lane = threadIdx.x;
-  if (blockIdx.x == 0 && lane == 0) *gstep += nsteps;
+  if (blockIdx.x == 0 && lane == 0 && gstep) *gstep += nsteps;  // no counter: the moves alone
   if (!moves) return;
   const int p = plist[blockIdx.x];
   const PointDev& P = pts[p];
