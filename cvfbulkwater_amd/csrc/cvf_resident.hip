@@ -350,6 +350,9 @@ __device__ __forceinline__ void eta_cube_one_q(uint32_t* lds, uint32_t B, uint32
   eta_cube_move(w, thr, etab, rw, [&](int b, uint32_t v) { lds[idx[b]] = v; });
 }
 
+// EDGE: the item may reach across the periodic edge (family 1; a
+// family-0 item's origins are even, below L/2 - 1, so it never does).
+template <bool EDGE>
 __device__ __forceinline__ void eta_cube_item_q(uint32_t* lds, uint32_t B, uint32_t U, const uint64_t* thr,
                                                 const uint64_t* etab, uint32_t hL, uint32_t ox, uint32_t oy,
                                                 const U4& r) {
@@ -364,10 +367,12 @@ __device__ __forceinline__ void eta_cube_item_q(uint32_t* lds, uint32_t B, uint3
     quad_load(lds, qw[b], v[b]);
   }
   // half c of every corner quad for cube c
+  if (EDGE) {
 #pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    if (b & 1) swap_halves(v[b], sx, 2);
-    if (b & 2) swap_halves(v[b], sy, 1);
+    for (int b = 0; b < 8; ++b) {
+      if (b & 1) swap_halves(v[b], sx, 2);
+      if (b & 2) swap_halves(v[b], sy, 1);
+    }
   }
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -379,8 +384,8 @@ __device__ __forceinline__ void eta_cube_item_q(uint32_t* lds, uint32_t B, uint3
   }
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
-    if (b & 2) swap_halves(v[b], sy, 1);
-    if (b & 1) swap_halves(v[b], sx, 2);
+    if (EDGE && (b & 2)) swap_halves(v[b], sy, 1);
+    if (EDGE && (b & 1)) swap_halves(v[b], sx, 2);
     quad_store(lds, qw[b], v[b]);
   }
 }
@@ -586,7 +591,10 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
             const U4 r = draw_site(cube_ref, pfe);
             const uint32_t wb = (uint32_t)(zr + (int)gofs);
             const uint32_t wt = ns > 1 ? wb + 1 : (wb + 1 == T ? 0u : wb + 1);
-            eta_cube_item_q(lds, lbase + wb * LL, lbase + wt * LL, thre, etab, hL, ox, oy, r);
+            if (fam)
+              eta_cube_item_q<true>(lds, lbase + wb * LL, lbase + wt * LL, thre, etab, hL, ox, oy, r);
+            else
+              eta_cube_item_q<false>(lds, lbase + wb * LL, lbase + wt * LL, thre, etab, hL, ox, oy, r);
           }
         }
       }
