@@ -247,6 +247,46 @@ __device__ __forceinline__ void quad_store(uint32_t* lds, uint32_t w, const uint
 // one along the axis (the previous one for a - arm); across the periodic
 // edge it is the quad at the other end, with the halves along the axis
 // swapped.
+template <int AX, bool WRAP>
+__device__ __forceinline__ void inplane_items(uint32_t* lds, uint32_t li, uint32_t L, uint32_t T, uint32_t z0,
+                                              uint32_t pbase, int arm, uint32_t q, uint32_t edge, int dn, int dw,
+                                              const PhiloxFixed& pf, FastDiv dH, FastDiv dQ, uint32_t mP8,
+                                              uint32_t dp8, uint32_t du8, const uint32_t* thr) {
+  const uint32_t hL = L / 2, qL = L / 4, LL = L * L, Q8 = LL / 8;
+  for (Walk it = walk_start(li, Q8, mP8); it.pi < T; advance(it, Q8, dp8, du8)) {
+    uint32_t x, y, quad;
+    if (AX == 0) {
+      y = fdiv(it.u, dQ);
+      x = 2 * (it.u - y * qL) + q;
+      quad = 2 * it.u + q;
+    } else {
+      const uint32_t hy = fdiv(it.u, dH);
+      x = it.u - hy * hL;
+      y = 2 * hy + q;
+      quad = it.u + hL * (hy + q);
+    }
+    const bool wrap = WRAP && (AX == 0 ? x : y) == edge;
+    const uint32_t cw = pbase + it.pi * LL + 4 * quad;
+    const uint32_t nw = cw + 4u * (uint32_t)(wrap ? dw : dn);
+    uint32_t w[4], wn[4];
+    quad_load(lds, cw, w);
+    quad_load(lds, nw, wn);
+    if (WRAP) {
+      // swap the halves along the axis where the facing quad wraps (a bit
+      // select, so that the four selects stay four instructions)
+      const uint32_t sel = wrap ? ~0u : 0u;
+      uint32_t nv[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) nv[m] = wn[m];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) wn[m] = bitsel(sel, nv[m ^ (AX == 0 ? 2 : 1)], nv[m]);
+    }
+    const U4 rq = draw_site(x + L * y + LL * (z0 + it.pi), pf);
+    sigma_quad_regs(w, wn, thr, arm, rq);
+    quad_store(lds, cw, w);
+  }
+}
+
 template <int AX, bool FINE>
 __device__ __forceinline__ void sweep_inplane(uint32_t* lds, uint32_t li, uint32_t L, uint32_t T, uint32_t z0,
                                               uint32_t pbase, uint64_t seed, uint64_t step, uint32_t k, int arm,
@@ -286,33 +326,13 @@ __device__ __forceinline__ void sweep_inplane(uint32_t* lds, uint32_t li, uint32
     }
     return;
   }
-  for (Walk it = walk_start(li, Q8, mP8); it.pi < T; advance(it, Q8, dp8, du8)) {
-    uint32_t x, y, quad;
-    if (AX == 0) {
-      y = fdiv(it.u, dQ);
-      x = 2 * (it.u - y * qL) + q;
-      quad = 2 * it.u + q;
-    } else {
-      const uint32_t hy = fdiv(it.u, dH);
-      x = it.u - hy * hL;
-      y = 2 * hy + q;
-      quad = it.u + hL * (hy + q);
-    }
-    const bool wrap = (AX == 0 ? x : y) == edge;
-    const uint32_t cw = pbase + it.pi * LL + 4 * quad;
-    const uint32_t nw = cw + 4u * (uint32_t)(wrap ? dw : dn);
-    uint32_t w[4], nv[4], wn[4];
-    quad_load(lds, cw, w);
-    quad_load(lds, nw, nv);
-    // swap the halves along the axis where the facing quad wraps (a bit
-    // select, so that the four selects stay four instructions)
-    const uint32_t sel = wrap ? ~0u : 0u;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) wn[m] = bitsel(sel, nv[m ^ (AX == 0 ? 2 : 1)], nv[m]);
-    const U4 rq = draw_site(x + L * y + LL * (z0 + it.pi), pf);
-    sigma_quad_regs(w, wn, thr, arm, rq);
-    quad_store(lds, cw, w);
-  }
+  // L/2 is even, so the origins along the axis have parity q and the wrapping
+  // origin has parity 1 (+ arm, L/2 - 1) or 0 (- arm, 0): a phase of the
+  // other parity never reaches across the edge (a wave-uniform choice)
+  if (q == (up ? 1u : 0u))
+    inplane_items<AX, true>(lds, li, L, T, z0, pbase, arm, q, edge, dn, dw, pf, dH, dQ, mP8, dp8, du8, thr);
+  else
+    inplane_items<AX, false>(lds, li, L, T, z0, pbase, arm, q, edge, dn, dw, pf, dH, dQ, mP8, dp8, du8, thr);
 }
 
 // Cube moves on the four cubes of a cube item in the quad layout: origins
