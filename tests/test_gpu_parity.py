@@ -115,15 +115,35 @@ def test_cluster_row_exact_and_rounded_sums(T, P, rounded):
             assert (a.largest, a.second, a.n_perc, a.n_clusters, a.mean, a.var) == tuple(b), (t, sq)
 
 
-def test_config3_L32_labels_every_step():
-    """Config 3 shape: L=32, T=0.70, P=0.45, 1 Metropolis + 1 Swendsen-Wang
-    step per MC step (gpu_water3D.cu:1444-1603, labels 2440-2529): sigma,
-    eta and the 6N canonical labels identical to the oracle after each of
-    the first 1000 steps, the cluster.out row every 50 steps."""
+@pytest.fixture(scope="module")
+def config3_run():
+    """One engine and one oracle run of config 3 shared by its segments, and
+    the number of steps compared so far."""
     eng, ora = pair(32, T=0.70, nc=1)
-    for t in range(1000):
+    run = {"eng": eng, "ora": ora, "done": 0}
+    yield run
+    eng.close()
+
+
+CONFIG3_SEGMENT = 1000
+
+
+@pytest.mark.parametrize("segment", range(7))
+def test_config3_L32_labels_every_step(config3_run, segment):
+    """Config 3 over its whole schedule, 2000 equilibration + 5000 production
+    steps, in seven segments of 1000 steps on one run: L=32, T=0.70, P=0.45,
+    1 Metropolis + 1 Swendsen-Wang step per MC step (gpu_water3D.cu:1444-1603,
+    labels 2440-2529).  Sigma, eta and the 6N canonical labels identical to
+    the oracle after every step, the cluster.out row every 50 steps, the
+    observables and the volume chain at the end of each segment.  A segment
+    run on its own first steps (and compares) the segments before it."""
+    eng, ora = config3_run["eng"], config3_run["ora"]
+    end = (segment + 1) * CONFIG3_SEGMENT
+    while config3_run["done"] < end:
+        t = config3_run["done"]
         eng.step(1)
         ora.step(1)
+        config3_run["done"] = t + 1
         la, lb = eng.labels(), ora.labels()
         if not np.array_equal(la, lb):
             bad = np.flatnonzero(la != lb)
@@ -132,27 +152,30 @@ def test_config3_L32_labels_every_step():
         if t % 50 == 49:
             a, b = eng.cluster_stats(), ora.cluster_stats()
             assert (a.largest, a.second, a.n_perc, a.n_clusters, a.mean, a.var) == tuple(b), t
-    assert_same_obs(eng, ora, "end")
+    assert_same_obs(eng, ora, f"after {end} steps")
     assert eng.volume() == ora.volume()
 
 
 def test_config5_L96_labels_first_steps():
     """Config 5 shape: L=96, T=0.65, P=0.30, 1 Metropolis + 1 Swendsen-Wang
     + 1 volume move, FP32-compensated vdW: the lattice (sigma, eta) and the
-    5.3 M labels identical to the oracle after each of the first 3 steps
-    (the volume chain does not enter the lattice rules)."""
+    5.3 M labels identical to the oracle after each of the first 30 steps,
+    the cluster.out row every 10 (the volume chain does not enter the
+    lattice rules)."""
     kw = dict(L=96, T=0.65, P=0.30, seed=44648604, n_cluster=1)
     eng = Simulation(Params(precision=FP32_COMPENSATED, volume_mode=VOLUME_DEVICE, **kw))
     ora = O.OracleSim(**kw)
-    for t in range(3):
+    for t in range(30):
         eng.step(1)
         ora.step(1)
         assert np.array_equal(eng.labels(), ora.labels()), f"labels step {t}"
         assert_same_state(eng, ora, f"step {t}")
-    a, b = eng.cluster_stats(), ora.cluster_stats()
-    assert (a.largest, a.second, a.n_perc, a.n_clusters, a.mean, a.var) == tuple(b)
+        if t % 10 == 9:
+            a, b = eng.cluster_stats(), ora.cluster_stats()
+            assert (a.largest, a.second, a.n_perc, a.n_clusters, a.mean, a.var) == tuple(b), t
     a, b = eng.observe(), ora.measure()
     assert (a.nhb2, a.n_sigma) == (b.nhb2, b.n_sigma)
+    eng.close()
 
 
 def test_device_volume_chain_equals_host_chain():
