@@ -318,13 +318,14 @@ def test_profile_steps_matches_plain_steps(schedule, tuning, classes):
 
 
 def test_config5_precision_form_errors():
-    """L=96, T=0.65, P=0.30 with SW, FP32-compensated vdW against FP64
-    everywhere from the same seed: the lattice trajectory is identical (N_HB
-    error exactly 0); E, H, V and density agree within 1e-6 relative per
-    sample, and so do their long-run means."""
+    """L=96, T=0.65, P=0.30 with SW over the config's 5000 steps sampled
+    every 10, FP32-compensated vdW against FP64 everywhere from the same
+    seed: the lattice trajectory is identical (N_HB error exactly 0); E, H,
+    V and density agree within 1e-6 relative per sample, and so do their
+    long-run means."""
     import bench
 
-    res = bench.precision_form(steps=1000, interval=10)
+    res = bench.precision_form(steps=5000, interval=10)
     err = res["errors"]
     print(json.dumps(err))
     assert err["N_HB"]["max_rel"] == 0.0
