@@ -246,7 +246,8 @@ __device__ __forceinline__ void quad_store(uint32_t* lds, uint32_t w, const uint
 // 2u + q, resp. u + (L/2)(u div L/2 + q).  The facing quad is the next
 // one along the axis (the previous one for a - arm); across the periodic
 // edge it is the quad at the other end, with the halves along the axis
-// swapped.
+// swapped.  WRAP: the phase's parity holds the origins next to the edge
+// (without it no item wraps, and the test and the selects are left out).
 template <int AX, bool WRAP>
 __device__ __forceinline__ void inplane_items(uint32_t* lds, uint32_t li, uint32_t L, uint32_t T, uint32_t z0,
                                               uint32_t pbase, int arm, uint32_t q, uint32_t edge, int dn, int dw,
