@@ -109,19 +109,35 @@ __device__ bool wait_version(const uint64_t* v, uint64_t want, uint32_t* status)
   }
 }
 
-// Load *p again until its tag (high half) is `tag`, starting from the
-// value x already loaded.  False on time-out or abort (x is then stale).
-__device__ __forceinline__ bool wait_tag(const uint64_t* p, uint32_t tag, uint64_t& x, uint32_t* status) {
-  for (uint32_t it = 0; (uint32_t)(x >> 32) != tag; ++it) {
+// Load each of a lane's N words p[m] again until its tag (high half) is
+// `tag`, starting from the values x[m] already loaded, in one loop of the
+// whole wave: every pass checks all the words of every active lane, leaves
+// when none is stale and reloads only the stale ones.  So every load is
+// consumed on every path, and the compiler needs no wait for a load that
+// might still be in flight later (on this GPU loads and stores share one
+// counter, and such a wait would also wait out the ring stores).  False
+// on time-out or abort (then some x[m] are stale).
+template <int N>
+__device__ __forceinline__ bool wait_tags(const uint64_t* const (&p)[N], uint32_t tag, uint64_t (&x)[N],
+                                          uint32_t* status) {
+  for (uint32_t it = 0;; ++it) {
+    bool stale[N], any = false;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      stale[m] = (uint32_t)(x[m] >> 32) != tag;
+      any = any || stale[m];
+    }
+    if (!__ballot(any)) return true;
     if (((it & 63u) == 63u && __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ||
         it >= kSpinLimit) {
       __hip_atomic_fetch_or(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
-    x = ring_load64(p);
+#pragma unroll
+    for (int m = 0; m < N; ++m)
+      if (stale[m]) x[m] = ring_load64(p[m]);
   }
-  return true;
 }
 
 // Store a version after the workgroup barrier that follows the slot's ring
@@ -550,19 +566,18 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
         bool ok = true;
         for (uint32_t i0 = li; i0 < n2; i0 += 8 * stride) {
           uint64_t v[8];
+          const uint64_t* src[8];
 #pragma unroll
           for (uint32_t u = 0; u < 8; ++u) {
             const uint32_t i = min(i0 + u * stride, n2 - 1);
-            v[u] = ring_load64(i < LL ? hi + i : lo + (i - LL));
+            src[u] = i < LL ? hi + i : lo + (i - LL);
+            v[u] = ring_load64(src[u]);
           }
+          ok = wait_tags(src, tag, v, status) && ok;
 #pragma unroll
           for (uint32_t u = 0; u < 8; ++u) {
             const uint32_t i = i0 + u * stride;
-            if (i < n2) {
-              const uint64_t* src = i < LL ? hi + i : lo + (i - LL);
-              ok = ok && wait_tag(src, tag, v[u], status);
-              D[i < LL ? (T + 1) * LL + i : i - LL] = (uint32_t)v[u];
-            }
+            if (i < n2) D[i < LL ? (T + 1) * LL + i : i - LL] = (uint32_t)v[u];
           }
         }
         if (__ballot(!ok)) abort_s = 1;
@@ -687,9 +702,10 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
               const uint32_t w = lds[cw];
               uint32_t wn;
               if (bnd && RG) {
-                uint64_t v = ring_load64(RG + o);
-                ok = wait_tag(RG + o, tag, v, status);
-                wn = (uint32_t)v;
+                const uint64_t* const src[1] = {RG + o};
+                uint64_t v[1] = {ring_load64(src[0])};
+                ok = wait_tags(src, tag, v, status);
+                wn = (uint32_t)v[0];
               } else {
                 wn = lds[cw + (uint32_t)(bnd ? dzb : dz)];
               }
@@ -737,13 +753,12 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
                   quad_load(lds, pl + o4 + (uint32_t)dzb, wn);
                 } else {
                   uint64_t v[4];
+                  const uint64_t* const src[4] = {RG + o4, RG + o4 + 1, RG + o4 + 2, RG + o4 + 3};
 #pragma unroll
-                  for (int m = 0; m < 4; ++m) v[m] = pf ? pre[m] : ring_load64(RG + o4 + m);
+                  for (int m = 0; m < 4; ++m) v[m] = pf ? pre[m] : ring_load64(src[m]);
+                  ok = wait_tags(src, tag, v, status) && ok;
 #pragma unroll
-                  for (int m = 0; m < 4; ++m) {
-                    ok = ok && wait_tag(RG + o4 + m, tag, v[m], status);
-                    wn[m] = (uint32_t)v[m];
-                  }
+                  for (int m = 0; m < 4; ++m) wn[m] = (uint32_t)v[m];
                 }
                 const U4 rq = draw_site(x + L * y + LL * (z0 + zb), pfz);
                 sigma_quad_regs(w, wn, thr, arm, rq);
