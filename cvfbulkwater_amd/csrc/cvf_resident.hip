@@ -448,6 +448,9 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
   // wr << 8 (rd, wr: ring slots, see below)
   __shared__ uint32_t sch_s[kMaxRSlabs][12];
   __shared__ uint32_t abort_s;
+  // per slab: the slab below is one plane thick (its step-end plane then
+  // sits in the lower half of its ring only)
+  __shared__ uint32_t below1_s[kMaxRSlabs];
   const RGroupEnt G = grp[blockIdx.x];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = uni(tid >> 6);
   {
@@ -472,6 +475,7 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
     const PointDev& P = pts[F(point)];
     if (lane < 33) thr_s[gs][lane] = sigma_thr32(P.thr_sigma[lane]);
     if (lane < 9) thr_e[gs][lane] = P.thr_eta[lane];
+    if (lane == 0) below1_s[gs] = F(ns) > 1 && slabs[S.below].T == 1 ? 1u : 0u;
   }
   if (has) {
     // the slab's window, staged by its waves one quad (16 bytes of LDS,
@@ -558,7 +562,10 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
         const uint32_t LL = L * L, tag = end_tag(step - 1);
         const uint32_t prev = (uint32_t)((step - 1) & 1u) * kRingHalf + kEndSlot;
         const uint64_t* hi = reinterpret_cast<const uint64_t*>(rings + F(ring_hi) + prev * LL);
-        const uint64_t* lo = reinterpret_cast<const uint64_t*>(rings + F(ring_lo) + (2 * kRingHalf + prev) * LL);
+        // the slab below's top plane; a one-plane slab stores its plane once,
+        // in the lower half
+        const uint32_t lo_half = uni(below1_s[gs]) ? 0u : 2 * kRingHalf;
+        const uint64_t* lo = reinterpret_cast<const uint64_t*>(rings + F(ring_lo) + (lo_half + prev) * LL);
         uint32_t* D = lds + F(lds);
         // eight loads in flight per lane (an index past the end re-reads
         // the last word and is not stored)
@@ -787,7 +794,10 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
       uint64_t* up = reinterpret_cast<uint64_t*>(rings + F(ring) + (2 * kRingHalf + half + kEndSlot) * LL);
       const uint32_t* bot = lds + F(lds) + LL;
       const uint32_t* top = lds + F(lds) + T * LL;
-      for (uint32_t i = li; i < 2 * LL; i += 64 * F(nw)) {
+      // a one-plane slab's bottom and top plane are one plane: stored once,
+      // in the lower half, where the slab above reads it too
+      const uint32_t nst = T == 1 ? LL : 2 * LL;
+      for (uint32_t i = li; i < nst; i += 64 * F(nw)) {
         if (i < LL)
           ring_store64(dn + i, tag | bot[i]);
         else
