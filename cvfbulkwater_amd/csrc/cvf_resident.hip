@@ -33,8 +33,9 @@
 //     and the same cube-keyed random numbers, so both copies agree.
 // A slab therefore publishes its bottom (top) plane after every -z (+z)
 // half-sweep in which that plane is active and a neighbour reads it before
-// it moves again, and both boundary planes at the end of every step, into
-// rings in global memory: slots (step & 1) * 32 + 2k, + 1 after half-sweep
+// it moves again, and both boundary planes at the end of every step (a
+// one-plane slab its plane once, in the lower half), into rings in global
+// memory: slots (step & 1) * 32 + 2k, + 1 after half-sweep
 // k, slots (step & 1) * 32 + 30, 31 at the step end.  Every ring word is 64
 // bits, a cell word and a tag of the step; a reader loads the words it
 // needs and loads a word again while its tag is older, so neither a
