@@ -752,7 +752,7 @@ int build_resident(cvf_sim* s, Group& g) {
   if (!resident_plan_valid(s, g, d, wpg, cap)) return fail(CVF_EINVAL, "internal: the resident plan is inconsistent");
   int occ = 0;
   // the fine instantiation for plans of slabs of at most two planes of
-  // which one has a phase that takes the fine path (the cells of its
+  // which one has a phase that takes a fine path (two lanes for each of its
   // in-plane quads, or the cubes of its smaller family's items, fit its
   // lanes and at most 512 of them); otherwise the instantiation without
   // those paths
@@ -760,7 +760,7 @@ int build_resident(cvf_sim* s, Group& g) {
   for (const auto& r : d.slabs) {
     thin = thin && r.T <= 2;
     const uint64_t lanes = std::min<uint64_t>(64ull * r.nw, 512), LL = (uint64_t)r.L * r.L;
-    fine_phase = fine_phase || 4 * (uint64_t)r.T * (LL / 8) <= lanes || 4 * ((r.T + 1) / 2) * (LL / 16) <= lanes;
+    fine_phase = fine_phase || 2 * (uint64_t)r.T * (LL / 8) <= lanes || 4 * ((r.T + 1) / 2) * (LL / 16) <= lanes;
   }
   const bool fine = thin && fine_phase;
   HIPC(resident_occupancy(&occ, (uint32_t)s->tune.block_chunk, d.lds_bytes, fine));

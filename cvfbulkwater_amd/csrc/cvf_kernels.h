@@ -169,9 +169,10 @@ uint32_t resident_lds_limit();
 // resident workgroups of `block` threads and `lds_bytes` of dynamic LDS that
 // fit on one CU at once
 hipError_t resident_occupancy(int* per_cu, uint32_t block, uint32_t lds_bytes, bool fine);
-// fine: the instantiation for plans of thin slabs (at most two planes), whose
-// phases that fill at most half of a slab's lanes run one cell (one cube)
-// per lane instead of one quad (one four-cube item)
+// fine: the instantiation for plans of thin slabs (at most two planes):
+// in-plane half-sweeps whose quads fit half of a slab's lanes run two cells
+// per lane, and cube families and z half-sweeps whose cubes (cells) fit
+// them run one per lane, instead of one quad (one four-cube item) per lane
 void launch_resident(hipStream_t st, const RGroupEnt* grp, uint32_t ngroups, uint32_t block, uint32_t lds_bytes,
                      const RSlab* slabs, const PointDev* pts, uint32_t* words, uint32_t* rings, uint64_t* ver,
                      const uint64_t* stepp, uint32_t step_off, uint32_t nsteps, uint32_t* status, bool fine);

@@ -317,12 +317,14 @@ __device__ __forceinline__ void sweep_inplane(uint32_t* lds, uint32_t li, uint32
   const int dn = up ? (int)st : -(int)st;                   // facing quad, no wrap
   const int dw = up ? -(int)((hL - 1) * st) : (int)((hL - 1) * st);  // facing quad across the edge
   const PhiloxFixed pf = phase_draws(seed, kStSigma, step, k);
-  if (FINE && fine_fits(4 * T * Q8, dp8 * Q8 + du8)) {
-    // fine: lane li takes cell li mod 4 of quad li div 4, so that a thin
-    // slab's phase is one short dependent chain per lane instead of one
-    // long one
-    const uint32_t h = li & 3u;
-    const Walk it = walk_start(li >> 2, Q8, mP8);
+  if (FINE && fine_fits(2 * T * Q8, dp8 * Q8 + du8)) {
+    // pairs: lane li takes cells h0 = 2 (li mod 2) and h0 + 1 of quad
+    // li div 2, so that a thin slab's phase is a short dependent chain per
+    // lane on at most one wave per SIMD.  The facing cells of the pair are
+    // the pair at h0 of the facing quad, across the edge the pair at h0 ^ 2
+    // (x arm) or the same pair in swapped order (y arm).
+    const uint32_t h0 = 2 * (li & 1u);
+    const Walk it = walk_start(li >> 1, Q8, mP8);
     if (it.pi < T) {
       uint32_t x, y, quad;
       if (AX == 0) {
@@ -336,11 +338,16 @@ __device__ __forceinline__ void sweep_inplane(uint32_t* lds, uint32_t li, uint32
         quad = it.u + hL * (hy + q);
       }
       const bool wrap = (AX == 0 ? x : y) == edge;
-      const uint32_t cw = pbase + it.pi * LL + 4 * quad;
-      const uint32_t nw = cw + 4u * (uint32_t)(wrap ? dw : dn) + (wrap ? h ^ (AX == 0 ? 2u : 1u) : h);
-      const uint32_t w = lds[cw + h], wn = lds[nw];
+      const uint32_t cw = pbase + it.pi * LL + 4 * quad + h0;
+      const uint32_t nw = cw - h0 + 4u * (uint32_t)(wrap ? dw : dn) + (AX == 0 && wrap ? h0 ^ 2u : h0);
+      const uint2 w = *reinterpret_cast<const uint2*>(lds + cw);
+      const uint2 n = *reinterpret_cast<const uint2*>(lds + nw);
+      const bool sw = AX == 1 && wrap;
       const U4 rq = draw_site(x + L * y + LL * (z0 + it.pi), pf);
-      lds[cw + h] = sigma_cell(w, wn, thr, arm, pick4(rq, h));
+      const uint32_t r0 = h0 ? rq.z : rq.x, r1 = h0 ? rq.w : rq.y;
+      const uint32_t a = sigma_cell(w.x, sw ? n.y : n.x, thr, arm, r0);
+      const uint32_t b = sigma_cell(w.y, sw ? n.x : n.y, thr, arm, r1);
+      *reinterpret_cast<uint2*>(lds + cw) = make_uint2(a, b);
     }
     return;
   }
