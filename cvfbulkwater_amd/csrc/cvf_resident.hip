@@ -320,7 +320,9 @@ __device__ __forceinline__ void sweep_inplane(uint32_t* lds, uint32_t li, uint32
   if (FINE && fine_fits(2 * T * Q8, dp8 * Q8 + du8)) {
     // pairs: lane li takes cells h0 = 2 (li mod 2) and h0 + 1 of quad
     // li div 2, so that a thin slab's phase is a short dependent chain per
-    // lane on at most one wave per SIMD.  The facing cells of the pair are
+    // lane on half the waves one cell per lane would take (four lanes
+    // recomputing a quad's Philox block cost more issue than a second cell's
+    // chain).  The facing cells of the pair are
     // the pair at h0 of the facing quad, across the edge the pair at h0 ^ 2
     // (x arm) or the same pair in swapped order (y arm).
     const uint32_t h0 = 2 * (li & 1u);
