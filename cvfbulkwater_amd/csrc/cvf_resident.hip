@@ -705,7 +705,42 @@ __global__ __launch_bounds__(1024) void k_resident(const RGroupEnt* __restrict__
           const PhiloxFixed pfz = phase_draws(seed, kStSigma, step, k);
           // item u of a plane is the quad with origin (u mod L/2, u div L/2),
           // word 4u of the plane
-          if (FINE && fine_fits(4 * npl * Q4, stride)) {
+          if (FINE && 4 * npl * Q4 > stride && fine_fits(2 * npl * Q4, stride)) {
+            // pairs: cells h0 = 2 (li mod 2) and h0 + 1 of item li div 2 (the
+            // facing cells are the same pair of the plane above or below),
+            // the boundary plane as plane nint
+            const uint32_t h0 = 2 * (li & 1u);
+            const Walk it = walk_start(li >> 1, Q4, F(mP4));
+            bool ok = true;
+            if (it.pi < npl) {
+              const bool bnd = bactive && it.pi == nint;
+              const uint32_t zr = bnd ? zb : zi0 + 2 * it.pi;
+              const uint32_t y = fdiv(it.u, dH), x = it.u - y * hL;
+              const uint32_t o = 4 * it.u + h0, cw = lbase + (zr + gofs) * LL + o;
+              const uint2 w = *reinterpret_cast<const uint2*>(lds + cw);
+              uint32_t n0, n1;
+              if (bnd && RG) {
+                const uint64_t* const src[2] = {RG + o, RG + o + 1};
+                uint64_t v[2] = {ring_load64(src[0]), ring_load64(src[1])};
+                ok = wait_tags(src, tag, v, status);
+                n0 = (uint32_t)v[0];
+                n1 = (uint32_t)v[1];
+              } else {
+                const uint2 n = *reinterpret_cast<const uint2*>(lds + cw + (uint32_t)(bnd ? dzb : dz));
+                n0 = n.x;
+                n1 = n.y;
+              }
+              const U4 rq = draw_site(x + L * y + LL * (z0 + zr), pfz);
+              const uint32_t a = sigma_cell(w.x, n0, thr, arm, h0 ? rq.z : rq.x);
+              const uint32_t b = sigma_cell(w.y, n1, thr, arm, h0 ? rq.w : rq.y);
+              *reinterpret_cast<uint2*>(lds + cw) = make_uint2(a, b);
+              if (bnd && RW) {
+                ring_store64(RW + o, tw | a);
+                ring_store64(RW + o + 1, tw | b);
+              }
+            }
+            if (__ballot(!ok)) abort_s = 1;
+          } else if (FINE && fine_fits(4 * npl * Q4, stride)) {
             // fine: one cell per lane (cell li mod 4 of item li div 4), the
             // boundary plane as plane nint
             const uint32_t h = li & 3u;
