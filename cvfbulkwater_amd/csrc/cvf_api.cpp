@@ -1306,8 +1306,10 @@ cvf_tuning default_tuning(const cvf_sim* s) {
   // steps, whose planes can each have a workgroup: one-plane resident slabs
   // in 64-step graphs (one point at L = 16 / 32 / 48 / 64: 13.7 / 16.9 /
   // 23.6 / 26.6 us per step, against 18.4 / 20.9 / 32.4 / 38.2 for
-  // whole-pass fused launches), 512-thread workgroups up to L = 32, whose
-  // in-plane phases then run one cell per lane on all eight waves
+  // whole-pass fused launches), 256-thread workgroups up to L = 16 and
+  // 512-thread ones up to L = 32, whose fine phases then fill their waves
+  // (L = 16: 13.74 us per step at 256 threads, 13.96 at 512; L = 24: 15.26
+  // at 512, 15.52 at 256; L = 32: 15.24 at 512, 15.64 at 256)
   uint64_t planes = 0;
   uint32_t maxL = 0;
   for (const auto& P : s->ph) {
@@ -1318,7 +1320,7 @@ cvf_tuning default_tuning(const cvf_sim* s) {
       s->total_cells < (1u << 20)) {
     t.fused = 2;
     t.slab_planes = 1;
-    t.block_chunk = maxL <= 32 ? 512 : 1024;
+    t.block_chunk = maxL <= 16 ? 256 : maxL <= 32 ? 512 : 1024;
     t.graph_steps = 64;
   }
   return t;
