@@ -171,7 +171,7 @@ uint32_t resident_lds_limit();
 hipError_t resident_occupancy(int* per_cu, uint32_t block, uint32_t lds_bytes, bool fine);
 // fine: the instantiation for plans of thin slabs (at most two planes):
 // in-plane half-sweeps whose quads fit half of a slab's lanes run two cells
-// per lane, cube families whose cubes fit the lanes one cube per lane, and
+// per lane (one, below 64 quads), cube families whose cubes fit the lanes one cube per lane, and
 // z half-sweeps one cell per lane where that fits, else two where that
 // does, instead of one quad (one four-cube item) per lane
 void launch_resident(hipStream_t st, const RGroupEnt* grp, uint32_t ngroups, uint32_t block, uint32_t lds_bytes,

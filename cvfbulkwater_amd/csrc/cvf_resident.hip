@@ -317,7 +317,34 @@ __device__ __forceinline__ void sweep_inplane(uint32_t* lds, uint32_t li, uint32
   const int dn = up ? (int)st : -(int)st;                   // facing quad, no wrap
   const int dw = up ? -(int)((hL - 1) * st) : (int)((hL - 1) * st);  // facing quad across the edge
   const PhiloxFixed pf = phase_draws(seed, kStSigma, step, k);
-  if (FINE && fine_fits(2 * T * Q8, dp8 * Q8 + du8)) {
+  const uint32_t lanes = dp8 * Q8 + du8;
+  if (FINE && fine_fits(4 * T * Q8, lanes) && 2 * T * Q8 < 128) {
+    // a phase of fewer than 64 quads: one cell per lane (cell li mod 4 of
+    // quad li div 4), so that it still spreads over two waves
+    const uint32_t h = li & 3u;
+    const Walk it = walk_start(li >> 2, Q8, mP8);
+    if (it.pi < T) {
+      uint32_t x, y, quad;
+      if (AX == 0) {
+        y = fdiv(it.u, dQ);
+        x = 2 * (it.u - y * qL) + q;
+        quad = 2 * it.u + q;
+      } else {
+        const uint32_t hy = fdiv(it.u, dH);
+        x = it.u - hy * hL;
+        y = 2 * hy + q;
+        quad = it.u + hL * (hy + q);
+      }
+      const bool wrap = (AX == 0 ? x : y) == edge;
+      const uint32_t cw = pbase + it.pi * LL + 4 * quad;
+      const uint32_t nw = cw + 4u * (uint32_t)(wrap ? dw : dn) + (wrap ? h ^ (AX == 0 ? 2u : 1u) : h);
+      const uint32_t w = lds[cw + h], wn = lds[nw];
+      const U4 rq = draw_site(x + L * y + LL * (z0 + it.pi), pf);
+      lds[cw + h] = sigma_cell(w, wn, thr, arm, pick4(rq, h));
+    }
+    return;
+  }
+  if (FINE && fine_fits(2 * T * Q8, lanes)) {
     // pairs: lane li takes cells h0 = 2 (li mod 2) and h0 + 1 of quad
     // li div 2, so that a thin slab's phase is a short dependent chain per
     // lane on half the waves one cell per lane would take (four lanes
